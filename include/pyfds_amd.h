@@ -1,0 +1,156 @@
+/*
+ * pyfds_amd C-ABI: MI355X (gfx950) time stepping for pyfds' staggered-grid solvers.
+ *
+ * The reference (emtpb/pyfds) is pure Python and has no FFI; its drop-in boundary is the
+ * Python class protocol.  Each entry point below replaces one piece of that protocol,
+ * cited as pyfds/<file>:<line>.  Conventions:
+ *   - every call returns 0 on success or a negative FDS_E_* code; fds_last_error() gives a
+ *     thread-local message for the last failure on the calling thread;
+ *   - the library owns all device memory; host pointers are borrowed for the call only;
+ *   - one context per simulation; a context has its own HIP stream and is used from one host
+ *     thread at a time;
+ *   - all host-side arrays are float64 and flat, index i = ix + nx*iy (pyfds/fields.py:377).
+ */
+#ifndef PYFDS_AMD_H
+#define PYFDS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fds_ctx fds_ctx;
+
+/* models (pyfds/acoustics.py, pyfds/thermal.py, pyfds/nonlinear_acoustics.py) */
+enum {
+  FDS_ACOUSTIC1D = 1,     /* Acoustic1D         pyfds/acoustics.py:9-52          */
+  FDS_ACOUSTIC2D = 2,     /* Acoustic2D         pyfds/acoustics.py:66-128        */
+  FDS_ACOUSTIC_AXI = 3,   /* Acoustic3DAxi      pyfds/acoustics.py:142-225       */
+  FDS_THERMAL1D = 4,      /* Thermal1D          pyfds/thermal.py:12-51           */
+  FDS_THERMAL2D = 5,      /* Thermal2D          pyfds/thermal.py:54-107          */
+  FDS_THERMAL_AXI = 6,    /* Thermal3DAxi       pyfds/thermal.py:110-176         */
+  FDS_IDEALGAS1D = 7,     /* IdealGas1D         pyfds/nonlinear_acoustics.py:10-89   */
+  FDS_ACOUSTIC2ND1D = 8   /* Acoustic2ndOrder1D pyfds/nonlinear_acoustics.py:103-171 */
+};
+
+/* numeric forms */
+enum {
+  FDS_FP64 = 0,        /* reproduces the reference's float64 arithmetic term by term        */
+  FDS_FP32 = 1,        /* float32 state and coefficients                                    */
+  FDS_FP32_KAHAN = 2   /* float32 state + float32 compensation (Kahan) for accumulated fields */
+};
+
+/* error codes */
+enum {
+  FDS_OK = 0,
+  FDS_E_ARG = -1,       /* bad argument (Python: ValueError)                          */
+  FDS_E_HIP = -2,       /* HIP runtime error                                          */
+  FDS_E_INDEX = -3,     /* index or signal step out of range (Python: IndexError)     */
+  FDS_E_STATE = -4,     /* call out of order / unsupported configuration              */
+  FDS_E_NONFINITE = -5  /* non-finite value detected                                  */
+};
+
+/* coefficient slots (fds_set_coeff); values are computed on the host with the reference's
+ * expression order (pyfds/acoustics.py:30-37, 92-107, 181-201; pyfds/thermal.py:32-37,
+ * 78-89, 147-158; pyfds/nonlinear_acoustics.py:252-263, 347-357) */
+enum {
+  /* acoustics 1D/2D/Axi */
+  FDS_C_KX = 0, FDS_C_KY = 1,     /* dt/d * c**2 * rho               (a_p_v*)            */
+  FDS_C_BX = 2, FDS_C_BY = 3,     /* dt/d / rho                      (a_v*_p)            */
+  FDS_C_MX = 4, FDS_C_MY = 5,     /* dt/d**2 * mu / rho              (a_v*_v*)           */
+  FDS_C_MC = 6,                   /* Axi: dt/dx * mu / rho  (divided by r in-kernel)     */
+  FDS_C_MT = 7,                   /* Axi: dt * mu / rho     (per-step vx/r**2 term)      */
+  /* thermal 1D/2D/Axi */
+  FDS_C_GX = 0, FDS_C_GY = 1,     /* dt/d / rho / c_p                (a_t_q*)            */
+  FDS_C_QX = 2, FDS_C_QY = 3,     /* 1/d * k                         (a_q*_t)            */
+  /* nonlinear 1D */
+  FDS_C_RHO0 = 0,                 /* static density                                      */
+  FDS_C_M = 1,                    /* dt/dx**2 * mu                   (a_v_v)             */
+  FDS_C_EOS1 = 2,                 /* IdealGas1D: rho0 * c**2 / gamma; 2nd order: d_rho_p */
+  FDS_C_EOS2 = 3,                 /* IdealGas1D: gamma;              2nd order: d_rho2_p / 2 */
+  FDS_C_CSQ = 4,                  /* IdealGas1D linear state: c**2                       */
+  FDS_NUM_COEFF = 8
+};
+
+/* scalars (fds_set_scalars, in this order) */
+enum {
+  FDS_S_DX = 0,       /* x increment (Axi radius r = ix*dx + dx/2, pyfds/acoustics.py:176) */
+  FDS_S_HALF_DX = 1,  /* dx / 2                                                            */
+  FDS_S_DTDX = 2,     /* nonlinear 1D: dt/dx          (a_d_v, a_v_p)                       */
+  FDS_S_DTDX_4 = 3,   /* nonlinear 1D: dt/dx/2/2      (a_v_v2, central)                    */
+  FDS_S_FLAGS = 4,    /* nonlinear 1D: bit0 convective, bit1 nonlinear state               */
+  FDS_NUM_SCALARS = 8
+};
+
+/* Create a context for one field of nx*ny cells (ny = 1 for 1D).  Replaces the field
+ * construction + FieldComponent allocation of pyfds/fields.py:133-150, 247-267, 577-589. */
+int fds_create(fds_ctx **out, int model, int form, int64_t nx, int64_t ny, int device);
+
+/* Release every device resource of the context. */
+int fds_destroy(fds_ctx *ctx);
+
+/* Per-context scalars, see FDS_S_*.  Part of assemble_matrices (pyfds/acoustics.py:27-38). */
+int fds_set_scalars(fds_ctx *ctx, const double *s, int n);
+
+/* Upload one per-cell coefficient array (n = nx*ny) into slot FDS_C_*.  n = 0 clears the slot.
+ * Replaces the sparse-matrix assembly of assemble_matrices (pyfds/fields.py:158-365 operators,
+ * pyfds/acoustics.py:89-109, pyfds/thermal.py:75-90).  Must be followed by fds_commit. */
+int fds_set_coeff(fds_ctx *ctx, int slot, const double *data, int64_t n);
+
+/* Finish a coefficient update: picks the kernel variant (lossless/lossy, x/y aliasing). */
+int fds_commit(fds_ctx *ctx);
+
+/* Copy a component's values to / from the device (FieldComponent.values,
+ * pyfds/fields.py:585).  comp follows the model's component order
+ * (pressure, velocity[_x], velocity_y | temperature, heat_flux[_x], heat_flux_y |
+ *  pressure, velocity, density). */
+int fds_set_state(fds_ctx *ctx, int comp, const double *data, int64_t n);
+int fds_get_state(fds_ctx *ctx, int comp, double *data, int64_t n);
+
+/* Boundary program of one component (FieldComponent.boundaries applied by apply_bounds,
+ * pyfds/fields.py:591-600, Boundary.apply pyfds/regions.py:125-145).  The host compiles the
+ * boundary list into ncell unique cells; cell u runs entries ptr[u]..ptr[u+1]-1 in list order,
+ * each  x = a[e]*x + table[off[e] + step*stride[e]]  (stride 0: scalar value, 1: signal).
+ * len[e] is the number of valid steps of entry e (INT64_MAX for scalars).  ncell = 0 clears. */
+int fds_set_boundaries(fds_ctx *ctx, int comp, int64_t ncell, const int64_t *cells,
+                       const int32_t *ptr, int64_t nent, const double *a, const int64_t *off,
+                       const int64_t *stride, const int64_t *len, int64_t ntable,
+                       const double *table);
+
+/* Recorded points of one component (FieldComponent.write_outputs, pyfds/fields.py:602-611):
+ * m slots, slot k records cells[k] every step after that component's boundaries. */
+int fds_set_outputs(fds_ctx *ctx, int comp, int64_t m, const int64_t *cells);
+
+/* Advance nsteps steps starting at pyfds step counter start_step (pyfds/fields.py:87-93 loop
+ * with each model's sim_step).  chunk > 0 replays HIP-graph captured chunks of that many
+ * steps; chunk = 0 launches step by step.  Output rows 0..nsteps-1 of this run are kept on the
+ * device until the next run.  Synchronous. */
+int fds_run(fds_ctx *ctx, int64_t start_step, int64_t nsteps, int chunk);
+
+/* GPU time in milliseconds between HIP events recorded on the context's stream before the
+ * first and after the last step launch of the last fds_run (no reference counterpart: the
+ * measurement hook for benchmarks and the autotuner). */
+int fds_last_run_ms(fds_ctx *ctx, double *ms);
+
+/* Read rows [row0, row0+nrows) of a component's recorded outputs from the last run into
+ * dst[nrows][m] (row-major, one row per step). */
+int fds_read_outputs(fds_ctx *ctx, int comp, int64_t row0, int64_t nrows, double *dst);
+
+/* Tile shape of the 2D step kernels (autotuner hook): index into the compiled tile table,
+ * or -1 for the default.  fds_tile_count reports the table size, fds_tile_shape its entries. */
+int fds_set_tile(fds_ctx *ctx, int tile_index);
+int fds_tile_count(void);
+int fds_tile_shape(int tile_index, int *tx, int *ty);
+
+/* Last error message of the calling thread ("" if none). */
+const char *fds_last_error(void);
+
+/* Library version string. */
+const char *fds_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYFDS_AMD_H */

@@ -1,0 +1,121 @@
+"""ctypes binding of libpyfds_amd.so (declared in include/pyfds_amd.h).
+
+The library is the only compute path of pyfds_amd: if it is missing or cannot be loaded,
+every simulation call raises :class:`NativeError` -- there is no CPU fallback.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, '_lib', 'libpyfds_amd.so')
+
+# models / forms / codes, mirroring include/pyfds_amd.h
+ACOUSTIC1D, ACOUSTIC2D, ACOUSTIC_AXI = 1, 2, 3
+THERMAL1D, THERMAL2D, THERMAL_AXI = 4, 5, 6
+IDEALGAS1D, ACOUSTIC2ND1D = 7, 8
+FORMS = {'fp64': 0, 'fp32': 1, 'fp32_kahan': 2}
+E_ARG, E_HIP, E_INDEX, E_STATE, E_NONFINITE = -1, -2, -3, -4, -5
+NUM_COEFF, NUM_SCALARS = 8, 8
+
+EXPORTS = ('fds_create', 'fds_destroy', 'fds_set_scalars', 'fds_set_coeff', 'fds_commit',
+           'fds_set_state', 'fds_get_state', 'fds_set_boundaries', 'fds_set_outputs',
+           'fds_run', 'fds_read_outputs', 'fds_set_tile', 'fds_tile_count', 'fds_tile_shape',
+           'fds_last_run_ms', 'fds_last_error', 'fds_version')
+
+
+class NativeError(RuntimeError):
+    """The HIP library is missing, failed to load, or a device call failed."""
+
+
+_lib = None
+
+_p = ct.c_void_p
+_i64 = ct.c_int64
+_i32 = ct.c_int32
+_dp = ct.POINTER(ct.c_double)
+_lp = ct.POINTER(ct.c_int64)
+_ip = ct.POINTER(ct.c_int32)
+
+
+def load():
+    """Load and type the library once; raise NativeError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeError('pyfds_amd HIP library not built (%s missing); run '
+                          '`python -m pyfds_amd.build`' % LIB_PATH)
+    try:
+        lib = ct.CDLL(LIB_PATH)
+    except OSError as exc:
+        raise NativeError('cannot load %s: %s' % (LIB_PATH, exc)) from exc
+    sig = {
+        'fds_create': (ct.c_int, [ct.POINTER(_p), ct.c_int, ct.c_int, _i64, _i64, ct.c_int]),
+        'fds_destroy': (ct.c_int, [_p]),
+        'fds_set_scalars': (ct.c_int, [_p, _dp, ct.c_int]),
+        'fds_set_coeff': (ct.c_int, [_p, ct.c_int, _dp, _i64]),
+        'fds_commit': (ct.c_int, [_p]),
+        'fds_set_state': (ct.c_int, [_p, ct.c_int, _dp, _i64]),
+        'fds_get_state': (ct.c_int, [_p, ct.c_int, _dp, _i64]),
+        'fds_set_boundaries': (ct.c_int, [_p, ct.c_int, _i64, _lp, _ip, _i64, _dp, _lp, _lp,
+                                          _lp, _i64, _dp]),
+        'fds_set_outputs': (ct.c_int, [_p, ct.c_int, _i64, _lp]),
+        'fds_run': (ct.c_int, [_p, _i64, _i64, ct.c_int]),
+        'fds_read_outputs': (ct.c_int, [_p, ct.c_int, _i64, _i64, _dp]),
+        'fds_set_tile': (ct.c_int, [_p, ct.c_int]),
+        'fds_last_run_ms': (ct.c_int, [_p, _dp]),
+        'fds_tile_count': (ct.c_int, []),
+        'fds_tile_shape': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
+        'fds_last_error': (ct.c_char_p, []),
+        'fds_version': (ct.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    """Turn a negative return code into the reference's exception types."""
+    if rc >= 0:
+        return rc
+    msg = load().fds_last_error().decode(errors='replace')
+    if rc == E_INDEX:
+        raise IndexError(msg)
+    if rc == E_ARG:
+        raise ValueError(msg)
+    raise NativeError('pyfds_amd device call failed (%d): %s' % (rc, msg))
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def lptr(a):
+    return a.ctypes.data_as(_lp)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def tile_shapes():
+    lib = load()
+    out = []
+    for i in range(lib.fds_tile_count()):
+        tx, ty = ct.c_int(), ct.c_int()
+        check(lib.fds_tile_shape(i, ct.byref(tx), ct.byref(ty)))
+        out.append((tx.value, ty.value))
+    return out

@@ -1,0 +1,791 @@
+// pyfds_amd C-ABI: contexts, uploads, per-tile event lists, the step loop and HIP graphs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pyfds_amd.h"
+#include "fds_common.h"
+#include "fds_kernels.h"
+#include "fds_step2d.h"
+#include "fds_step_nl.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t err_ = (expr);                                                                \
+    if (err_ != hipSuccess)                                                                  \
+      return fail(FDS_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(err_),       \
+                  __FILE__, __LINE__);                                                       \
+  } while (0)
+
+// Owned device allocation.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t ensure(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    release();
+    if (b == 0) b = 16;
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  template <typename U>
+  U *as() const { return static_cast<U *>(p); }
+};
+
+template <typename U>
+hipError_t upload(DevBuf &d, const std::vector<U> &h, hipStream_t s) {
+  hipError_t e = d.ensure(std::max<size_t>(h.size() * sizeof(U), 16));
+  if (e != hipSuccess) return e;
+  if (h.empty()) return hipSuccess;
+  e = hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(U), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
+
+int64_t floordiv(int64_t a, int64_t b) {
+  int64_t q = a / b;
+  if ((a % b != 0) && ((a < 0) != (b < 0))) --q;
+  return q;
+}
+
+int64_t ceildiv(int64_t a, int64_t b) { return -floordiv(-a, b); }
+
+// Where a component's values live inside a tile's LDS arrays: rows [rmin, rmax] x cols
+// [cmin, cmax] of the tile (relative to its origin), stored row-major with row width w.
+struct Region {
+  int rmin, rmax, cmin, cmax, w;
+  int pos(int r, int c) const { return (r - rmin) * w + (c - cmin); }
+};
+
+struct Events {
+  std::vector<int32_t> ptr, pos, id;
+};
+
+struct Comp {
+  bool present = false;
+  bool accumulated = false;   // updated with -= (Kahan compensation applies)
+  DevBuf buf[2], cbuf[2];
+  // boundary program (host copy kept for validation and event building)
+  std::vector<int64_t> cells;
+  std::vector<int32_t> ptr;
+  std::vector<double> a, table;
+  std::vector<int64_t> off, stride, len;
+  DevBuf d_ptr, d_a, d_off, d_stride, d_table;
+  DevBuf ev_ptr, ev_pos, ev_id;
+  // outputs
+  std::vector<int64_t> out_cells;
+  DevBuf oev_ptr, oev_pos, oev_id;
+  DevBuf obuf;
+  int64_t obuf_rows = 0;
+};
+
+enum Family { kAcoustic, kThermal, kNonlinear };
+
+}  // namespace
+
+struct fds_ctx {
+  int model = 0, form = 0, device = 0;
+  int64_t nx = 0, ny = 0, n = 0;
+  Family family = kAcoustic;
+  bool one_d = false;     // 1D linear model run on the 2D kernels (nx = n, ny = 1)
+  bool axi = false;
+  hipStream_t stream = nullptr;
+  Comp comp[3];
+  int parity = 0;
+  DevBuf coef[FDS_NUM_COEFF];
+  bool coef_set[FDS_NUM_COEFF] = {};
+  DevBuf zeros;           // n zeros of the state type (1D y coefficients, absent slots)
+  double s[FDS_NUM_SCALARS] = {};
+  bool lossy = false, iso_b = true, iso_k = true;
+  bool committed = false;
+  int tile = -1;
+  int tx = 0, ty = 0, tiles_x = 0, tiles_y = 0, ntiles = 0;
+  bool events_dirty = true;
+  DevBuf dyn;
+  DevBuf zero_ptr;        // ntiles+1 zeros: empty event lists
+  const fds::KernelSet *ks = nullptr;
+  size_t elem = 8;
+  // graphs: key (chunk, parity) -> exec
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+  uint64_t generation = 0, graph_generation = ~0ull;
+  std::vector<hipGraph_t> graph_objs;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;   // bracket the launches of the last run
+  double last_run_ms = 0.0;
+};
+
+namespace {
+
+bool is_kahan(const fds_ctx *c) { return c->form == FDS_FP32_KAHAN; }
+
+void drop_graphs(fds_ctx *c) {
+  for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto g : c->graph_objs) (void)hipGraphDestroy(g);
+  c->graphs.clear();
+  c->graph_objs.clear();
+}
+
+void touch(fds_ctx *c) { c->generation++; }
+
+Region region_of(const fds_ctx *c, int k) {
+  const int tx = c->tx, ty = c->ty;
+  if (c->family == kNonlinear) {
+    if (k == 0) return Region{0, 0, -1, tx, tx + 2};
+    if (k == 1) return Region{0, 0, 0, tx, tx + 1};
+    return Region{0, 0, 0, tx - 1, tx};
+  }
+  if (k == 0) return Region{-1, ty, -1, tx, tx + 2};
+  if (k == 1) return Region{0, ty - 1, 0, tx, tx + 1};
+  return Region{0, ty, 0, tx - 1, tx};
+}
+
+void build_bound_events(const fds_ctx *c, int k, Events &ev) {
+  const Comp &cp = c->comp[k];
+  const Region R = region_of(c, k);
+  const int64_t nx = c->nx, tx = c->tx, ty = c->ty;
+  std::vector<std::pair<int32_t, std::pair<int32_t, int32_t>>> all;   // tile, (pos, u)
+  const int64_t cmax_abs = (int64_t)(c->tiles_x - 1) * tx + R.cmax;
+  const int64_t rmax_abs = (int64_t)(c->tiles_y - 1) * ty + R.rmax;
+  for (size_t u = 0; u < cp.cells.size(); ++u) {
+    const int64_t j = cp.cells[u];
+    // absolute row Rw and column C with Rw*nx + C = j, C in [cmin, cmax_abs]
+    const int64_t rlo = std::max<int64_t>(R.rmin, ceildiv(j - cmax_abs, nx));
+    const int64_t rhi = std::min<int64_t>(rmax_abs, floordiv(j - R.cmin, nx));
+    for (int64_t Rw = rlo; Rw <= rhi; ++Rw) {
+      const int64_t C = j - Rw * nx;
+      const int64_t ty_lo = std::max<int64_t>(0, ceildiv(Rw - R.rmax, ty));
+      const int64_t ty_hi = std::min<int64_t>(c->tiles_y - 1, floordiv(Rw - R.rmin, ty));
+      const int64_t tx_lo = std::max<int64_t>(0, ceildiv(C - R.cmax, tx));
+      const int64_t tx_hi = std::min<int64_t>(c->tiles_x - 1, floordiv(C - R.cmin, tx));
+      for (int64_t tyi = ty_lo; tyi <= ty_hi; ++tyi)
+        for (int64_t txi = tx_lo; txi <= tx_hi; ++txi) {
+          const int r = (int)(Rw - tyi * ty), col = (int)(C - txi * tx);
+          all.push_back({(int32_t)(tyi * c->tiles_x + txi), {R.pos(r, col), (int32_t)u}});
+        }
+    }
+  }
+  std::sort(all.begin(), all.end());
+  ev.ptr.assign(c->ntiles + 1, 0);
+  ev.pos.clear();
+  ev.id.clear();
+  for (auto &t : all) {
+    ev.ptr[t.first + 1]++;
+    ev.pos.push_back(t.second.first);
+    ev.id.push_back(t.second.second);
+  }
+  for (int t = 0; t < c->ntiles; ++t) ev.ptr[t + 1] += ev.ptr[t];
+}
+
+void build_output_events(const fds_ctx *c, int k, Events &ev) {
+  const Comp &cp = c->comp[k];
+  const Region R = region_of(c, k);
+  std::vector<std::pair<int32_t, std::pair<int32_t, int32_t>>> all;
+  for (size_t s = 0; s < cp.out_cells.size(); ++s) {
+    const int64_t j = cp.out_cells[s];
+    const int64_t iy = j / c->nx, ix = j % c->nx;
+    const int64_t tyi = iy / c->ty, txi = ix / c->tx;
+    const int r = (int)(iy - tyi * c->ty), col = (int)(ix - txi * c->tx);
+    all.push_back({(int32_t)(tyi * c->tiles_x + txi), {R.pos(r, col), (int32_t)s}});
+  }
+  std::sort(all.begin(), all.end());
+  ev.ptr.assign(c->ntiles + 1, 0);
+  ev.pos.clear();
+  ev.id.clear();
+  for (auto &t : all) {
+    ev.ptr[t.first + 1]++;
+    ev.pos.push_back(t.second.first);
+    ev.id.push_back(t.second.second);
+  }
+  for (int t = 0; t < c->ntiles; ++t) ev.ptr[t + 1] += ev.ptr[t];
+}
+
+int choose_tiles(fds_ctx *c) {
+  if (c->family == kNonlinear) {
+    c->tx = fds::kNlTile;
+    c->ty = 1;
+  } else {
+    int t = c->tile;
+    if (t < 0) t = (c->ny == 1) ? fds::kTileRow : fds::kDefaultTile2D;
+    c->tx = fds::kTileShapes[t][0];
+    c->ty = fds::kTileShapes[t][1];
+  }
+  c->tiles_x = (int)((c->nx + c->tx - 1) / c->tx);
+  c->tiles_y = (int)((c->ny + c->ty - 1) / c->ty);
+  const int64_t nt = (int64_t)c->tiles_x * c->tiles_y;
+  if (nt > (int64_t)std::numeric_limits<int32_t>::max() / 2)
+    return fail(FDS_E_ARG, "grid too large for the tile index (%lld tiles)", (long long)nt);
+  c->ntiles = (int)nt;
+  c->events_dirty = true;
+  return 0;
+}
+
+size_t lds_bytes(const fds_ctx *c) {
+  if (c->family == kNonlinear) return 0;
+  const bool sep_k = c->axi || !c->iso_k;
+  fds::Lds2D L = fds::lds2d_layout(c->tx, c->ty, !c->iso_b, sep_k, is_kahan(c));
+  return (size_t)L.total * c->elem;
+}
+
+int rebuild_events(fds_ctx *c) {
+  if (!c->events_dirty) return 0;
+  std::vector<int32_t> zp(c->ntiles + 1, 0);
+  HIP_TRY(upload(c->zero_ptr, zp, c->stream));
+  for (int k = 0; k < 3; ++k) {
+    Comp &cp = c->comp[k];
+    if (!cp.present) continue;
+    Events ev;
+    build_bound_events(c, k, ev);
+    HIP_TRY(upload(cp.ev_ptr, ev.ptr, c->stream));
+    HIP_TRY(upload(cp.ev_pos, ev.pos, c->stream));
+    HIP_TRY(upload(cp.ev_id, ev.id, c->stream));
+    Events oe;
+    build_output_events(c, k, oe);
+    HIP_TRY(upload(cp.oev_ptr, oe.ptr, c->stream));
+    HIP_TRY(upload(cp.oev_pos, oe.pos, c->stream));
+    HIP_TRY(upload(cp.oev_id, oe.id, c->stream));
+  }
+  c->events_dirty = false;
+  touch(c);
+  return 0;
+}
+
+// Convert host float64 to the state type and copy to the device.
+int to_device(fds_ctx *c, void *dst, const double *src, int64_t n) {
+  if (c->elem == 8) {
+    HIP_TRY(hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, c->stream));
+  } else {
+    std::vector<float> tmp(n);
+    for (int64_t i = 0; i < n; ++i) tmp[i] = (float)src[i];
+    HIP_TRY(hipMemcpyAsync(dst, tmp.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+template <typename T>
+void fill_args(const fds_ctx *c, fds::StepArgs<T> &a, int parity) {
+  std::memset(&a, 0, sizeof a);
+  a.nx = (int32_t)c->nx;
+  a.ny = (int32_t)c->ny;
+  a.n = c->n;
+  a.tiles_x = c->tiles_x;
+  a.ntiles = c->ntiles;
+  for (int k = 0; k < 3; ++k) {
+    const Comp &cp = c->comp[k];
+    const bool pp = cp.accumulated || c->family != kThermal || k == 0;
+    const int rd = pp ? parity : 0, wr = pp ? parity ^ 1 : 0;
+    a.in[k] = cp.buf[rd].as<const T>();
+    a.out[k] = cp.buf[wr].as<T>();
+    a.cin[k] = cp.cbuf[rd].as<const T>();
+    a.cout[k] = cp.cbuf[wr].as<T>();
+    const bool has_b = !cp.cells.empty();
+    a.bev[k].ptr = has_b ? cp.ev_ptr.as<const int32_t>() : c->zero_ptr.as<const int32_t>();
+    a.bev[k].pos = cp.ev_pos.as<const int32_t>();
+    a.bev[k].id = cp.ev_id.as<const int32_t>();
+    a.prog[k].ptr = cp.d_ptr.as<const int32_t>();
+    a.prog[k].a = cp.d_a.as<const double>();
+    a.prog[k].off = cp.d_off.as<const int64_t>();
+    a.prog[k].stride = cp.d_stride.as<const int64_t>();
+    a.prog[k].table = cp.d_table.as<const double>();
+    const bool has_o = !cp.out_cells.empty();
+    a.outp[k].ev.ptr = has_o ? cp.oev_ptr.as<const int32_t>() : c->zero_ptr.as<const int32_t>();
+    a.outp[k].ev.pos = cp.oev_pos.as<const int32_t>();
+    a.outp[k].ev.id = cp.oev_id.as<const int32_t>();
+    a.outp[k].buf = cp.obuf.as<double>();
+    a.outp[k].m = (int32_t)cp.out_cells.size();
+  }
+  for (int s = 0; s < FDS_NUM_COEFF; ++s)
+    a.coef[s] = c->coef_set[s] ? c->coef[s].as<const T>() : c->zeros.as<const T>();
+  if (c->family != kNonlinear) {
+    // cleared y slots alias their x slot, except for 1D models whose y terms are zero
+    const int pairs[3][2] = {{0, 1}, {2, 3}, {4, 5}};
+    for (auto &pr : pairs)
+      if (!c->coef_set[pr[1]]) a.coef[pr[1]] = c->one_d ? c->zeros.as<const T>() : a.coef[pr[0]];
+  }
+  for (int s = 0; s < FDS_NUM_SCALARS; ++s) a.s[s] = c->s[s];
+  a.dyn = c->dyn.as<const int64_t>();
+  a.flags = (c->iso_b ? fds::kFlagIsoB : 0) | (c->iso_k ? fds::kFlagIsoK : 0);
+}
+
+fds::LaunchFn pick_launch(const fds_ctx *c) {
+  int t = 0;
+  for (int i = 0; i < fds::kNumTiles; ++i)
+    if (fds::kTileShapes[i][0] == c->tx && fds::kTileShapes[i][1] == c->ty) t = i;
+  switch (c->family) {
+    case kAcoustic: return c->ks->acoustic[t][c->axi ? 1 : 0][c->lossy ? 1 : 0];
+    case kThermal: return c->ks->thermal[t][c->axi ? 1 : 0];
+    default: return c->ks->nonlinear;
+  }
+}
+
+__global__ void k_advance(int64_t *dyn, int64_t by) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    dyn[0] += by;
+    dyn[1] += by;
+  }
+}
+
+// Launch one step: reads state buffer `parity`, writes parity^1.
+template <typename T>
+int launch_step(fds_ctx *c, int parity, int k, bool store) {
+  fds::StepArgs<T> a;
+  fill_args<T>(c, a, parity);
+  a.k = k;
+  if (store) a.flags |= fds::kFlagStore;
+  const int nb = fds::xcd_grid(c->ntiles);
+  pick_launch(c)(&a, nb, lds_bytes(c), c->stream);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int launch_step_any(fds_ctx *c, int parity, int k, bool store) {
+  return c->elem == 8 ? launch_step<double>(c, parity, k, store)
+                      : launch_step<float>(c, parity, k, store);
+}
+
+int get_graph(fds_ctx *c, int chunk, int parity, hipGraphExec_t *out) {
+  if (c->graph_generation != c->generation) {
+    drop_graphs(c);
+    c->graph_generation = c->generation;
+  }
+  auto key = std::make_pair(chunk, parity);
+  auto it = c->graphs.find(key);
+  if (it != c->graphs.end()) {
+    *out = it->second;
+    return 0;
+  }
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int k = 0; k < chunk && rc == 0; ++k) rc = launch_step_any(c, parity ^ (k & 1), k, false);
+  if (rc == 0) {
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->dyn.as<int64_t>(),
+                       (int64_t)chunk);
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(FDS_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  hipGraphExec_t ex = nullptr;
+  e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(FDS_E_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+  }
+  c->graph_objs.push_back(g);
+  c->graphs[key] = ex;
+  *out = ex;
+  return 0;
+}
+
+int check_ctx(fds_ctx *c) {
+  if (!c) return fail(FDS_E_ARG, "null context");
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return fail(FDS_E_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int check_comp(fds_ctx *c, int comp) {
+  if (comp < 0 || comp > 2 || !c->comp[comp].present)
+    return fail(FDS_E_ARG, "component %d does not exist for this model", comp);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fds_last_error(void) { return g_last_error.c_str(); }
+
+const char *fds_version(void) { return "pyfds_amd 0.1 (gfx950)"; }
+
+int fds_tile_count(void) { return fds::kNumTiles; }
+
+int fds_tile_shape(int i, int *tx, int *ty) {
+  if (i < 0 || i >= fds::kNumTiles) return fail(FDS_E_ARG, "tile index %d out of range", i);
+  *tx = fds::kTileShapes[i][0];
+  *ty = fds::kTileShapes[i][1];
+  return 0;
+}
+
+int fds_create(fds_ctx **out, int model, int form, int64_t nx, int64_t ny, int device) {
+  g_last_error.clear();
+  if (!out) return fail(FDS_E_ARG, "null output pointer");
+  *out = nullptr;
+  if (form < FDS_FP64 || form > FDS_FP32_KAHAN) return fail(FDS_E_ARG, "unknown form %d", form);
+  if (nx < 1 || ny < 1) return fail(FDS_E_ARG, "grid must be at least 1x1");
+  if (nx * ny > (int64_t)std::numeric_limits<int32_t>::max())
+    return fail(FDS_E_ARG, "grid of %lld cells exceeds 2^31", (long long)(nx * ny));
+  auto *c = new fds_ctx();
+  c->model = model;
+  c->form = form;
+  c->device = device;
+  c->n = nx * ny;
+  c->nx = nx;
+  c->ny = ny;
+  switch (model) {
+    case FDS_ACOUSTIC1D: c->family = kAcoustic; c->one_d = true; break;
+    case FDS_ACOUSTIC2D: c->family = kAcoustic; break;
+    case FDS_ACOUSTIC_AXI: c->family = kAcoustic; c->axi = true; break;
+    case FDS_THERMAL1D: c->family = kThermal; c->one_d = true; break;
+    case FDS_THERMAL2D: c->family = kThermal; break;
+    case FDS_THERMAL_AXI: c->family = kThermal; c->axi = true; break;
+    case FDS_IDEALGAS1D:
+    case FDS_ACOUSTIC2ND1D: c->family = kNonlinear; break;
+    default: delete c; return fail(FDS_E_ARG, "unknown model %d", model);
+  }
+  if ((c->one_d || c->family == kNonlinear) && ny != 1) {
+    delete c;
+    return fail(FDS_E_ARG, "1D models need ny = 1");
+  }
+  if (c->one_d || c->family == kNonlinear) {
+    c->nx = c->n;
+    c->ny = 1;
+  }
+  c->elem = (form == FDS_FP64) ? 8 : 4;
+  c->ks = form == FDS_FP64 ? &fds::kernels_f64()
+                           : (form == FDS_FP32 ? &fds::kernels_f32() : &fds::kernels_k32());
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(FDS_E_HIP, "hipSetDevice(%d): %s", device, hipGetErrorString(e));
+  }
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(FDS_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  // components: all three exist for every model (1D linear models carry an inert third one)
+  for (int k = 0; k < 3; ++k) {
+    Comp &cp = c->comp[k];
+    cp.present = true;
+    cp.accumulated = (c->family != kThermal) ? !(c->family == kNonlinear && k == 0) : (k == 0);
+    const int nbuf = (c->family == kThermal && k > 0) ? 1 : 2;
+    for (int b = 0; b < nbuf; ++b) {
+      if (cp.buf[b].ensure(c->n * c->elem) != hipSuccess ||
+          hipMemsetAsync(cp.buf[b].p, 0, c->n * c->elem, c->stream) != hipSuccess) {
+        fds_destroy(c);
+        return fail(FDS_E_HIP, "out of device memory for %lld cells", (long long)c->n);
+      }
+      if (form == FDS_FP32_KAHAN && cp.accumulated) {
+        if (cp.cbuf[b].ensure(c->n * c->elem) != hipSuccess ||
+            hipMemsetAsync(cp.cbuf[b].p, 0, c->n * c->elem, c->stream) != hipSuccess) {
+          fds_destroy(c);
+          return fail(FDS_E_HIP, "out of device memory (compensation)");
+        }
+      }
+    }
+  }
+  if (c->zeros.ensure(c->n * c->elem) != hipSuccess ||
+      hipMemsetAsync(c->zeros.p, 0, c->n * c->elem, c->stream) != hipSuccess ||
+      c->dyn.ensure(64) != hipSuccess) {
+    fds_destroy(c);
+    return fail(FDS_E_HIP, "out of device memory");
+  }
+  c->s[FDS_S_DX] = 1.0;
+  c->s[FDS_S_HALF_DX] = 0.5;
+  int rc = choose_tiles(c);
+  if (rc) {
+    fds_destroy(c);
+    return rc;
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    fds_destroy(c);
+    return fail(FDS_E_HIP, "stream synchronize failed");
+  }
+  *out = c;
+  return 0;
+}
+
+int fds_destroy(fds_ctx *c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graphs(c);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int fds_set_scalars(fds_ctx *c, const double *s, int n) {
+  if (int rc = check_ctx(c)) return rc;
+  if (n < 0 || n > FDS_NUM_SCALARS) return fail(FDS_E_ARG, "bad scalar count %d", n);
+  for (int i = 0; i < n; ++i) c->s[i] = s[i];
+  touch(c);
+  return 0;
+}
+
+int fds_set_coeff(fds_ctx *c, int slot, const double *data, int64_t n) {
+  if (int rc = check_ctx(c)) return rc;
+  if (slot < 0 || slot >= FDS_NUM_COEFF) return fail(FDS_E_ARG, "bad coefficient slot %d", slot);
+  if (n == 0) {
+    c->coef[slot].release();
+    c->coef_set[slot] = false;
+  } else {
+    if (n != c->n) return fail(FDS_E_ARG, "coefficient needs %lld values, got %lld",
+                               (long long)c->n, (long long)n);
+    HIP_TRY(c->coef[slot].ensure(n * c->elem));
+    if (int rc = to_device(c, c->coef[slot].p, data, n)) return rc;
+    c->coef_set[slot] = true;
+  }
+  c->committed = false;
+  touch(c);
+  return 0;
+}
+
+int fds_commit(fds_ctx *c) {
+  if (int rc = check_ctx(c)) return rc;
+  if (c->family == kAcoustic) {
+    if (!c->coef_set[FDS_C_KX] || !c->coef_set[FDS_C_BX])
+      return fail(FDS_E_STATE, "acoustic coefficients K and B must be set");
+    c->lossy = c->coef_set[FDS_C_MX];
+    if (c->axi && c->lossy && (!c->coef_set[FDS_C_MC] || !c->coef_set[FDS_C_MT]))
+      return fail(FDS_E_STATE, "lossy Axi field needs the MC and MT coefficients");
+  } else if (c->family == kThermal) {
+    if (!c->coef_set[FDS_C_GX] || !c->coef_set[FDS_C_QX])
+      return fail(FDS_E_STATE, "thermal coefficients G and Q must be set");
+  } else {
+    if (!c->coef_set[FDS_C_RHO0]) return fail(FDS_E_STATE, "static density must be set");
+  }
+  c->iso_b = !c->one_d && !c->coef_set[FDS_C_BY];   // (thermal: QY)
+  c->iso_k = !c->one_d && !c->coef_set[FDS_C_KY];   // (thermal: GY)
+  if (c->family != kNonlinear) {
+    const size_t lds = lds_bytes(c);
+    if (lds > 65536) {
+      // fall back to the default shape, which always fits
+      c->tile = -1;
+      if (int rc = choose_tiles(c)) return rc;
+    }
+  }
+  c->committed = true;
+  touch(c);
+  return 0;
+}
+
+int fds_set_state(fds_ctx *c, int comp, const double *data, int64_t n) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_comp(c, comp)) return rc;
+  if (n != c->n) return fail(FDS_E_ARG, "state needs %lld values", (long long)c->n);
+  Comp &cp = c->comp[comp];
+  const int b = (cp.buf[1].p && (cp.accumulated || c->family != kThermal || comp == 0)) ? c->parity : 0;
+  if (c->form == FDS_FP32_KAHAN && cp.accumulated) {
+    std::vector<float> x(n), cc(n);
+    for (int64_t i = 0; i < n; ++i) {
+      x[i] = (float)data[i];
+      cc[i] = (float)((double)x[i] - data[i]);
+    }
+    HIP_TRY(hipMemcpyAsync(cp.buf[b].p, x.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(cp.cbuf[b].p, cc.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+  }
+  return to_device(c, cp.buf[b].p, data, n);
+}
+
+int fds_get_state(fds_ctx *c, int comp, double *data, int64_t n) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_comp(c, comp)) return rc;
+  if (n != c->n) return fail(FDS_E_ARG, "state holds %lld values", (long long)c->n);
+  Comp &cp = c->comp[comp];
+  const int b = (cp.buf[1].p && (cp.accumulated || c->family != kThermal || comp == 0)) ? c->parity : 0;
+  if (c->elem == 8) {
+    HIP_TRY(hipMemcpyAsync(data, cp.buf[b].p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+  }
+  std::vector<float> x(n), cc;
+  HIP_TRY(hipMemcpyAsync(x.data(), cp.buf[b].p, n * 4, hipMemcpyDeviceToHost, c->stream));
+  const bool kahan = c->form == FDS_FP32_KAHAN && cp.accumulated;
+  if (kahan) {
+    cc.resize(n);
+    HIP_TRY(hipMemcpyAsync(cc.data(), cp.cbuf[b].p, n * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int64_t i = 0; i < n; ++i) data[i] = kahan ? (double)x[i] - (double)cc[i] : (double)x[i];
+  return 0;
+}
+
+int fds_set_boundaries(fds_ctx *c, int comp, int64_t ncell, const int64_t *cells,
+                       const int32_t *ptr, int64_t nent, const double *a, const int64_t *off,
+                       const int64_t *stride, const int64_t *len, int64_t ntable,
+                       const double *table) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_comp(c, comp)) return rc;
+  Comp &cp = c->comp[comp];
+  if (ncell < 0 || nent < 0 || ntable < 0) return fail(FDS_E_ARG, "negative size");
+  if (ncell > 0 && (int64_t)ptr[ncell] != nent) return fail(FDS_E_ARG, "ptr[ncell] != nent");
+  for (int64_t u = 0; u < ncell; ++u) {
+    if (cells[u] < 0 || cells[u] >= c->n)
+      return fail(FDS_E_INDEX, "boundary cell %lld out of range", (long long)cells[u]);
+    if (ptr[u] > ptr[u + 1]) return fail(FDS_E_ARG, "ptr not monotone");
+  }
+  for (int64_t e = 0; e < nent; ++e) {
+    if (off[e] < 0 || stride[e] < 0 || len[e] < 1 ||
+        (stride[e] > 0 && off[e] + (len[e] - 1) * stride[e] >= ntable) ||
+        (stride[e] == 0 && off[e] >= ntable))
+      return fail(FDS_E_ARG, "boundary entry %lld addresses outside the table", (long long)e);
+  }
+  cp.cells.assign(cells, cells + ncell);
+  cp.ptr.assign(ptr, ptr + (ncell > 0 ? ncell + 1 : 0));
+  cp.a.assign(a, a + nent);
+  cp.off.assign(off, off + nent);
+  cp.stride.assign(stride, stride + nent);
+  cp.len.assign(len, len + nent);
+  cp.table.assign(table, table + ntable);
+  HIP_TRY(upload(cp.d_ptr, cp.ptr, c->stream));
+  HIP_TRY(upload(cp.d_a, cp.a, c->stream));
+  HIP_TRY(upload(cp.d_off, cp.off, c->stream));
+  HIP_TRY(upload(cp.d_stride, cp.stride, c->stream));
+  HIP_TRY(upload(cp.d_table, cp.table, c->stream));
+  c->events_dirty = true;
+  touch(c);
+  return 0;
+}
+
+int fds_set_outputs(fds_ctx *c, int comp, int64_t m, const int64_t *cells) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_comp(c, comp)) return rc;
+  if (m < 0) return fail(FDS_E_ARG, "negative output count");
+  for (int64_t k = 0; k < m; ++k)
+    if (cells[k] < 0 || cells[k] >= c->n)
+      return fail(FDS_E_INDEX, "output cell %lld out of range", (long long)cells[k]);
+  Comp &cp = c->comp[comp];
+  cp.out_cells.assign(cells, cells + m);
+  c->events_dirty = true;
+  touch(c);
+  return 0;
+}
+
+int fds_set_tile(fds_ctx *c, int t) {
+  if (int rc = check_ctx(c)) return rc;
+  if (t < -1 || t >= fds::kNumTiles) return fail(FDS_E_ARG, "tile index %d out of range", t);
+  if (c->family == kNonlinear) return 0;
+  const int old = c->tile;
+  c->tile = t;
+  if (int rc = choose_tiles(c)) return rc;
+  if (c->committed && lds_bytes(c) > 65536) {
+    c->tile = old;
+    choose_tiles(c);
+    return fail(FDS_E_ARG, "tile %d needs more than 64 KiB of LDS for this field", t);
+  }
+  touch(c);
+  return 0;
+}
+
+int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->committed) return fail(FDS_E_STATE, "coefficients not committed");
+  if (nsteps < 0 || start < 0) return fail(FDS_E_ARG, "negative step count");
+  if (nsteps == 0) return 0;
+  // every boundary entry must have a value for every step of the run
+  for (int k = 0; k < 3; ++k) {
+    const Comp &cp = c->comp[k];
+    for (size_t e = 0; e < cp.len.size(); ++e)
+      if (start + nsteps > cp.len[e])
+        return fail(FDS_E_INDEX, "boundary signal of component %d has %lld samples, step %lld "
+                    "requested", k, (long long)cp.len[e], (long long)(start + nsteps - 1));
+  }
+  if (int rc = rebuild_events(c)) return rc;
+  // output rows for this run
+  for (int k = 0; k < 3; ++k) {
+    Comp &cp = c->comp[k];
+    const size_t need = std::max<size_t>(cp.out_cells.size() * nsteps * 8, 16);
+    if (cp.obuf.bytes < need) {
+      // grow geometrically (and to at least 4096 rows) so repeated runs keep their graphs
+      const size_t rows = std::max<int64_t>(nsteps, 4096);
+      HIP_TRY(cp.obuf.ensure(std::max(std::max(need, cp.obuf.bytes * 2),
+                                      cp.out_cells.size() * rows * 8)));
+      touch(c);
+    }
+    cp.obuf_rows = cp.out_cells.empty() ? 0 : nsteps;
+  }
+  int64_t dyn0[2] = {start, 0};
+  HIP_TRY(hipMemcpyAsync(c->dyn.p, dyn0, sizeof dyn0, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (!c->ev0) HIP_TRY(hipEventCreate(&c->ev0));
+  if (!c->ev1) HIP_TRY(hipEventCreate(&c->ev1));
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  int64_t done = 0;
+  if (chunk > 0) {
+    // graph-replayed full chunks, never including the run's last step (which stores the
+    // assigned components)
+    while (nsteps - done > chunk) {
+      hipGraphExec_t ex;
+      if (int rc = get_graph(c, chunk, c->parity, &ex)) return rc;
+      HIP_TRY(hipGraphLaunch(ex, c->stream));
+      c->parity ^= (chunk & 1);
+      done += chunk;
+    }
+  }
+  for (int64_t k = 0; done + k < nsteps; ++k) {
+    const bool last = (done + k == nsteps - 1);
+    if (int rc = launch_step_any(c, c->parity, (int)k, last)) return rc;
+    c->parity ^= 1;
+  }
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_run_ms = ms;
+  return 0;
+}
+
+int fds_last_run_ms(fds_ctx *c, double *ms) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!ms) return fail(FDS_E_ARG, "null output pointer");
+  *ms = c->last_run_ms;
+  return 0;
+}
+
+int fds_read_outputs(fds_ctx *c, int comp, int64_t row0, int64_t nrows, double *dst) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_comp(c, comp)) return rc;
+  Comp &cp = c->comp[comp];
+  const int64_t m = (int64_t)cp.out_cells.size();
+  if (m == 0 || nrows == 0) return 0;
+  if (row0 < 0 || nrows < 0 || row0 + nrows > cp.obuf_rows)
+    return fail(FDS_E_INDEX, "output rows [%lld, %lld) not recorded", (long long)row0,
+                (long long)(row0 + nrows));
+  HIP_TRY(hipMemcpyAsync(dst, cp.obuf.as<double>() + row0 * m, nrows * m * 8,
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,28 @@
+// Launch tables of the step kernels, one per numeric form.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fds {
+
+// Tile shapes (TX cells along x, TY rows) of the 2D step kernels.  The last entry serves
+// one-row grids (1D models run on the 2D kernels with nx = n, ny = 1).
+constexpr int kNumTiles = 6;
+constexpr int kTileShapes[kNumTiles][2] = {
+    {64, 16}, {64, 8}, {128, 8}, {256, 4}, {128, 4}, {256, 1}};
+constexpr int kDefaultTile2D = 0;
+constexpr int kTileRow = 5;
+
+using LaunchFn = void (*)(const void *args, int nblocks, size_t lds_bytes, hipStream_t s);
+
+struct KernelSet {
+  LaunchFn acoustic[kNumTiles][2][2];   // [tile][axi][lossy]
+  LaunchFn thermal[kNumTiles][2];       // [tile][axi]
+  LaunchFn nonlinear;
+};
+
+const KernelSet &kernels_f64();
+const KernelSet &kernels_f32();
+const KernelSet &kernels_k32();
+
+}  // namespace fds

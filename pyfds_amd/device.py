@@ -1,0 +1,291 @@
+"""Device side of a field: one HIP context per field, kept in step with the host objects.
+
+``DeviceRunner`` replaces the reference's per-step work of ``Field.simulate`` +
+``sim_step`` (``pyfds/fields.py:67-95``, ``pyfds/acoustics.py:111-128`` ...): before a run it
+uploads whatever the host changed (coefficients after ``assemble_matrices``, component
+values the user read or assigned, the boundary and output lists), then hands the whole run
+to ``fds_run`` and appends the recorded samples to the ``Output`` objects.
+
+Boundary lists are compiled into per-cell programs with the reference's semantics
+(``pyfds/fields.py:591-600``, ``pyfds/regions.py:125-145``): boundaries apply in list order;
+within one boundary all old values are read before any write and, for a repeated index, the
+last occurrence wins; the value is a scalar, ``signal[step]`` or one signal per point.
+"""
+import ctypes as ct
+
+import numpy as np
+
+from . import _native as nat
+
+INT64_MAX = np.iinfo(np.int64).max
+OUTPUT_BYTES_PER_SEGMENT = 256 << 20
+
+
+class BoundaryProgram:
+    """A component's boundary list compiled for the device (see fds_set_boundaries)."""
+
+    __slots__ = ('cells', 'ptr', 'a', 'off', 'stride', 'len', 'table')
+
+    def __init__(self, cells, ptr, a, off, stride, length, table):
+        self.cells, self.ptr, self.a, self.off = cells, ptr, a, off
+        self.stride, self.len, self.table = stride, length, table
+
+    def same_as(self, other):
+        return other is not None and all(
+            np.array_equal(getattr(self, f), getattr(other, f)) for f in self.__slots__)
+
+    @property
+    def valid_steps(self):
+        """Steps with a value for every entry (None when all values are scalars)."""
+        finite = self.len[self.len < INT64_MAX]
+        return int(finite.min()) if finite.size else None
+
+
+def normalise_indices(indices, n, what):
+    """Region indices as int64 with numpy's negative-index rule; IndexError if out of range."""
+    idx = np.asarray(indices, dtype=np.int64).ravel()
+    if idx.size and (idx.min() < -n or idx.max() >= n):
+        bad = idx[(idx < -n) | (idx >= n)][0]
+        raise IndexError('index {} is out of bounds for axis 0 with size {} ({})'
+                         .format(bad, n, what))
+    return np.where(idx < 0, idx + n, idx)
+
+
+def compile_boundaries(boundaries, n):
+    """Compile a boundary list into a :class:`BoundaryProgram`."""
+    cells, a_all, off_all, stride_all, len_all = [], [], [], [], []
+    table, toff = [], 0
+    for b in boundaries:
+        idx = normalise_indices(b.region.indices, n, 'boundary')
+        k = idx.size
+        if k == 0:
+            continue
+        a = float(b.additive)
+        v = b.value
+        if np.ndim(v) == 0:
+            seg = np.array([float(v)])
+            off = np.full(k, toff, np.int64)
+            stride = np.zeros(k, np.int64)
+            length = np.full(k, INT64_MAX, np.int64)
+        elif isinstance(v, np.ndarray):
+            arr = np.asarray(v, dtype=np.float64)
+            if arr.ndim == 1:
+                seg = arr
+                off = np.full(k, toff, np.int64)
+                stride = np.ones(k, np.int64)
+                length = np.full(k, arr.shape[0], np.int64)
+            elif arr.ndim == 2 and arr.shape[1] in (1, k):
+                # value[step] is a row: one column per point (or one shared column)
+                seg = np.ascontiguousarray(arr).ravel()
+                col = np.arange(k) if arr.shape[1] == k else np.zeros(k, np.int64)
+                off = toff + col.astype(np.int64)
+                stride = np.full(k, arr.shape[1], np.int64)
+                length = np.full(k, arr.shape[0], np.int64)
+            else:
+                raise ValueError('boundary signal of shape {} does not match {} points'
+                                 .format(arr.shape, k))
+        else:
+            sigs = [np.asarray(s, dtype=np.float64).ravel() for s in v]
+            if len(sigs) != k:
+                raise ValueError('shape mismatch: {} per-point signals for {} boundary points'
+                                 .format(len(sigs), k))
+            lens = np.array([s.shape[0] for s in sigs], np.int64)
+            seg = np.concatenate(sigs) if sigs else np.zeros(0)
+            off = toff + np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+            stride = np.ones(k, np.int64)
+            length = lens
+        # repeated index inside one boundary: the last occurrence wins
+        rev_u, rev_first = np.unique(idx[::-1], return_index=True)
+        last = k - 1 - rev_first
+        cells.append(rev_u)
+        a_all.append(np.full(rev_u.size, a))
+        off_all.append(off[last])
+        stride_all.append(stride[last])
+        len_all.append(length[last])
+        table.append(seg)
+        toff += seg.size
+    if not cells:
+        z = np.zeros(0, np.int64)
+        return BoundaryProgram(z, np.zeros(0, np.int32), np.zeros(0), z, z, z, np.zeros(0))
+    cell = np.concatenate(cells)
+    order = np.argsort(cell, kind='stable')      # keeps list order per cell
+    cell = cell[order]
+    ucell, counts = np.unique(cell, return_counts=True)
+    ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    return BoundaryProgram(ucell.astype(np.int64), ptr, np.concatenate(a_all)[order],
+                           np.concatenate(off_all)[order], np.concatenate(stride_all)[order],
+                           np.concatenate(len_all)[order],
+                           np.ascontiguousarray(np.concatenate(table), dtype=np.float64))
+
+
+class DeviceRunner:
+    """Owns the ``fds_ctx`` of one field."""
+
+    def __init__(self, field):
+        self.lib = nat.load()
+        self.ctx = None
+        self.coef_version = None
+        self.programs = [None, None, None]
+        self.out_cells = [None, None, None]
+        self.out_layout = [[], [], []]
+        self.owners = [None, None, None]
+        self.chunk = getattr(field, 'graph_chunk', 50)
+        self.tile = getattr(field, 'tile', None)
+
+    # -- lifetime -------------------------------------------------------------------------
+    def _create(self, field):
+        form = nat.FORMS.get(getattr(field, 'form', 'fp64'))
+        if form is None:
+            raise ValueError('unknown form {!r}; use one of {}'.format(field.form,
+                                                                        sorted(nat.FORMS)))
+        nx, ny = field._device_grid()
+        ctx = ct.c_void_p()
+        nat.check(self.lib.fds_create(ct.byref(ctx), field._model_id, form, nx, ny,
+                                      int(getattr(field, 'device', 0))))
+        self.ctx = ctx
+        self.n = nx * ny
+        if self.tile is not None:
+            nat.check(self.lib.fds_set_tile(self.ctx, int(self.tile)))
+
+    def close(self):
+        if self.ctx is not None and self.lib is not None:
+            self.lib.fds_destroy(self.ctx)
+        self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- host -> device ----------------------------------------------------------------
+    def _components(self, field):
+        return [getattr(field, name) if name else None for name in field._component_names]
+
+    def sync(self, field):
+        if self.ctx is None:
+            self._create(field)
+        lib, ctx = self.lib, self.ctx
+        if self.coef_version != field._coef_version:
+            coefs, scalars = field._device_coefficients()
+            for slot in range(nat.NUM_COEFF):
+                arr = coefs.get(slot)
+                if arr is None:
+                    nat.check(lib.fds_set_coeff(ctx, slot, None, 0))
+                else:
+                    arr = nat.f64(np.broadcast_to(arr, (self.n,)))
+                    nat.check(lib.fds_set_coeff(ctx, slot, nat.dptr(arr), arr.size))
+            s = nat.f64(np.zeros(nat.NUM_SCALARS))
+            for i, v in scalars.items():
+                s[i] = v
+            nat.check(lib.fds_set_scalars(ctx, nat.dptr(s), s.size))
+            nat.check(lib.fds_commit(ctx))
+            self.coef_version = field._coef_version
+        comps = self._components(field)
+        for k, comp in enumerate(comps):
+            if comp is None:
+                continue
+            if comp._host_dirty or self.owners[k] is not comp or comp._device is not None \
+                    and comp._device[0] is not self:
+                vals = nat.f64(comp._values)
+                if vals.shape != (self.n,):
+                    raise ValueError('component values must hold {} points, got shape {}'
+                                     .format(self.n, np.shape(comp._values)))
+                nat.check(lib.fds_set_state(ctx, k, nat.dptr(vals), vals.size))
+                comp._host_dirty = False
+                comp._device = None
+                self.owners[k] = comp
+            prog = compile_boundaries(comp.boundaries, self.n)
+            if not prog.same_as(self.programs[k]):
+                nat.check(lib.fds_set_boundaries(
+                    ctx, k, prog.cells.size, nat.lptr(prog.cells), nat.iptr(prog.ptr),
+                    prog.a.size, nat.dptr(prog.a), nat.lptr(prog.off), nat.lptr(prog.stride),
+                    nat.lptr(prog.len), prog.table.size, nat.dptr(prog.table)))
+                self.programs[k] = prog
+            layout, cells, start = [], [], 0
+            for out in comp.outputs:
+                idx = normalise_indices(out.region.indices, self.n, 'output')
+                layout.append((out, start, idx.size))
+                cells.append(idx)
+                start += idx.size
+            cells = np.concatenate(cells) if cells else np.zeros(0, np.int64)
+            if self.out_cells[k] is None or not np.array_equal(cells, self.out_cells[k]):
+                cells = nat.i64(cells)
+                nat.check(lib.fds_set_outputs(ctx, k, cells.size, nat.lptr(cells)))
+                self.out_cells[k] = cells
+            self.out_layout[k] = layout
+        return comps
+
+    # -- running --------------------------------------------------------------------------
+    def _valid_steps(self):
+        v = [p.valid_steps for p in self.programs if p is not None]
+        v = [x for x in v if x is not None]
+        return min(v) if v else None
+
+    def _segment(self, num_steps, progress):
+        m = sum(c.size for c in self.out_cells if c is not None)
+        seg = num_steps if m == 0 else max(1, OUTPUT_BYTES_PER_SEGMENT // (8 * m))
+        if progress is not None:
+            seg = min(seg, max(self.chunk, -(-num_steps // 20)))
+        return max(1, min(seg, num_steps))
+
+    def run_steps(self, field, start, nsteps):
+        """Run ``nsteps`` steps from pyfds step ``start``; append the recorded outputs."""
+        comps = self.sync(field)
+        nat.check(self.lib.fds_run(self.ctx, start, nsteps, int(self.chunk)))
+        for k, comp in enumerate(comps):
+            if comp is None:
+                continue
+            m = self.out_cells[k].size
+            if m:
+                data = np.empty((nsteps, m))
+                nat.check(self.lib.fds_read_outputs(self.ctx, k, 0, nsteps, nat.dptr(data)))
+                for out, s0, cnt in self.out_layout[k]:
+                    cols = data[:, s0:s0 + cnt].T
+                    if not out.signals:
+                        out.signals = [list(col) for col in cols]
+                    else:
+                        for sig, col in zip(out.signals, cols):
+                            sig.extend(col)
+            comp._device = (self, k)
+            comp._host_dirty = False
+
+    def simulate(self, field, start, num_steps, progress):
+        """``Field.simulate``'s loop; runs until the shortest boundary signal ends and then
+        raises IndexError like the reference's ``signal[step]`` would."""
+        self.sync(field)
+        valid = self._valid_steps()
+        n_ok = num_steps if valid is None else max(0, min(num_steps, valid - start))
+        seg = self._segment(num_steps, progress)
+        done = 0
+        while done < n_ok:
+            k = min(seg, n_ok - done)
+            self.run_steps(field, field.step, k)
+            if progress is not None:
+                for s in range(field.step, field.step + k):
+                    progress.log(s)
+            field.step += k
+            done += k
+        if n_ok < num_steps:
+            raise IndexError('index {} is out of bounds for axis 0 with size {}'
+                             .format(start + n_ok, valid))
+
+    def single_step(self, field):
+        """One ``sim_step`` at ``field.step`` (the caller advances the step counter)."""
+        self.sync(field)
+        valid = self._valid_steps()
+        if valid is not None and field.step >= valid:
+            raise IndexError('index {} is out of bounds for axis 0 with size {}'
+                             .format(field.step, valid))
+        self.run_steps(field, field.step, 1)
+
+    # -- device -> host -------------------------------------------------------------------
+    def download(self, k):
+        out = np.empty(self.n)
+        nat.check(self.lib.fds_get_state(self.ctx, k, nat.dptr(out), self.n))
+        return out
+
+    def download_all(self, field):
+        for comp in self._components(field):
+            if comp is not None and comp._device is not None:
+                comp.values  # noqa: B018 (property access downloads)

@@ -1,0 +1,129 @@
+"""Device parity: pyfds_amd (HIP kernels through the C-ABI) against the reference's golden
+fixtures and the CPU oracle.
+
+* FP64 form: bit-identical to the reference's float64 results (golden fixtures written by
+  running the reference) for every scene; the only exception is IdealGas1D's nonlinear
+  equation of state, which calls pow() (libm on the host, ocml on the GPU): relative L2
+  <= 1e-12 there.
+* FP32 and FP32-Kahan forms: relative L2 error against the FP64 oracle within the per-scene
+  tolerances below (stated per result; FP32 rounding of coefficients and states).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyfds_amd
+import scenes
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+POW_SCENES = {'ig1d_cnl'}
+
+pytestmark = pytest.mark.gpu
+
+
+def run_device(scene, form='fp64', **attrs):
+    fld = scenes.build_field(pyfds_amd, pyfds_amd.regions, scene)
+    fld.form = form
+    for k, v in attrs.items():
+        setattr(fld, k, v)
+    scenes.run_plan(fld, scene)
+    return scenes.collect(fld, scene)
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+@pytest.mark.parametrize('name', sorted(scenes.SCENES))
+def test_fp64_matches_reference_golden(name):
+    scene = scenes.SCENES[name]()
+    gold = np.load(os.path.join(GOLDEN, name + '.npz'))
+    print('scene', name, flush=True)
+    res = run_device(scene)
+    keys = [k for k in gold.files if k != 'fingerprint']
+    assert sorted(keys) == sorted(res)
+    for k in keys:
+        if name in POW_SCENES:
+            assert rel_l2(res[k], gold[k]) <= 1e-12, (name, k, rel_l2(res[k], gold[k]))
+        else:
+            np.testing.assert_array_equal(res[k], gold[k], err_msg='%s %s' % (name, k))
+
+
+# FP32 forms: relative L2 bounds per scene (values and signals), checked against the oracle
+FP32_TOL = {
+    'a1d_rigid': 1e-5, 'a1d_lossy_src': 1e-5, 'a2d_main': 1e-4, 'a2d_lossy': 1e-4,
+    'axi_lossless': 1e-4, 'axi_lossy': 1e-4, 't2d_blocks': 1e-4, 't2d_aniso': 1e-4,
+    't1d': 1e-5, 'taxi': 1e-4, 'a2o1d': 1e-4, 'bounds_semantics': 1e-4, 'a2d_wrap': 1e-4,
+    'ig1d_cnl': 1e-4, 'ig1d_nlin': 1e-4,
+}
+
+
+@pytest.mark.parametrize('form', ['fp32', 'fp32_kahan'])
+@pytest.mark.parametrize('name', sorted(scenes.SCENES))
+def test_fp32_forms_against_oracle(name, form):
+    scene = scenes.SCENES[name]()
+    ref = scenes.run_oracle(scene)
+    print('scene', name, form, flush=True)
+    res = run_device(scene, form=form)
+    for k in ref:
+        if k == 'step':
+            assert int(res[k]) == int(ref[k])
+            continue
+        err = rel_l2(res[k], ref[k])
+        print('  %-28s rel L2 %.3e' % (k, err))
+        assert err <= FP32_TOL[name], (name, form, k, err)
+
+
+@pytest.mark.parametrize('tile', range(5))
+def test_fp64_tile_shapes_bit_identical(tile):
+    """Every 2D tile shape gives the same bits (tiling must not change the arithmetic)."""
+    scene = scenes.scene_a2d_main()
+    gold = np.load(os.path.join(GOLDEN, 'a2d_main.npz'))
+    res = run_device(scene, tile=tile)
+    for k in gold.files:
+        if k != 'fingerprint':
+            np.testing.assert_array_equal(res[k], gold[k], err_msg=k)
+
+
+@pytest.mark.parametrize('chunk', [0, 1, 7, 50])
+def test_graph_chunks_bit_identical(chunk):
+    """HIP-graph replayed step chunks (and plain launches) give identical results."""
+    scene = scenes.scene_bounds_semantics()
+    gold = np.load(os.path.join(GOLDEN, 'bounds_semantics.npz'))
+    res = run_device(scene, graph_chunk=chunk)
+    for k in gold.files:
+        if k != 'fingerprint':
+            np.testing.assert_array_equal(res[k], gold[k], err_msg=k)
+
+
+def _big_acoustic(nx, ny, steps, form='fp64', seed=1):
+    dx = 1e-4
+    dt = 0.5 * dx / 5900.0
+    t = np.arange(steps) * dt
+    rng = np.random.default_rng(seed)
+    w, h = rng.integers(nx // 8, nx // 4, size=2)
+    x0, y0 = rng.integers(0, nx - w), rng.integers(0, ny - h)
+    incl = scenes.rect_indices(nx, int(x0), int(y0), int(w), int(h))
+    return dict(
+        name='big', model='Acoustic2D',
+        kwargs=dict(x_samples=nx, x_delta=dx, y_samples=ny, y_delta=dx, t_samples=steps,
+                    t_delta=dt),
+        material=('AcousticMaterial', dict(scenes.WATER)),
+        regions=[(incl, ('AcousticMaterial', dict(scenes.STEEL)))],
+        init={},
+        boundaries={'pressure': [dict(indices=[nx // 2 + nx * (ny // 2)],
+                                      value=np.sin(2 * np.pi * 1e6 * t) * (t < 5e-6),
+                                      additive=True)]},
+        outputs={'pressure': [[int(v) for v in rng.choice(nx * ny, 16, replace=False)]]},
+        plan=[('simulate', None)])
+
+
+def test_fp64_256x200_matches_oracle_bitwise():
+    scene = _big_acoustic(256, 200, 200)
+    ref = scenes.run_oracle(scene)
+    res = run_device(scene)
+    for k in ref:
+        np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
