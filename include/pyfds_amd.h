@@ -143,6 +143,10 @@ int fds_set_tile(fds_ctx *ctx, int tile_index);
 int fds_tile_count(void);
 int fds_tile_shape(int tile_index, int *tx, int *ty);
 
+/* Calibration (no reference counterpart): copies `bytes` bytes device-to-device `reps` times
+ * with a plain streaming kernel; returns read+write GB/s and milliseconds per copy. */
+int fds_stream_copy_bench(int64_t bytes, int reps, double *gbs, double *ms_each);
+
 /* Last error message of the calling thread ("" if none). */
 const char *fds_last_error(void);
 

@@ -161,9 +161,18 @@ __device__ void nonlinear1d_tile(const StepArgs<T> &a, int tile, int64_t step, i
       p = pa + pb;
     } else if (mode & kNlState) {
       const T r0 = s0[c];
-      const T ratio = (r0 + dr) / r0;
-      const T pw = (T)pow((double)ratio, (double)e2[j]);
-      p = e1[j] * (pw - T(1));
+      if (sizeof(T) == 8) {
+        // float64: the reference's expression, term by term
+        const T ratio = (r0 + dr) / r0;
+        const T pw = (T)pow((double)ratio, (double)e2[j]);
+        p = e1[j] * (pw - T(1));
+      } else {
+        // float32: ((rho0 + rho')/rho0)**gamma - 1 cancels catastrophically for small rho';
+        // evaluate it as expm1(gamma * log1p(rho'/rho0)) in float64 instead
+        const double xr = (double)dr / (double)r0;
+        const double pw1 = expm1((double)e2[j] * log1p(xr));
+        p = (T)((double)e1[j] * pw1);
+      }
     } else {
       p = csq[j] * dr;
     }

@@ -291,7 +291,7 @@ def scene_a2o1d():
 
 def scene_bounds_semantics():
     nx, ny, dx = 12, 10, 1e-4
-    dt = 0.5 * dx / 1500.0
+    dt = 0.5 * dx / 5900.0
     rng = np.random.default_rng(14)
     nt = 40
     per_point = [rng.standard_normal(nt + 10) for _ in range(3)]
@@ -323,7 +323,7 @@ def scene_bounds_semantics():
 
 def scene_a2d_wrap():
     nx, ny, dx = 7, 5, 1e-4
-    dt = 0.5 * dx / 1500.0
+    dt = 0.5 * dx / 5900.0
     rng = np.random.default_rng(15)
     return dict(
         name='a2d_wrap', model='Acoustic2D',

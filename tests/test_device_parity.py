@@ -3,8 +3,11 @@ fixtures and the CPU oracle.
 
 * FP64 form: bit-identical to the reference's float64 results (golden fixtures written by
   running the reference) for every scene; the only exception is IdealGas1D's nonlinear
-  equation of state, which calls pow() (libm on the host, ocml on the GPU): relative L2
-  <= 1e-12 there.
+  equation of state p = rho0 c^2/gamma (x**gamma - 1), x = (rho0 + rho')/rho0, which calls
+  pow() (glibc on the host, ocml on the GPU; both within about 1 ulp but not the same
+  bits).  The subtraction amplifies a 1-ulp pow difference by x**gamma/(x**gamma - 1)
+  ~ 1/(gamma |rho'/rho0|) ~ 1e3..1e4 for acoustic amplitudes, so that scene is held to
+  relative L2 <= 1e-10 (measured 2.7e-12 after 150 steps).
 * FP32 and FP32-Kahan forms: relative L2 error against the FP64 oracle within the per-scene
   tolerances below (stated per result; FP32 rounding of coefficients and states).
 """
@@ -47,7 +50,7 @@ def test_fp64_matches_reference_golden(name):
     assert sorted(keys) == sorted(res)
     for k in keys:
         if name in POW_SCENES:
-            assert rel_l2(res[k], gold[k]) <= 1e-12, (name, k, rel_l2(res[k], gold[k]))
+            assert rel_l2(res[k], gold[k]) <= 1e-10, (name, k, rel_l2(res[k], gold[k]))
         else:
             np.testing.assert_array_equal(res[k], gold[k], err_msg='%s %s' % (name, k))
 
