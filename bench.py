@@ -1,7 +1,8 @@
-"""Benchmark: Acoustic2D 2048x2048, 2000 steps, on one MI355X.
+"""Benchmark on one MI355X: Acoustic2D 2048x2048 (headline), batched sweep, Thermal2D 4096^2.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,fp32,fp32_kahan]
-                    [--cpu-steps S] [--no-cpu]
+                    [--sections acoustic2d,batched,thermal] [--cpu-steps S] [--no-cpu]
+                    [--batch-steps 1000] [--thermal-steps 2000]
 
 Workload (one step = one full time step of the field): water background, one seeded steel
 inclusion (c = 5900 m/s, rho = 7850 kg/m^3) of 128-512 cells per side, a 5-cycle 1 MHz sine
@@ -16,6 +17,17 @@ update (FP64: p, vx, vy read + write and the two coefficients K, B = 8 words = 6
 kernel time from HIP events recorded on the library's stream around the timed launches.
 ``cpu_baseline`` times the numpy oracle (the plain method, single core) on a bounded
 number of steps of the same workload.
+
+Further sections (reported under ``batched`` and ``thermal2d``):
+
+* batched: 256 Acoustic2D / Acoustic3DAxi instances, sides uniform in 128..1024 (seed 2),
+  1-4 seeded inclusions each (c 1000-6000 m/s, rho 800-8000 kg/m^3), dt at CFL 0.5 on each
+  instance's own c_max, a 1 MHz burst source at each centre and 16 recorded points, FP32 and
+  FP32-Kahan, one launch per step over all instances' tiles in HIP-graph chunks of 50.
+* thermal2d: Thermal2D 4096x4096, k in {0.6, 50} W/mK in seeded 256-cell blocks (seed 3),
+  rho c_p identical, Fourier number 0.2 on k_max, Gaussian hot spot (sigma 64 cells); FP64
+  and FP32-Kahan over a bounded number of the 100 000 steps (per-step cost is constant), with
+  the Kahan result's relative L2 error against FP64.
 """
 import argparse
 import json
@@ -29,6 +41,7 @@ NX = NY = 2048
 STEPS = 2000
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {'fp64': 64, 'fp32': 32, 'fp32_kahan': 56}
+THERMAL_BYTES_PER_CELL = {'fp64': 32, 'fp32': 16, 'fp32_kahan': 24}
 METRIC = 'Gcell-updates/s & % HBM roofline; FP32/batched/learned-BC vs plain FP64'
 
 
@@ -86,6 +99,151 @@ def time_form(ns, wl, form, steps, warmup):
                 kernel_us=kernel_s * 1e6, achieved_gbs=achieved, p=p)
 
 
+def batched_fields(ns, form, n_inst=256, seed=2, steps=1000):
+    """The 256-instance parameter sweep (see module docstring)."""
+    rng = np.random.default_rng(seed)
+    dx = 1e-4
+    fields = []
+    for i in range(n_inst):
+        nx, ny = (int(v) for v in rng.integers(128, 1025, size=2))
+        cls = ns.Acoustic2D if i % 2 == 0 else ns.Acoustic3DAxi
+        incl = []
+        for _ in range(int(rng.integers(1, 5))):
+            w = int(rng.integers(max(2, nx // 10), max(3, nx // 3)))
+            h = int(rng.integers(max(2, ny // 10), max(3, ny // 3)))
+            x0, y0 = int(rng.integers(0, nx - w)), int(rng.integers(0, ny - h))
+            incl.append((x0, y0, w, h, float(rng.uniform(1000, 6000)),
+                         float(rng.uniform(800, 8000))))
+        cmax = max([1500.0] + [c for *_, c, _ in incl])
+        dt = 0.5 * dx / cmax
+        fld = cls(x_samples=nx, x_delta=dx, y_samples=ny, y_delta=dx, t_samples=steps,
+                  t_delta=dt, material=ns.AcousticMaterial(1500.0, 1000.0))
+        fld.form = form
+        for x0, y0, w, h, c, rho in incl:
+            xs, ys = np.arange(x0, x0 + w), np.arange(y0, y0 + h)
+            fld.add_material_region(ns.regions.RectRegion(
+                (xs[:, None] + ys[None, :] * nx).ravel(), (x0, y0, w, h)),
+                ns.AcousticMaterial(c, rho))
+        t = np.arange(steps + 64) * dt
+        fld.pressure.add_boundary(ns.regions.PointRegion([nx // 2 + nx * (ny // 2)], None),
+                                  value=np.sin(2 * np.pi * 1e6 * t) * (t < 5e-6), additive=True)
+        fld.pressure.add_output(ns.regions.Region(
+            [int(v) for v in rng.choice(nx * ny, 16, replace=False)]))
+        fields.append(fld)
+    return fields
+
+
+def time_batched(ns, form, steps, warmup, chunk=50):
+    import torch
+    from pyfds_amd.batch import BatchRunner
+    fields = batched_fields(ns, form, steps=steps + warmup)
+    batch = BatchRunner(fields, chunk=chunk)
+    if warmup:
+        batch.simulate(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.simulate(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = batch.cells
+    kernel_s = batch.last_run_ms() / 1e3 / steps
+    achieved = BYTES_PER_CELL[form] * cells / kernel_s / 1e9
+    res = dict(gcells=cells * steps / wall / 1e9, ms_per_step=wall / steps * 1e3,
+               kernel_us=kernel_s * 1e6, achieved_gbs=achieved, cells=cells)
+    batch.close()
+    return res, fields
+
+
+def thermal_field(ns, form, steps, nx=4096, seed=3):
+    dx = 1e-3
+    rho, cp, kmax = 1000.0, 4000.0, 50.0
+    dt = 0.2 * dx ** 2 * rho * cp / kmax
+    fld = ns.Thermal2D(x_samples=nx, x_delta=dx, y_samples=nx, y_delta=dx, t_samples=steps,
+                       t_delta=dt, material=ns.ThermalMaterial(cp, rho, 0.6))
+    fld.form = form
+    rng = np.random.default_rng(seed)
+    nb = nx // 256
+    hot = rng.random((nb, nb)) < 0.5
+    blk = np.kron(hot, np.ones((256, 256), bool)).ravel()
+    fld.add_material_region(ns.regions.Region(np.flatnonzero(blk)),
+                            ns.ThermalMaterial(cp, rho, kmax))
+    iy, ix = np.divmod(np.arange(nx * nx), nx)
+    fld.temperature.values = np.exp(-0.5 * ((ix - nx / 2) ** 2 + (iy - nx / 2) ** 2) / 64.0 ** 2)
+    fld.temperature.add_output(ns.regions.Region(
+        [int(v) for v in rng.choice(nx * nx, 4, replace=False)]))
+    return fld
+
+
+def time_thermal(ns, form, steps, warmup):
+    import torch
+    fld = thermal_field(ns, form, steps + warmup)
+    fld.assemble_matrices()
+    if warmup:
+        fld.simulate(warmup)
+    runner = fld._runner()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fld.simulate(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    import ctypes as ct
+    ms = ct.c_double()
+    runner.lib.fds_last_run_ms(runner.ctx, ct.byref(ms))
+    cells = fld.num_points
+    kernel_s = ms.value / 1e3 / steps
+    achieved = THERMAL_BYTES_PER_CELL[form] * cells / kernel_s / 1e9
+    return dict(gcells=cells * steps / wall / 1e9, ms_per_step=wall / steps * 1e3,
+                kernel_us=kernel_s * 1e6, achieved_gbs=achieved,
+                t=np.array(fld.temperature.values))
+
+
+def cpu_baseline_batched(steps):
+    """Oracle over every instance of the sweep, ``steps`` steps each."""
+    import oracle
+    import pyfds_amd as ns
+    fields = batched_fields(ns, 'fp64', steps=steps)
+    cells, wall = 0, 0.0
+    for f in fields:
+        n = f.num_points
+        mats = [(np.asarray(m.region.indices), dict(sound_velocity=m.materials[0].sound_velocity,
+                                                     density=m.materials[0].density,
+                                                     absorption_coef=0))
+                for m in f.material_regions]
+        cls = oracle.PlainAcoustic2D if type(f).__name__ == 'Acoustic2D' \
+            else oracle.PlainAcoustic3DAxi
+        o = cls(f.x.samples, f.x.increment, steps, f.t.increment, mats, ny=f.y.samples,
+                dy=f.y.increment)
+        b = f.pressure.boundaries[0]
+        o.boundaries['pressure'] = [dict(indices=b.region.indices, value=b.value, additive=True)]
+        o.outputs['pressure'] = [dict(indices=f.pressure.outputs[0].region.indices, signals=[])]
+        o.assemble()
+        t0 = time.perf_counter()
+        o.simulate(steps)
+        wall += time.perf_counter() - t0
+        cells += n * steps
+    return cells / wall / 1e9
+
+
+def cpu_baseline_thermal(steps):
+    import oracle
+    import pyfds_amd as ns
+    f = thermal_field(ns, 'fp64', steps)
+    mats = [(np.asarray(m.region.indices), dict(heat_capacity=m.materials[0].heat_capacity,
+                                                 density=m.materials[0].density,
+                                                 thermal_conductivity_x=m.materials[0]
+                                                 .thermal_conductivity_x,
+                                                 thermal_conductivity_y=m.materials[0]
+                                                 .thermal_conductivity_y))
+            for m in f.material_regions]
+    o = oracle.PlainThermal2D(f.x.samples, f.x.increment, steps, f.t.increment, mats,
+                              ny=f.y.samples, dy=f.y.increment)
+    o.components['temperature'] = np.array(f.temperature.values)
+    o.assemble()
+    t0 = time.perf_counter()
+    o.simulate(steps)
+    return f.num_points * steps / (time.perf_counter() - t0) / 1e9
+
+
 def cpu_baseline(wl, steps):
     """The numpy oracle (plain FP64 method) on the same workload, single core."""
     import oracle
@@ -116,6 +274,11 @@ def main():
     ap.add_argument('--forms', default='fp64,fp32,fp32_kahan')
     ap.add_argument('--cpu-steps', type=int, default=20)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--sections', default='acoustic2d,batched,thermal')
+    ap.add_argument('--batch-steps', type=int, default=1000)
+    ap.add_argument('--batch-forms', default='fp32,fp32_kahan')
+    ap.add_argument('--thermal-steps', type=int, default=2000)
+    ap.add_argument('--thermal-forms', default='fp64,fp32_kahan')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build has no multi-GPU path '
@@ -167,6 +330,46 @@ def main():
         'cpu_baseline': cpu,
         'forms': out_forms,
     }
+    sections = set(args.sections.split(','))
+    if 'batched' in sections:
+        bat = {}
+        for f in [x for x in args.batch_forms.split(',') if x]:
+            r, _ = time_batched(ns, f, args.batch_steps, 10)
+            bat[f] = dict(value=round(r['gcells'], 3), ms_per_step=round(r['ms_per_step'], 4),
+                          kernel_us=round(r['kernel_us'], 2),
+                          achieved_gbs=round(r['achieved_gbs'], 1),
+                          roofline_frac=round(r['achieved_gbs'] / HBM_PEAK_GBS, 4),
+                          bytes_per_cell=BYTES_PER_CELL[f], cells=r['cells'])
+        bat['config'] = ('256 Acoustic2D/Acoustic3DAxi instances, sides 128-1024, %d steps, '
+                         'HIP-graph chunks of 50' % args.batch_steps)
+        if not args.no_cpu:
+            bat['cpu_baseline'] = dict(value=round(cpu_baseline_batched(2), 5),
+                                       unit='Gcell-updates/s', cores=1, kind='port',
+                                       sample='numpy oracle, all 256 instances, 2 steps')
+        line['batched'] = bat
+    if 'thermal' in sections:
+        th = {}
+        tvals = {}
+        for f in [x for x in args.thermal_forms.split(',') if x]:
+            r = time_thermal(ns, f, args.thermal_steps, 10)
+            tvals[f] = r['t']
+            th[f] = dict(value=round(r['gcells'], 3), ms_per_step=round(r['ms_per_step'], 4),
+                         kernel_us=round(r['kernel_us'], 2),
+                         achieved_gbs=round(r['achieved_gbs'], 1),
+                         roofline_frac=round(r['achieved_gbs'] / HBM_PEAK_GBS, 4),
+                         bytes_per_cell=THERMAL_BYTES_PER_CELL[f])
+        if 'fp64' in tvals:
+            for f, t in tvals.items():
+                if f != 'fp64':
+                    th[f]['rel_l2_vs_fp64'] = float(np.linalg.norm(t - tvals['fp64'])
+                                                    / np.linalg.norm(tvals['fp64']))
+        th['config'] = ('Thermal2D 4096x4096, k 0.6/50 in 256-cell blocks, Fo 0.2, %d of the '
+                        '100000 steps' % args.thermal_steps)
+        if not args.no_cpu:
+            th['cpu_baseline'] = dict(value=round(cpu_baseline_thermal(3), 5),
+                                      unit='Gcell-updates/s', cores=1, kind='port',
+                                      sample='numpy oracle, Thermal2D 4096x4096, 3 steps')
+        line['thermal2d'] = th
     print(json.dumps(line), flush=True)
 
 

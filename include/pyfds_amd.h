@@ -143,6 +143,18 @@ int fds_set_tile(fds_ctx *ctx, int tile_index);
 int fds_tile_count(void);
 int fds_tile_shape(int tile_index, int *tx, int *ty);
 
+/* Batched sweeps (no reference counterpart: pyfds runs one field per Python loop).  A batch
+ * groups contexts that are fully set up (coefficients committed, state, boundaries, outputs)
+ * and share device and form; fds_batch_run advances all of them nsteps steps from start_step
+ * with one launch per step over the tiles of every member (one tile shape for all, index
+ * `tile` into the tile table or -1), replayed in HIP-graph chunks.  Afterwards each member's
+ * state and outputs read back with fds_get_state / fds_read_outputs as after fds_run. */
+typedef struct fds_batch fds_batch;
+int fds_batch_create(fds_batch **out, fds_ctx *const *ctxs, int n, int tile);
+int fds_batch_run(fds_batch *b, int64_t start_step, int64_t nsteps, int chunk);
+int fds_batch_last_run_ms(fds_batch *b, double *ms);
+int fds_batch_destroy(fds_batch *b);
+
 /* Calibration (no reference counterpart): copies `bytes` bytes device-to-device `reps` times
  * with a plain streaming kernel; returns read+write GB/s and milliseconds per copy. */
 int fds_stream_copy_bench(int64_t bytes, int reps, double *gbs, double *ms_each);

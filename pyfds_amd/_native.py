@@ -22,7 +22,8 @@ NUM_COEFF, NUM_SCALARS = 8, 8
 EXPORTS = ('fds_create', 'fds_destroy', 'fds_set_scalars', 'fds_set_coeff', 'fds_commit',
            'fds_set_state', 'fds_get_state', 'fds_set_boundaries', 'fds_set_outputs',
            'fds_run', 'fds_read_outputs', 'fds_set_tile', 'fds_tile_count', 'fds_tile_shape',
-           'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_last_error', 'fds_version')
+           'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_batch_create', 'fds_batch_run',
+           'fds_batch_last_run_ms', 'fds_batch_destroy', 'fds_last_error', 'fds_version')
 
 
 class NativeError(RuntimeError):
@@ -67,6 +68,10 @@ def load():
         'fds_set_tile': (ct.c_int, [_p, ct.c_int]),
         'fds_last_run_ms': (ct.c_int, [_p, _dp]),
         'fds_stream_copy_bench': (ct.c_int, [_i64, ct.c_int, _dp, _dp]),
+        'fds_batch_create': (ct.c_int, [ct.POINTER(_p), ct.POINTER(_p), ct.c_int, ct.c_int]),
+        'fds_batch_run': (ct.c_int, [_p, _i64, _i64, ct.c_int]),
+        'fds_batch_last_run_ms': (ct.c_int, [_p, _dp]),
+        'fds_batch_destroy': (ct.c_int, [_p]),
         'fds_tile_count': (ct.c_int, []),
         'fds_tile_shape': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         'fds_last_error': (ct.c_char_p, []),

@@ -1,0 +1,84 @@
+"""Batched runs: many independent fields (a parameter sweep) advanced together.
+
+pyfds steps one field per Python loop.  ``BatchRunner`` keeps that per-field API -- every
+member is an ordinary ``pyfds_amd`` field with its own materials, boundaries and outputs -- but
+advances all members with one kernel launch per time step over the tiles of every member
+(``fds_batch_run``), replayed in HIP-graph chunks.  Members may differ in model (Acoustic2D,
+Acoustic3DAxi, Thermal2D, ...) and grid size; they must share the numeric form and device,
+and they step in lock step from a common step counter.
+"""
+import ctypes as ct
+
+import numpy as np
+
+from . import _native as nat
+
+
+class BatchRunner:
+    """Advance ``fields`` together; ``tile`` picks one shape of the tile table for all."""
+
+    def __init__(self, fields, tile=None, chunk=50):
+        self.fields = list(fields)
+        if not self.fields:
+            raise ValueError('empty batch')
+        forms = {getattr(f, 'form', 'fp64') for f in self.fields}
+        if len(forms) != 1:
+            raise ValueError('batch members must share one form, got {}'.format(sorted(forms)))
+        self.tile = -1 if tile is None else int(tile)
+        self.chunk = int(chunk)
+        self.lib = nat.load()
+        self.handle = None
+        self._members = None
+
+    def close(self):
+        if self.handle is not None:
+            self.lib.fds_batch_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _prepare(self):
+        runners = []
+        for f in self.fields:
+            if not f.matrices_assembled:
+                f.assemble_matrices()
+            r = f._runner()
+            r.sync(f)
+            runners.append(r)
+        members = tuple(r.ctx.value for r in runners)
+        if self.handle is None or members != self._members:
+            self.close()
+            arr = (ct.c_void_p * len(members))(*members)
+            handle = ct.c_void_p()
+            nat.check(self.lib.fds_batch_create(ct.byref(handle), arr, len(members),
+                                                self.tile))
+            self.handle, self._members = handle, members
+        return runners
+
+    def simulate(self, num_steps):
+        """Advance every member ``num_steps`` steps (all members must be at the same step)."""
+        steps = {f.step for f in self.fields}
+        if len(steps) != 1:
+            raise ValueError('batch members are at different steps: {}'.format(sorted(steps)))
+        start = steps.pop()
+        runners = self._prepare()
+        valid = [v for v in (r._valid_steps() for r in runners) if v is not None]
+        if valid and start + num_steps > min(valid):
+            raise IndexError('a boundary signal ends at step {}'.format(min(valid)))
+        nat.check(self.lib.fds_batch_run(self.handle, start, num_steps, self.chunk))
+        for f, r in zip(self.fields, runners):
+            r.collect(f, num_steps)
+            f.step += num_steps
+
+    def last_run_ms(self):
+        ms = ct.c_double()
+        nat.check(self.lib.fds_batch_last_run_ms(self.handle, ct.byref(ms)))
+        return ms.value
+
+    @property
+    def cells(self):
+        return int(np.sum([f.num_points for f in self.fields]))

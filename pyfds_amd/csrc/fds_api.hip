@@ -145,6 +145,25 @@ struct fds_ctx {
   double last_run_ms = 0.0;
 };
 
+struct fds_batch {
+  std::vector<fds_ctx *> ctxs;
+  int device = 0, form = 0, tile = -1;
+  size_t elem = 8;
+  hipStream_t stream = nullptr;
+  const fds::KernelSet *ks = nullptr;
+  int tx = 0, ty = 0, ntiles = 0;
+  size_t lds = 0;
+  DevBuf descs[2][2];          // [parity][store]
+  DevBuf tile_inst, inst_base, dyn;
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+  std::vector<hipGraph_t> graph_objs;
+  std::vector<uint64_t> build_gen;   // member generations the descriptors were built for
+  std::vector<int> build_parity;     // member parities at build time
+  int flip = 0;                      // members flipped this many times (mod 2) since build
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_run_ms = 0.0;
+};
+
 namespace {
 
 bool is_kahan(const fds_ctx *c) { return c->form == FDS_FP32_KAHAN; }
@@ -785,6 +804,246 @@ int fds_read_outputs(fds_ctx *c, int comp, int64_t row0, int64_t nrows, double *
   HIP_TRY(hipMemcpyAsync(dst, cp.obuf.as<double>() + row0 * m, nrows * m * 8,
                          hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+bool batch_stale(const fds_batch *b) {
+  if (b->build_gen.size() != b->ctxs.size()) return true;
+  for (size_t i = 0; i < b->ctxs.size(); ++i) {
+    const fds_ctx *c = b->ctxs[i];
+    if (c->generation != b->build_gen[i] || (c->parity ^ b->build_parity[i]) != b->flip)
+      return true;
+  }
+  return false;
+}
+
+void batch_drop_graphs(fds_batch *b) {
+  for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto g : b->graph_objs) (void)hipGraphDestroy(g);
+  b->graphs.clear();
+  b->graph_objs.clear();
+}
+
+template <typename T>
+int batch_build(fds_batch *b, int64_t nsteps) {
+  for (auto *c : b->ctxs) {
+    if (int rc = rebuild_events(c)) return rc;
+    for (int k = 0; k < 3; ++k) {
+      Comp &cp = c->comp[k];
+      const size_t need = std::max<size_t>(cp.out_cells.size() * nsteps * 8, 16);
+      if (cp.obuf.bytes < need) {
+        const size_t rows = std::max<int64_t>(nsteps, 4096);
+        HIP_TRY(cp.obuf.ensure(std::max(std::max(need, cp.obuf.bytes * 2),
+                                        cp.out_cells.size() * rows * 8)));
+        touch(c);
+      }
+      cp.obuf_rows = cp.out_cells.empty() ? 0 : nsteps;
+    }
+  }
+  if (!batch_stale(b)) return 0;
+  const int ni = (int)b->ctxs.size();
+  for (int parity = 0; parity < 2; ++parity)
+    for (int store = 0; store < 2; ++store) {
+      std::vector<fds::StepArgs<T>> d(ni);
+      for (int i = 0; i < ni; ++i) {
+        fds_ctx *c = b->ctxs[i];
+        // members flip in lock step: descriptor set `parity` reads each member's buffer
+        // (its parity at build time) ^ parity
+        fill_args<T>(c, d[i], c->parity ^ parity);
+        d[i].dyn = b->dyn.as<const int64_t>();
+        if (c->axi) d[i].flags |= fds::kFlagAxi;
+        if (c->lossy) d[i].flags |= fds::kFlagLossy;
+        if (c->family == kThermal) d[i].flags |= fds::kFlagThermal;
+        if (store) d[i].flags |= fds::kFlagStore;
+      }
+      HIP_TRY(upload(b->descs[parity][store], d, b->stream));
+    }
+  batch_drop_graphs(b);
+  b->build_gen.clear();
+  b->build_parity.clear();
+  for (auto *c : b->ctxs) {
+    b->build_gen.push_back(c->generation);
+    b->build_parity.push_back(c->parity);
+  }
+  b->flip = 0;
+  return 0;
+}
+
+int batch_launch(fds_batch *b, int rel_parity, int k, bool store) {
+  fds::BatchLaunch L;
+  L.descs = b->descs[rel_parity][store ? 1 : 0].p;
+  L.tile_inst = b->tile_inst.as<const int32_t>();
+  L.inst_base = b->inst_base.as<const int32_t>();
+  L.ntiles = b->ntiles;
+  L.k = k;
+  int t = 0;
+  for (int i = 0; i < fds::kNumTiles; ++i)
+    if (fds::kTileShapes[i][0] == b->tx && fds::kTileShapes[i][1] == b->ty) t = i;
+  b->ks->batch[t](L, fds::xcd_grid(b->ntiles), b->lds, b->stream);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int batch_graph(fds_batch *b, int chunk, int rel_parity, hipGraphExec_t *out) {
+  auto key = std::make_pair(chunk, rel_parity);
+  auto it = b->graphs.find(key);
+  if (it != b->graphs.end()) {
+    *out = it->second;
+    return 0;
+  }
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int k = 0; k < chunk && rc == 0; ++k) rc = batch_launch(b, rel_parity ^ (k & 1), k, false);
+  if (rc == 0)
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, b->stream, b->dyn.as<int64_t>(),
+                       (int64_t)chunk);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(FDS_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  hipGraphExec_t ex = nullptr;
+  e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(FDS_E_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+  }
+  b->graph_objs.push_back(g);
+  b->graphs[key] = ex;
+  *out = ex;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fds_batch_create(fds_batch **out, fds_ctx *const *ctxs, int n, int tile) {
+  if (!out || !ctxs || n < 1) return fail(FDS_E_ARG, "batch needs at least one context");
+  *out = nullptr;
+  auto *b = new fds_batch();
+  b->ctxs.assign(ctxs, ctxs + n);
+  fds_ctx *c0 = ctxs[0];
+  b->device = c0->device;
+  b->form = c0->form;
+  b->elem = c0->elem;
+  b->ks = c0->ks;
+  b->tile = tile;
+  for (auto *c : b->ctxs) {
+    if (!c || !c->committed) { delete b; return fail(FDS_E_STATE, "batch member not committed"); }
+    if (c->device != b->device || c->form != b->form) {
+      delete b;
+      return fail(FDS_E_ARG, "batch members must share device and form");
+    }
+    if (c->family == kNonlinear) {
+      delete b;
+      return fail(FDS_E_ARG, "nonlinear 1D models cannot be batched");
+    }
+  }
+  hipError_t e = hipSetDevice(b->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete b; return fail(FDS_E_HIP, "stream: %s", hipGetErrorString(e)); }
+  // one tile shape for every member (re-derives each member's events)
+  const int t = tile < 0 ? fds::kDefaultTile2D : tile;
+  std::vector<int32_t> tinst, base;
+  size_t lds = 0;
+  for (int i = 0; i < n; ++i) {
+    fds_ctx *c = b->ctxs[i];
+    c->tile = t;
+    if (int rc = choose_tiles(c)) { fds_batch_destroy(b); return rc; }
+    touch(c);
+    lds = std::max(lds, lds_bytes(c));
+    base.push_back((int32_t)tinst.size());
+    for (int k = 0; k < c->ntiles; ++k) tinst.push_back(i);
+  }
+  if (lds > 65536) {
+    fds_batch_destroy(b);
+    return fail(FDS_E_ARG, "tile %d needs more than 64 KiB of LDS for some member", t);
+  }
+  b->tx = fds::kTileShapes[t][0];
+  b->ty = fds::kTileShapes[t][1];
+  b->lds = lds;
+  b->ntiles = (int)tinst.size();
+  if (upload(b->tile_inst, tinst, b->stream) != hipSuccess ||
+      upload(b->inst_base, base, b->stream) != hipSuccess || b->dyn.ensure(64) != hipSuccess) {
+    fds_batch_destroy(b);
+    return fail(FDS_E_HIP, "out of device memory for the batch tables");
+  }
+  *out = b;
+  return 0;
+}
+
+int fds_batch_destroy(fds_batch *b) {
+  if (!b) return 0;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graphs(b);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+int fds_batch_run(fds_batch *b, int64_t start, int64_t nsteps, int chunk) {
+  if (!b) return fail(FDS_E_ARG, "null batch");
+  HIP_TRY(hipSetDevice(b->device));
+  if (nsteps < 0 || start < 0) return fail(FDS_E_ARG, "negative step count");
+  if (nsteps == 0) return 0;
+  for (auto *c : b->ctxs) {
+    if (!c->committed) return fail(FDS_E_STATE, "coefficients not committed");
+    for (int k = 0; k < 3; ++k)
+      for (size_t e = 0; e < c->comp[k].len.size(); ++e)
+        if (start + nsteps > c->comp[k].len[e])
+          return fail(FDS_E_INDEX, "boundary signal too short for step %lld",
+                      (long long)(start + nsteps - 1));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  int rc = b->elem == 8 ? batch_build<double>(b, nsteps) : batch_build<float>(b, nsteps);
+  if (rc) return rc;
+  int64_t dyn0[2] = {start, 0};
+  HIP_TRY(hipMemcpyAsync(b->dyn.p, dyn0, sizeof dyn0, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (!b->ev0) HIP_TRY(hipEventCreate(&b->ev0));
+  if (!b->ev1) HIP_TRY(hipEventCreate(&b->ev1));
+  HIP_TRY(hipEventRecord(b->ev0, b->stream));
+  int rel = b->flip;
+  int64_t done = 0;
+  if (chunk > 0) {
+    while (nsteps - done > chunk) {
+      hipGraphExec_t ex;
+      if (int rc2 = batch_graph(b, chunk, rel, &ex)) return rc2;
+      HIP_TRY(hipGraphLaunch(ex, b->stream));
+      rel ^= (chunk & 1);
+      done += chunk;
+    }
+  }
+  for (int64_t k = 0; done + k < nsteps; ++k) {
+    if (int rc2 = batch_launch(b, rel, (int)k, done + k == nsteps - 1)) return rc2;
+    rel ^= 1;
+  }
+  HIP_TRY(hipEventRecord(b->ev1, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_run_ms = ms;
+  // members advanced nsteps steps in lock step
+  if (nsteps & 1) {
+    for (auto *c : b->ctxs) c->parity ^= 1;
+    b->flip ^= 1;
+  }
+  return 0;
+}
+
+int fds_batch_last_run_ms(fds_batch *b, double *ms) {
+  if (!b || !ms) return fail(FDS_E_ARG, "null argument");
+  *ms = b->last_run_ms;
   return 0;
 }
 

@@ -15,10 +15,23 @@ constexpr int kTileRow = 5;
 
 using LaunchFn = void (*)(const void *args, int nblocks, size_t lds_bytes, hipStream_t s);
 
+// Batched launch: one workgroup per tile of every instance; descs[i] are the instances'
+// step arguments, tile_inst maps a global tile to its instance, inst_base[i] is the first
+// global tile of instance i.
+struct BatchLaunch {
+  const void *descs;
+  const int32_t *tile_inst;
+  const int32_t *inst_base;
+  int ntiles;
+  int k;
+};
+using BatchFn = void (*)(const BatchLaunch &b, int nblocks, size_t lds_bytes, hipStream_t s);
+
 struct KernelSet {
   LaunchFn acoustic[kNumTiles][2][2];   // [tile][axi][lossy]
   LaunchFn thermal[kNumTiles][2];       // [tile][axi]
   LaunchFn nonlinear;
+  BatchFn batch[kNumTiles];
 };
 
 const KernelSet &kernels_f64();

@@ -59,6 +59,9 @@ __host__ __device__ inline Lds2D lds2d_layout(int tx, int ty, bool sep_b, bool s
 constexpr int kFlagStore = 1;    // write assigned components (thermal heat flux)
 constexpr int kFlagIsoB = 2;     // y backward coefficients alias the x ones
 constexpr int kFlagIsoK = 4;     // y forward coefficients alias the x ones
+constexpr int kFlagAxi = 8;      // batched launches: axial-symmetric instance
+constexpr int kFlagLossy = 16;   // batched launches: lossy acoustic instance
+constexpr int kFlagThermal = 32; // batched launches: thermal instance
 
 template <typename T>
 __device__ __forceinline__ T radius(int64_t j, int64_t nx, double dx, double half_dx) {
@@ -389,6 +392,23 @@ __device__ void thermal_tile(const StepArgs<T> &a, int tile, int64_t step, int64
       a.out[1][j] = sqx_new[ki];
       a.out[2][j] = sqy_new[vi];
     }
+  }
+}
+
+// One tile of instance `a` in a batched launch: the model variant comes from a.flags.
+template <typename T, bool KAHAN, int TX, int TY>
+__device__ __forceinline__ void any_tile(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
+                                         T *lds) {
+  const int f = a.flags;
+  if (f & kFlagThermal) {
+    if (f & kFlagAxi) thermal_tile<T, KAHAN, TX, TY, true>(a, tile, step, row, lds);
+    else thermal_tile<T, KAHAN, TX, TY, false>(a, tile, step, row, lds);
+  } else if (f & kFlagAxi) {
+    if (f & kFlagLossy) acoustic_tile<T, KAHAN, TX, TY, true, true>(a, tile, step, row, lds);
+    else acoustic_tile<T, KAHAN, TX, TY, true, false>(a, tile, step, row, lds);
+  } else {
+    if (f & kFlagLossy) acoustic_tile<T, KAHAN, TX, TY, false, true>(a, tile, step, row, lds);
+    else acoustic_tile<T, KAHAN, TX, TY, false, false>(a, tile, step, row, lds);
   }
 }
 

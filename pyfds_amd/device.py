@@ -231,8 +231,14 @@ class DeviceRunner:
 
     def run_steps(self, field, start, nsteps):
         """Run ``nsteps`` steps from pyfds step ``start``; append the recorded outputs."""
-        comps = self.sync(field)
+        self.sync(field)
         nat.check(self.lib.fds_run(self.ctx, start, nsteps, int(self.chunk)))
+        self.collect(field, nsteps)
+
+    def collect(self, field, nsteps):
+        """After a device run of ``nsteps`` steps: append the recorded samples to the
+        ``Output`` objects and mark the components as held by the device."""
+        comps = self._components(field)
         for k, comp in enumerate(comps):
             if comp is None:
                 continue

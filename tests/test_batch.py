@@ -1,0 +1,62 @@
+"""Batched runs (one launch per step over the tiles of every member) give the same bits as
+running each member alone, and the FP64 batch matches the oracle bit for bit."""
+import numpy as np
+import pytest
+
+import pyfds_amd
+import scenes
+from pyfds_amd.batch import BatchRunner
+
+pytestmark = pytest.mark.gpu
+
+MEMBERS = ['a2d_main', 'axi_lossless', 'axi_lossy', 'a2d_lossy', 't2d_blocks', 't2d_aniso',
+           'bounds_semantics', 'taxi']
+PLAN = [('simulate', 17), ('simulate', 23)]
+
+
+def _members(form='fp64'):
+    out = []
+    for name in MEMBERS:
+        scene = dict(scenes.SCENES[name](), plan=PLAN)
+        fld = scenes.build_field(pyfds_amd, pyfds_amd.regions, scene)
+        fld.form = form
+        out.append((scene, fld))
+    return out
+
+
+@pytest.mark.parametrize('chunk', [0, 5])
+def test_fp64_batch_matches_oracle(chunk):
+    members = _members()
+    batch = BatchRunner([f for _, f in members], chunk=chunk)
+    for _, n in PLAN:
+        batch.simulate(n)
+    for scene, fld in members:
+        print('member', scene['name'], flush=True)
+        ref = scenes.run_oracle(scene)
+        got = scenes.collect(fld, scene)
+        for k in ref:
+            np.testing.assert_array_equal(got[k], ref[k], err_msg='%s %s' % (scene['name'], k))
+
+
+@pytest.mark.parametrize('tile', [1, 3])
+def test_kahan_batch_equals_single_runs(tile):
+    members = _members('fp32_kahan')
+    batch = BatchRunner([f for _, f in members], tile=tile, chunk=4)
+    for _, n in PLAN:
+        batch.simulate(n)
+    for scene, fld in members:
+        single = scenes.build_field(pyfds_amd, pyfds_amd.regions, scene)
+        single.form = 'fp32_kahan'
+        single.tile = tile
+        scenes.run_plan(single, scene)
+        a, b = scenes.collect(fld, scene), scenes.collect(single, scene)
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k], err_msg='%s %s' % (scene['name'], k))
+
+
+def test_batch_rejects_mixed_forms():
+    a = scenes.build_field(pyfds_amd, pyfds_amd.regions, scenes.scene_a2d_wrap())
+    b = scenes.build_field(pyfds_amd, pyfds_amd.regions, scenes.scene_a2d_wrap())
+    b.form = 'fp32'
+    with pytest.raises(ValueError):
+        BatchRunner([a, b])
