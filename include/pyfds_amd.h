@@ -155,8 +155,10 @@ int fds_batch_run(fds_batch *b, int64_t start_step, int64_t nsteps, int chunk);
 int fds_batch_last_run_ms(fds_batch *b, double *ms);
 int fds_batch_destroy(fds_batch *b);
 
-/* Calibration (no reference counterpart): copies `bytes` bytes device-to-device `reps` times
- * with a plain streaming kernel; returns read+write GB/s and milliseconds per copy. */
+/* Calibration (no reference counterpart): copies `bytes` bytes device-to-device |reps| times
+ * with a plain grid-stride streaming kernel (reps > 0) or an unrolled kernel with four loads in
+ * flight per thread and non-temporal stores (reps < 0); returns read+write GB/s and
+ * milliseconds per copy. */
 int fds_stream_copy_bench(int64_t bytes, int reps, double *gbs, double *ms_each);
 
 /* Last error message of the calling thread ("" if none). */

@@ -63,12 +63,21 @@ constexpr int kFlagAxi = 8;      // batched launches: axial-symmetric instance
 constexpr int kFlagLossy = 16;   // batched launches: lossy acoustic instance
 constexpr int kFlagThermal = 32; // batched launches: thermal instance
 
+// Radius of the cells in column ix (pyfds/acoustics.py:176: np.tile(x.vector, ny) + dx/2 with
+// x.vector = arange(nx)*dx), rounded like the reference: (ix*dx) + dx/2 in float64.
 template <typename T>
-__device__ __forceinline__ T radius(int64_t j, int64_t nx, double dx, double half_dx) {
-  // pyfds/acoustics.py:176: np.tile(x.vector, ny) + dx/2 with x.vector = arange(nx)*dx
-  const double ix = (double)(j % nx);
-  const double xi = ix * dx;
+__device__ __forceinline__ T radius(int ix, double dx, double half_dx) {
+  const double xi = (double)ix * dx;
   return (T)(xi + half_dx);
+}
+
+// Column of box column c of a tile starting at x0: the flat x +-1 neighbour wraps into the
+// previous / next row, so column x0 + c is taken modulo nx (c >= -1).
+__device__ __forceinline__ int wrap_col(int v, int nx) {
+  if (v < 0) v += nx;
+  if (v >= nx) v -= nx;
+  if (v >= nx) v %= nx;
+  return v;
 }
 
 template <typename T, bool KAHAN>
@@ -129,8 +138,8 @@ __device__ void acoustic_tile(const StepArgs<T> &a, int tile, int64_t step, int6
   const int tid = threadIdx.x;
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
   const int x0 = tx * TX, y0 = ty * TY;
-  const int64_t nx = a.nx, n = a.n;
-  const int64_t base = (int64_t)y0 * nx + x0;
+  const int nx = a.nx, n = (int)a.n;
+  const int base = y0 * nx + x0;
   const double dx = a.s[FDS_S_DX], hdx = a.s[FDS_S_HALF_DX];
   const T *Kx = a.coef[FDS_C_KX], *Ky = a.coef[FDS_C_KY];
   const T *Bx = a.coef[FDS_C_BX], *By = a.coef[FDS_C_BY];
@@ -142,7 +151,7 @@ __device__ void acoustic_tile(const StepArgs<T> &a, int tile, int64_t step, int6
   for (int e = tid; e < L.nb; e += kThreads) {
     const int rr = e / BW;
     const int r = rr - 1, c = e - rr * BW - 1;
-    const int64_t j = base + r * nx + c;
+    const int j = base + r * nx + c;
     const bool in = (j >= 0) && (j < n);
     sp[e] = in ? a.in[0][j] : T(0);
     sb[e] = in ? Bx[j] : T(0);
@@ -152,10 +161,10 @@ __device__ void acoustic_tile(const StepArgs<T> &a, int tile, int64_t step, int6
   // forward coefficients over rows [0, TY] x cols [0, TX]; Axi: x factor is K/r
   for (int e = tid; e < L.nk; e += kThreads) {
     const int r = e / KW, c = e - r * KW;
-    const int64_t j = base + r * nx + c;
+    const int j = base + r * nx + c;
     const bool in = j < n;
     if (AXI) {
-      sk[e] = in ? Kx[j] / radius<T>(j, nx, dx, hdx) : T(0);
+      sk[e] = in ? Kx[j] / radius<T>(wrap_col(x0 + c, nx), dx, hdx) : T(0);
       sky[e] = in ? Ky[j] : T(0);
     } else {
       sk[e] = in ? Kx[j] : T(0);
@@ -173,12 +182,15 @@ __device__ void acoustic_tile(const StepArgs<T> &a, int tile, int64_t step, int6
 
   // viscous operator V (a_vx_vx == a_vy_vy) at flat j applied to field f:
   // offsets [-nx, -1, 0, 1, nx] in that order; Axi adds the central term at +-1.
-  auto visc = [&](const T *f, int64_t j) -> T {
+  auto visc = [&](const T *f, int j, int ix) -> T {
     T y = T(0);
     if (j - nx >= 0) { const T t = My[j - nx] * f[j - nx]; y = y + t; }
     if (j - 1 >= 0) {
       T cm = Mx[j - 1];
-      if (AXI) { const T fc = Mc[j - 1] / radius<T>(j - 1, nx, dx, hdx); cm = cm + (-fc) / T(2); }
+      if (AXI) {
+        const T fc = Mc[j - 1] / radius<T>(wrap_col(ix - 1, nx), dx, hdx);
+        cm = cm + (-fc) / T(2);
+      }
       const T t = cm * f[j - 1];
       y = y + t;
     }
@@ -189,7 +201,10 @@ __device__ void acoustic_tile(const StepArgs<T> &a, int tile, int64_t step, int6
     }
     if (j + 1 < n) {
       T cp = Mx[j + 1];
-      if (AXI) { const T fc = Mc[j + 1] / radius<T>(j + 1, nx, dx, hdx); cp = cp + fc / T(2); }
+      if (AXI) {
+        const T fc = Mc[j + 1] / radius<T>(wrap_col(ix + 1, nx), dx, hdx);
+        cp = cp + fc / T(2);
+      }
       const T t = cp * f[j + 1];
       y = y + t;
     }
@@ -200,15 +215,16 @@ __device__ void acoustic_tile(const StepArgs<T> &a, int tile, int64_t step, int6
   // ---- velocities on tile + one halo column / row
   for (int e = tid; e < L.nvx; e += kThreads) {
     const int r = e / KW, c = e - r * KW;
-    const int64_t j = base + r * nx + c;
+    const int j = base + r * nx + c;
     T x = T(0), cc = T(0);
     if (j < n) {
       const int bm = (r + 1) * BW + c, bc = bm + 1;   // box (r, c-1), (r, c)
       T d = dbwd(sb[bm], sp[bm], sb[bc], sp[bc]);
       if (LOSSY) {
-        d = d - visc(vx_in, j);
+        const int ix = wrap_col(x0 + c, nx);
+        d = d - visc(vx_in, j, ix);
         if (AXI) {
-          const T rj = radius<T>(j, nx, dx, hdx);
+          const T rj = radius<T>(ix, dx, hdx);
           const T term = (Mt[j] * vx_in[j]) / (rj * rj);
           d = d + term;
         }
@@ -221,12 +237,12 @@ __device__ void acoustic_tile(const StepArgs<T> &a, int tile, int64_t step, int6
   }
   for (int e = tid; e < L.nvy; e += kThreads) {
     const int r = e / TX, c = e - r * TX;
-    const int64_t j = base + r * nx + c;
+    const int j = base + r * nx + c;
     T x = T(0), cc = T(0);
     if (j < n) {
       const int bm = r * BW + c + 1, bc = bm + BW;    // box (r-1, c), (r, c)
       T d = dbwd(sby[bm], sp[bm], sby[bc], sp[bc]);
-      if (LOSSY) d = d - visc(vy_in, j);
+      if (LOSSY) d = d - visc(vy_in, j, wrap_col(x0 + c, nx));
       x = vy_in[j];
       if (KAHAN) { cc = a.cin[2][j]; kahan_sub(x, cc, d); } else { x = x - d; }
     }
@@ -249,12 +265,12 @@ __device__ void acoustic_tile(const StepArgs<T> &a, int tile, int64_t step, int6
   for (int e = tid; e < TX * TY; e += kThreads) {
     const int r = e / TX, c = e - r * TX;
     if (x0 + c >= a.nx || y0 + r >= a.ny) continue;
-    const int64_t j = base + r * nx + c;
+    const int j = base + r * nx + c;
     const int ki = r * KW + c;
     T wc = svx[ki], wp = svx[ki + 1];
     if (AXI) {
-      wc = wc * radius<T>(j, nx, dx, hdx);
-      wp = wp * radius<T>(j + 1, nx, dx, hdx);
+      wc = wc * radius<T>(x0 + c, dx, hdx);
+      wp = wp * radius<T>(wrap_col(x0 + c + 1, nx), dx, hdx);
     }
     const T dxt = dfwd(sk[ki], wc, sk[ki + 1], wp);
     const int vi = r * TX + c;
@@ -294,8 +310,8 @@ __device__ void thermal_tile(const StepArgs<T> &a, int tile, int64_t step, int64
   const int tid = threadIdx.x;
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
   const int x0 = tx * TX, y0 = ty * TY;
-  const int64_t nx = a.nx, n = a.n;
-  const int64_t base = (int64_t)y0 * nx + x0;
+  const int nx = a.nx, n = (int)a.n;
+  const int base = y0 * nx + x0;
   const double dx = a.s[FDS_S_DX], hdx = a.s[FDS_S_HALF_DX];
   const T *Gx = a.coef[FDS_C_GX], *Gy = a.coef[FDS_C_GY];
   const T *Qx = a.coef[FDS_C_QX], *Qy = a.coef[FDS_C_QY];
@@ -303,7 +319,7 @@ __device__ void thermal_tile(const StepArgs<T> &a, int tile, int64_t step, int64
   for (int e = tid; e < L.nb; e += kThreads) {
     const int rr = e / BW;
     const int r = rr - 1, c = e - rr * BW - 1;
-    const int64_t j = base + r * nx + c;
+    const int j = base + r * nx + c;
     const bool in = (j >= 0) && (j < n);
     st[e] = in ? a.in[0][j] : T(0);
     sq[e] = in ? Qx[j] : T(0);
@@ -312,10 +328,10 @@ __device__ void thermal_tile(const StepArgs<T> &a, int tile, int64_t step, int64
   }
   for (int e = tid; e < L.nk; e += kThreads) {
     const int r = e / KW, c = e - r * KW;
-    const int64_t j = base + r * nx + c;
+    const int j = base + r * nx + c;
     const bool in = j < n;
     if (AXI) {
-      sg[e] = in ? Gx[j] / radius<T>(j, nx, dx, hdx) : T(0);
+      sg[e] = in ? Gx[j] / radius<T>(wrap_col(x0 + c, nx), dx, hdx) : T(0);
       sgy[e] = in ? Gy[j] : T(0);
     } else {
       sg[e] = in ? Gx[j] : T(0);
@@ -333,7 +349,7 @@ __device__ void thermal_tile(const StepArgs<T> &a, int tile, int64_t step, int64
   // heat flux q = -(Dbwd(k/d) T), assigned every step (pyfds/thermal.py:98-99)
   for (int e = tid; e < L.nvx; e += kThreads) {
     const int r = e / KW, c = e - r * KW;
-    const int64_t j = base + r * nx + c;
+    const int j = base + r * nx + c;
     T q = T(0);
     if (j < n) {
       const int bm = (r + 1) * BW + c, bc = bm + 1;
@@ -343,7 +359,7 @@ __device__ void thermal_tile(const StepArgs<T> &a, int tile, int64_t step, int64
   }
   for (int e = tid; e < L.nvy; e += kThreads) {
     const int r = e / TX, c = e - r * TX;
-    const int64_t j = base + r * nx + c;
+    const int j = base + r * nx + c;
     T q = T(0);
     if (j < n) {
       const int bm = r * BW + c + 1, bc = bm + BW;
@@ -367,12 +383,12 @@ __device__ void thermal_tile(const StepArgs<T> &a, int tile, int64_t step, int64
   for (int e = tid; e < TX * TY; e += kThreads) {
     const int r = e / TX, c = e - r * TX;
     if (x0 + c >= a.nx || y0 + r >= a.ny) continue;
-    const int64_t j = base + r * nx + c;
+    const int j = base + r * nx + c;
     const int ki = r * KW + c;
     T wc = sqx_new[ki], wp = sqx_new[ki + 1];
     if (AXI) {
-      wc = wc * radius<T>(j, nx, dx, hdx);
-      wp = wp * radius<T>(j + 1, nx, dx, hdx);
+      wc = wc * radius<T>(x0 + c, dx, hdx);
+      wp = wp * radius<T>(wrap_col(x0 + c + 1, nx), dx, hdx);
     }
     const T dxt = dfwd(sg[ki], wc, sg[ki + 1], wp);
     const int vi = r * TX + c;

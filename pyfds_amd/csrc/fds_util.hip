@@ -16,10 +16,27 @@ __global__ __launch_bounds__(256) void k_stream_copy(const float4 *__restrict__ 
     dst[i] = src[i];
 }
 
+// Four independent 16-byte loads in flight per thread before the stores, non-temporal stores.
+__global__ __launch_bounds__(256) void k_stream_copy4(const float4 *__restrict__ src,
+                                                      float4 *__restrict__ dst, int64_t n4) {
+  const int64_t base = ((int64_t)blockIdx.x * 256 * 4) + threadIdx.x;
+  float4 v[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t i = base + u * 256;
+    if (i < n4) v[u] = __builtin_nontemporal_load(src + i);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t i = base + u * 256;
+    if (i < n4) __builtin_nontemporal_store(v[u], dst + i);
+  }
+}
+
 }  // namespace
 
 extern "C" int fds_stream_copy_bench(int64_t bytes, int reps, double *gbs, double *ms_each) {
-  if (bytes < 16 || reps < 1 || !gbs || !ms_each) return FDS_E_ARG;
+  if (bytes < 16 || reps == 0 || !gbs || !ms_each) return FDS_E_ARG;
   const int64_t n4 = bytes / 16;
   void *a = nullptr, *b = nullptr;
   hipStream_t s = nullptr;
@@ -31,12 +48,21 @@ extern "C" int fds_stream_copy_bench(int64_t bytes, int reps, double *gbs, doubl
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) goto done;
   {
     const int grid = 256 * 16;
-    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, s, (const float4 *)a,
-                       (float4 *)b, n4);
+    const bool unrolled = reps < 0;
+    const int nrep = unrolled ? -reps : reps;
+    const int grid4 = (int)((n4 + 1023) / 1024);
+    auto launch = [&]() {
+      if (unrolled)
+        hipLaunchKernelGGL(k_stream_copy4, dim3(grid4), dim3(256), 0, s, (const float4 *)a,
+                           (float4 *)b, n4);
+      else
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, s, (const float4 *)a,
+                           (float4 *)b, n4);
+    };
+    launch();
     if (hipEventRecord(e0, s) != hipSuccess) goto done;
-    for (int r = 0; r < reps; ++r)
-      hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, s, (const float4 *)a,
-                         (float4 *)b, n4);
+    for (int r = 0; r < nrep; ++r) launch();
+    reps = nrep;
     if (hipEventRecord(e1, s) != hipSuccess) goto done;
     if (hipStreamSynchronize(s) != hipSuccess) goto done;
     float ms = 0.f;

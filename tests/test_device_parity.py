@@ -102,7 +102,7 @@ def test_graph_chunks_bit_identical(chunk):
             np.testing.assert_array_equal(res[k], gold[k], err_msg=k)
 
 
-def _big_acoustic(nx, ny, steps, form='fp64', seed=1):
+def _big_acoustic(nx, ny, steps, form='fp64', seed=1, model='Acoustic2D', mu=0.0):
     dx = 1e-4
     dt = 0.5 * dx / 5900.0
     t = np.arange(steps) * dt
@@ -111,10 +111,10 @@ def _big_acoustic(nx, ny, steps, form='fp64', seed=1):
     x0, y0 = rng.integers(0, nx - w), rng.integers(0, ny - h)
     incl = scenes.rect_indices(nx, int(x0), int(y0), int(w), int(h))
     return dict(
-        name='big', model='Acoustic2D',
+        name='big', model=model,
         kwargs=dict(x_samples=nx, x_delta=dx, y_samples=ny, y_delta=dx, t_samples=steps,
                     t_delta=dt),
-        material=('AcousticMaterial', dict(scenes.WATER)),
+        material=('AcousticMaterial', dict(scenes.WATER, absorption_coef=mu)),
         regions=[(incl, ('AcousticMaterial', dict(scenes.STEEL)))],
         init={},
         boundaries={'pressure': [dict(indices=[nx // 2 + nx * (ny // 2)],
@@ -124,8 +124,38 @@ def _big_acoustic(nx, ny, steps, form='fp64', seed=1):
         plan=[('simulate', None)])
 
 
-def test_fp64_256x200_matches_oracle_bitwise():
-    scene = _big_acoustic(256, 200, 200)
+@pytest.mark.parametrize('model,mu,nx,ny', [('Acoustic2D', 0.0, 256, 200),
+                                            ('Acoustic3DAxi', 0.0, 300, 130),
+                                            ('Acoustic3DAxi', 3e-3, 201, 77),
+                                            ('Acoustic2D', 2e-3, 130, 101)])
+def test_fp64_multi_tile_grids_match_oracle_bitwise(model, mu, nx, ny):
+    """Grids spanning many tiles, with partial tiles at the right and bottom edges."""
+    scene = _big_acoustic(nx, ny, 120, model=model, mu=mu)
+    ref = scenes.run_oracle(scene)
+    res = run_device(scene)
+    for k in ref:
+        np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
+
+
+@pytest.mark.parametrize('model', ['Thermal2D', 'Thermal3DAxi'])
+def test_fp64_thermal_multi_tile_matches_oracle_bitwise(model):
+    nx, ny, dx = 190, 150, 1e-3
+    rng = np.random.default_rng(5)
+    scene = dict(
+        name='tbig', model=model,
+        kwargs=dict(x_samples=nx, x_delta=dx, y_samples=ny, y_delta=dx, t_samples=150,
+                    t_delta=0.2 * dx ** 2 * 4e6 / 50.0),
+        material=('ThermalMaterial', dict(heat_capacity=4000.0, density=1000.0,
+                                          thermal_conductivity=0.6)),
+        regions=[(scenes._thermal_blocks(nx, ny, 16, 7),
+                  ('ThermalMaterial', dict(heat_capacity=4000.0, density=1000.0,
+                                           thermal_conductivity=(50.0, 30.0))))],
+        init={'temperature': rng.random(nx * ny)},
+        boundaries={'temperature': [dict(indices=list(range(0, nx * ny, nx)), value=0.0,
+                                         additive=False)]},
+        outputs={'temperature': [[int(v) for v in rng.choice(nx * ny, 8, replace=False)]],
+                 'heat_flux_y': [[nx * 70 + 3]]},
+        plan=[('simulate', 77), ('simulate', 73)])
     ref = scenes.run_oracle(scene)
     res = run_device(scene)
     for k in ref:
