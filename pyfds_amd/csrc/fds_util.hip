@@ -16,11 +16,13 @@ __global__ __launch_bounds__(256) void k_stream_copy(const float4 *__restrict__ 
     dst[i] = src[i];
 }
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 // Four independent 16-byte loads in flight per thread before the stores, non-temporal stores.
-__global__ __launch_bounds__(256) void k_stream_copy4(const float4 *__restrict__ src,
-                                                      float4 *__restrict__ dst, int64_t n4) {
+__global__ __launch_bounds__(256) void k_stream_copy4(const v4f *__restrict__ src,
+                                                      v4f *__restrict__ dst, int64_t n4) {
   const int64_t base = ((int64_t)blockIdx.x * 256 * 4) + threadIdx.x;
-  float4 v[4];
+  v4f v[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int64_t i = base + u * 256;
@@ -53,8 +55,8 @@ extern "C" int fds_stream_copy_bench(int64_t bytes, int reps, double *gbs, doubl
     const int grid4 = (int)((n4 + 1023) / 1024);
     auto launch = [&]() {
       if (unrolled)
-        hipLaunchKernelGGL(k_stream_copy4, dim3(grid4), dim3(256), 0, s, (const float4 *)a,
-                           (float4 *)b, n4);
+        hipLaunchKernelGGL(k_stream_copy4, dim3(grid4), dim3(256), 0, s, (const v4f *)a,
+                           (v4f *)b, n4);
       else
         hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, s, (const float4 *)a,
                            (float4 *)b, n4);
