@@ -155,6 +155,49 @@ int fds_batch_run(fds_batch *b, int64_t start_step, int64_t nsteps, int chunk);
 int fds_batch_last_run_ms(fds_batch *b, double *ms);
 int fds_batch_destroy(fds_batch *b);
 
+/* Learned absorbing boundary (no reference counterpart; pyfds has only the plain method).
+ * After every step of an FP32 Acoustic2D context on a square G x G grid, a BF16-MFMA network
+ * predicts the outer `layer` cells (16) of p, vx, vy on all four sides from the 8 adjacent
+ * interior rows at the current and previous time level.  w1 (bf16 [4][32][5][48]) and w2
+ * (bf16 [4][48][5][32]) are device pointers owned by the caller; comp_scale converts p, vx, vy
+ * to network units.  w1 = NULL switches the boundary off. */
+int fds_set_learned_boundary(fds_ctx *ctx, const uint16_t *w1, const uint16_t *w2, int layer,
+                             const float *comp_scale);
+
+/* Training kernels of the learned boundary (device pointers, `stream` = hipStream_t).  1D
+ * convolutions along a strip, zero padded: y[b,p,co] = sum_k sum_ci w[side[b],co,k,ci]
+ * x[b,p+k-K/2,ci]; x, dy: bf16 [B][P][C]; w: bf16 [S][Co][K][Ci]; wt: bf16 [S][Ci][K][Co];
+ * Ci and Co multiples of 16, K odd. */
+int fds_lbc_conv_fwd(const uint16_t *x, const uint16_t *w, const int32_t *side, float *y,
+                     uint16_t *yb, int B, int P, int Ci, int Co, int K, void *stream);
+int fds_lbc_conv_bwd_data(const uint16_t *dy, const uint16_t *wt, const int32_t *side,
+                          float *dx, uint16_t *dxb, int B, int P, int Ci, int Co, int K,
+                          void *stream);
+int fds_lbc_conv_bwd_weight(const uint16_t *dy, const uint16_t *x, float *part, int B, int P,
+                            int Ci, int Co, int K, void *stream);
+int fds_lbc_reduce_weight(const float *part, const int32_t *side, float *dw, int B, int S, int n,
+                          void *stream);
+int fds_lbc_mse(const float *y, const float *t, const int32_t *lo, const int32_t *hi,
+                const float *wch, uint16_t *dyb, float *loss_part, int nblocks, int B, int P,
+                int C, float norm, void *stream);
+int fds_lbc_adam(float *w, float *m1, float *m2, const float *g, int n, float lr, float b1,
+                 float b2, float eps, int step, void *stream);
+int fds_lbc_pack(const float *w, uint16_t *wb, uint16_t *wtb, int S, int Co, int K, int Ci,
+                 void *stream);
+int fds_lbc_gather(const float *frames, const int32_t *run, const int32_t *step,
+                   const int32_t *side, const float *scale, uint16_t *x, float *t, int B, int T,
+                   int D, int G, int L, const float *comp_scale, void *stream);
+int fds_lbc_band_rms(const float *frames, float *rms, int F, int D, int G, int L, void *stream);
+
+/* Diagnostics in float64 on the device: over a wx x wy window at (ax0, ay0) of context a and
+ * (bx0, by0) of context b, num = sum (a - b)^2 and den = sum b^2 of the current states. */
+int fds_window_sq_diff(fds_ctx *a, int comp_a, fds_ctx *b, int comp_b, int wx, int wy, int ax0,
+                       int ay0, int bx0, int by0, double *num, double *den);
+
+/* Device pointer of a component's recorded outputs ([rows][m] float64) of the last run, for
+ * in-process consumers that keep them on the GPU. */
+int fds_output_device_ptr(fds_ctx *ctx, int comp, double **ptr, int64_t *rows, int64_t *m);
+
 /* Calibration (no reference counterpart): copies `bytes` bytes device-to-device |reps| times
  * with a plain grid-stride streaming kernel (reps > 0) or an unrolled kernel with four loads in
  * flight per thread and non-temporal stores (reps < 0); returns read+write GB/s and

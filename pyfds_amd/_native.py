@@ -23,7 +23,11 @@ EXPORTS = ('fds_create', 'fds_destroy', 'fds_set_scalars', 'fds_set_coeff', 'fds
            'fds_set_state', 'fds_get_state', 'fds_set_boundaries', 'fds_set_outputs',
            'fds_run', 'fds_read_outputs', 'fds_set_tile', 'fds_tile_count', 'fds_tile_shape',
            'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_batch_create', 'fds_batch_run',
-           'fds_batch_last_run_ms', 'fds_batch_destroy', 'fds_last_error', 'fds_version')
+           'fds_batch_last_run_ms', 'fds_batch_destroy', 'fds_set_learned_boundary',
+           'fds_lbc_conv_fwd', 'fds_lbc_conv_bwd_data', 'fds_lbc_conv_bwd_weight',
+           'fds_lbc_reduce_weight', 'fds_lbc_mse', 'fds_lbc_adam', 'fds_lbc_pack',
+           'fds_lbc_gather', 'fds_lbc_band_rms', 'fds_window_sq_diff', 'fds_output_device_ptr',
+           'fds_last_error', 'fds_version')
 
 
 class NativeError(RuntimeError):
@@ -72,6 +76,20 @@ def load():
         'fds_batch_run': (ct.c_int, [_p, _i64, _i64, ct.c_int]),
         'fds_batch_last_run_ms': (ct.c_int, [_p, _dp]),
         'fds_batch_destroy': (ct.c_int, [_p]),
+        'fds_set_learned_boundary': (ct.c_int, [_p, _p, _p, ct.c_int, _p]),
+        'fds_lbc_conv_fwd': (ct.c_int, [_p, _p, _p, _p, _p] + [ct.c_int] * 5 + [_p]),
+        'fds_lbc_conv_bwd_data': (ct.c_int, [_p, _p, _p, _p, _p] + [ct.c_int] * 5 + [_p]),
+        'fds_lbc_conv_bwd_weight': (ct.c_int, [_p, _p, _p] + [ct.c_int] * 5 + [_p]),
+        'fds_lbc_reduce_weight': (ct.c_int, [_p, _p, _p] + [ct.c_int] * 3 + [_p]),
+        'fds_lbc_mse': (ct.c_int, [_p] * 7 + [ct.c_int] * 4 + [ct.c_float, _p]),
+        'fds_lbc_adam': (ct.c_int, [_p] * 4 + [ct.c_int] + [ct.c_float] * 4 + [ct.c_int, _p]),
+        'fds_lbc_pack': (ct.c_int, [_p] * 3 + [ct.c_int] * 4 + [_p]),
+        'fds_lbc_gather': (ct.c_int, [_p] * 7 + [ct.c_int] * 5 + [_p, _p]),
+        'fds_lbc_band_rms': (ct.c_int, [_p, _p] + [ct.c_int] * 4 + [_p]),
+        'fds_window_sq_diff': (ct.c_int, [_p, ct.c_int, _p, ct.c_int] + [ct.c_int] * 6
+                               + [_dp, _dp]),
+        'fds_output_device_ptr': (ct.c_int, [_p, ct.c_int, ct.POINTER(_p), ct.POINTER(_i64),
+                                             ct.POINTER(_i64)]),
         'fds_tile_count': (ct.c_int, []),
         'fds_tile_shape': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         'fds_last_error': (ct.c_char_p, []),

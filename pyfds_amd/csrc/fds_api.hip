@@ -14,6 +14,7 @@
 #include "../../include/pyfds_amd.h"
 #include "fds_common.h"
 #include "fds_kernels.h"
+#include "fds_lbc.h"
 #include "fds_step2d.h"
 #include "fds_step_nl.h"
 
@@ -143,6 +144,10 @@ struct fds_ctx {
   std::vector<hipGraph_t> graph_objs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;   // bracket the launches of the last run
   double last_run_ms = 0.0;
+  // learned absorbing boundary (caller-owned bf16 weights), applied after every step
+  const uint16_t *lbc_w1 = nullptr, *lbc_w2 = nullptr;
+  int lbc_layer = 0;
+  float lbc_cs[3] = {1.f, 1.f, 1.f};
 };
 
 struct fds_batch {
@@ -378,6 +383,25 @@ __global__ void k_advance(int64_t *dyn, int64_t by) {
 
 // Launch one step: reads state buffer `parity`, writes parity^1.
 template <typename T>
+void launch_lbc(fds_ctx *, const fds::StepArgs<T> &) {}
+
+template <>
+void launch_lbc<float>(fds_ctx *c, const fds::StepArgs<float> &a) {
+  fds::LbcStep st;
+  for (int k = 0; k < 3; ++k) {
+    st.prev[k] = a.in[k];
+    st.cur[k] = a.out[k];
+    st.cur_comp[k] = is_kahan(c) ? a.cout[k] : nullptr;
+    st.cs[k] = c->lbc_cs[k];
+  }
+  st.w1 = c->lbc_w1;
+  st.w2 = c->lbc_w2;
+  st.g = (int)c->nx;
+  st.l = c->lbc_layer;
+  fds::launch_lbc_step(st, c->stream);
+}
+
+template <typename T>
 int launch_step(fds_ctx *c, int parity, int k, bool store) {
   fds::StepArgs<T> a;
   fill_args<T>(c, a, parity);
@@ -386,6 +410,10 @@ int launch_step(fds_ctx *c, int parity, int k, bool store) {
   const int nb = fds::xcd_grid(c->ntiles);
   pick_launch(c)(&a, nb, lds_bytes(c), c->stream);
   HIP_TRY(hipGetLastError());
+  if (c->lbc_w1) {
+    launch_lbc<T>(c, a);
+    HIP_TRY(hipGetLastError());
+  }
   return 0;
 }
 
@@ -785,6 +813,32 @@ int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
   return 0;
 }
 
+int fds_set_learned_boundary(fds_ctx *c, const uint16_t *w1, const uint16_t *w2, int layer,
+                             const float *comp_scale) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!w1) {
+    c->lbc_w1 = c->lbc_w2 = nullptr;
+    touch(c);
+    return 0;
+  }
+  if (!w2 || !comp_scale) return fail(FDS_E_ARG, "weights and component scales are required");
+  if (c->family != kAcoustic || c->axi || c->one_d)
+    return fail(FDS_E_ARG, "the learned boundary is implemented for Acoustic2D");
+  if (c->elem != 4) return fail(FDS_E_ARG, "the learned boundary needs an FP32 form");
+  if (c->nx != c->ny) return fail(FDS_E_ARG, "the learned boundary needs a square grid");
+  if (layer != fds::kLbcDepth || c->nx < 2 * layer + fds::kLbcBand + 1)
+    return fail(FDS_E_ARG, "layer depth must be %d on a grid wider than %d", fds::kLbcDepth,
+                2 * fds::kLbcDepth + fds::kLbcBand);
+  for (int k = 0; k < 3; ++k)
+    if (!(comp_scale[k] > 0.f)) return fail(FDS_E_ARG, "component scales must be positive");
+  c->lbc_w1 = w1;
+  c->lbc_w2 = w2;
+  c->lbc_layer = layer;
+  for (int k = 0; k < 3; ++k) c->lbc_cs[k] = comp_scale[k];
+  touch(c);
+  return 0;
+}
+
 int fds_last_run_ms(fds_ctx *c, double *ms) {
   if (int rc = check_ctx(c)) return rc;
   if (!ms) return fail(FDS_E_ARG, "null output pointer");
@@ -1048,3 +1102,97 @@ int fds_batch_last_run_ms(fds_batch *b, double *ms) {
 }
 
 }  // extern "C"
+
+namespace {
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_window_sq_diff(const TA *__restrict__ a, int anx,
+                                                        const TB *__restrict__ b, int bnx,
+                                                        int wx, int wy, int ax0, int ay0,
+                                                        int bx0, int by0,
+                                                        double *__restrict__ out) {
+  __shared__ double r0[256], r1[256];
+  double num = 0.0, den = 0.0;
+  const int64_t n = (int64_t)wx * wy;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const int iy = (int)(i / wx), ix = (int)(i % wx);
+    const double va = (double)a[(int64_t)(ay0 + iy) * anx + ax0 + ix];
+    const double vb = (double)b[(int64_t)(by0 + iy) * bnx + bx0 + ix];
+    const double d = va - vb;
+    num += d * d;
+    den += vb * vb;
+  }
+  r0[threadIdx.x] = num;
+  r1[threadIdx.x] = den;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      r0[threadIdx.x] += r0[threadIdx.x + s];
+      r1[threadIdx.x] += r1[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = r0[0];
+    out[2 * blockIdx.x + 1] = r1[0];
+  }
+}
+
+const void *current_state(const fds_ctx *c, int comp) {
+  const Comp &cp = c->comp[comp];
+  const int b = (cp.buf[1].p && (cp.accumulated || c->family != kThermal || comp == 0)) ? c->parity : 0;
+  return cp.buf[b].p;
+}
+
+}  // namespace
+
+extern "C" int fds_window_sq_diff(fds_ctx *a, int comp_a, fds_ctx *b, int comp_b, int wx, int wy,
+                                  int ax0, int ay0, int bx0, int by0, double *num, double *den) {
+  if (int rc = check_ctx(a)) return rc;
+  if (!b || b->device != a->device) return fail(FDS_E_ARG, "contexts must share a device");
+  if (int rc = check_comp(a, comp_a)) return rc;
+  if (int rc = check_comp(b, comp_b)) return rc;
+  if (wx < 1 || wy < 1 || ax0 < 0 || ay0 < 0 || bx0 < 0 || by0 < 0 || ax0 + wx > a->nx ||
+      ay0 + wy > a->ny || bx0 + wx > b->nx || by0 + wy > b->ny)
+    return fail(FDS_E_INDEX, "window outside a grid");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  const int blocks = 512;
+  DevBuf part;
+  HIP_TRY(part.ensure(blocks * 2 * sizeof(double)));
+  const void *pa = current_state(a, comp_a), *pb = current_state(b, comp_b);
+  const int anx = (int)a->nx, bnx = (int)b->nx;
+  double *o = part.as<double>();
+  if (a->elem == 8 && b->elem == 8)
+    hipLaunchKernelGGL((k_window_sq_diff<double, double>), dim3(blocks), dim3(256), 0, a->stream,
+                       (const double *)pa, anx, (const double *)pb, bnx, wx, wy, ax0, ay0, bx0, by0, o);
+  else if (a->elem == 4 && b->elem == 4)
+    hipLaunchKernelGGL((k_window_sq_diff<float, float>), dim3(blocks), dim3(256), 0, a->stream,
+                       (const float *)pa, anx, (const float *)pb, bnx, wx, wy, ax0, ay0, bx0, by0, o);
+  else if (a->elem == 4)
+    hipLaunchKernelGGL((k_window_sq_diff<float, double>), dim3(blocks), dim3(256), 0, a->stream,
+                       (const float *)pa, anx, (const double *)pb, bnx, wx, wy, ax0, ay0, bx0, by0, o);
+  else
+    hipLaunchKernelGGL((k_window_sq_diff<double, float>), dim3(blocks), dim3(256), 0, a->stream,
+                       (const double *)pa, anx, (const float *)pb, bnx, wx, wy, ax0, ay0, bx0, by0, o);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> h(blocks * 2);
+  HIP_TRY(hipMemcpyAsync(h.data(), o, h.size() * sizeof(double), hipMemcpyDeviceToHost, a->stream));
+  HIP_TRY(hipStreamSynchronize(a->stream));
+  double s0 = 0, s1 = 0;
+  for (int i = 0; i < blocks; ++i) {
+    s0 += h[2 * i];
+    s1 += h[2 * i + 1];
+  }
+  *num = s0;
+  *den = s1;
+  return 0;
+}
+
+extern "C" int fds_output_device_ptr(fds_ctx *c, int comp, double **ptr, int64_t *rows, int64_t *m) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_comp(c, comp)) return rc;
+  *ptr = c->comp[comp].obuf.as<double>();
+  *rows = c->comp[comp].obuf_rows;
+  *m = (int64_t)c->comp[comp].out_cells.size();
+  return 0;
+}

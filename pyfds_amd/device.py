@@ -256,6 +256,27 @@ class DeviceRunner:
             comp._device = (self, k)
             comp._host_dirty = False
 
+    def run_recording(self, field, start, nsteps, cells):
+        """Run ``nsteps`` steps recording ``cells[k]`` (flat indices) of component k every
+        step; returns one float32 array [nsteps][len(cells[k])] per component.  The field's own
+        ``Output`` objects are not touched (they are re-synchronised on the next run)."""
+        self.sync(field)
+        for k, idx in enumerate(cells):
+            idx = nat.i64(idx)
+            nat.check(self.lib.fds_set_outputs(self.ctx, k, idx.size, nat.lptr(idx)))
+            self.out_cells[k] = None
+        nat.check(self.lib.fds_run(self.ctx, start, nsteps, int(self.chunk)))
+        out = []
+        for k, idx in enumerate(cells):
+            data = np.empty((nsteps, len(idx)))
+            nat.check(self.lib.fds_read_outputs(self.ctx, k, 0, nsteps, nat.dptr(data)))
+            out.append(data.astype(np.float32))
+        for k, comp in enumerate(self._components(field)):
+            if comp is not None:
+                comp._device = (self, k)
+                comp._host_dirty = False
+        return out
+
     def simulate(self, field, start, num_steps, progress):
         """``Field.simulate``'s loop; runs until the shortest boundary signal ends and then
         raises IndexError like the reference's ``signal[step]`` would."""
