@@ -24,6 +24,10 @@ Further sections (reported under ``batched`` and ``thermal2d``):
   1-4 seeded inclusions each (c 1000-6000 m/s, rho 800-8000 kg/m^3), dt at CFL 0.5 on each
   instance's own c_max, a 1 MHz burst source at each centre and 16 recorded points, FP32 and
   FP32-Kahan, one launch per step over all instances' tiles in HIP-graph chunks of 50.
+* learned_bc: Acoustic2D 1024x1024 FP32 interior + 16-cell learned layer (BF16-MFMA conv net,
+  61 440 parameters) trained on plain 3072x3072 FP32 runs with random Ricker sources and
+  inclusions (seeds 100..), judged on a held-out scene against the plain 3072^2 run:
+  reflected energy, the same small grid without the net (ablation), wall-clock of both methods.
 * thermal2d: Thermal2D 4096x4096, k in {0.6, 50} W/mK in seeded 256-cell blocks (seed 3),
   (16x16), rho c_p identical, Fourier number 0.2 on k_max, Gaussian hot spot (sigma 64 cells); FP64
   and FP32-Kahan over a bounded number of the 100 000 steps (per-step cost is constant), with
@@ -244,6 +248,55 @@ def cpu_baseline_thermal(steps):
     return f.num_points * steps / (time.perf_counter() - t0) / 1e9
 
 
+def learned_section(no_cpu, runs=3, iters=1500):
+    import numpy as np
+    from pyfds_amd.learned import LearnedBoundary, Scene
+    t0 = time.perf_counter()
+    lb = LearnedBoundary(n_int=1024, big=3072, seed=0)
+    shape = lb.generate(n_runs=runs, seed=100, after=1200)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist = lb.train(iters=iters, batch=32, lr=2e-3, log_every=max(1, iters // 5))
+    t_train = time.perf_counter() - t0
+    scene = Scene.random(np.random.default_rng(7), 1024)
+    res = lb.validate(scene, tolerance=0.01, after=1200)
+    steps = res['learned_steps']
+    macs = 4 * lb.G * (48 * 32 * 5 + 32 * 48 * 5)
+    out = dict(reflected_energy=round(res['learned'], 6),
+               reflected_energy_without_net=round(res['plain_small'], 6),
+               window_steps=list(res['window']), finite=res['finite'],
+               learned_s=round(res['learned_s'], 4), enlarged_s=round(res['enlarged_s'], 4),
+               speedup_vs_enlarged=round(res['speedup'], 3), timed_steps=steps,
+               learned_gcells=round(lb.G ** 2 * steps / res['learned_s'] / 1e9, 3),
+               enlarged_gcells=round(lb.big ** 2 * steps / res['enlarged_s'] / 1e9, 3),
+               params=int(lb.n_params), net_gflop_per_step=round(2 * macs / 1e9, 4),
+               training=dict(runs=runs, frames=list(shape), iters=iters, batch=32,
+                             loss=[(int(i), round(v, 5)) for i, v in hist],
+                             data_s=round(t_gen, 2), train_s=round(t_train, 2)),
+               uses=('learned' if lb.validated <= 0.01 else 'enlarged (fallback)'))
+    if not no_cpu:
+        out['cpu_baseline'] = dict(value=round(cpu_baseline_enlarged(lb, scene, 3), 5),
+                                   unit='Gcell-updates/s', cores=1, kind='port',
+                                   sample='numpy oracle, plain 3072x3072 enlarged run, 3 steps')
+    return out
+
+
+def cpu_baseline_enlarged(lb, scene, steps):
+    import oracle
+    f = lb.field(scene, lb.big, lb.off, steps)
+    mats = [(np.asarray(m.region.indices), dict(sound_velocity=m.materials[0].sound_velocity,
+                                                 density=m.materials[0].density,
+                                                 absorption_coef=0))
+            for m in f.material_regions]
+    o = oracle.PlainAcoustic2D(lb.big, lb.dx, steps, lb.dt, mats, ny=lb.big, dy=lb.dx)
+    b = f.pressure.boundaries[0]
+    o.boundaries['pressure'] = [dict(indices=b.region.indices, value=b.value, additive=True)]
+    o.assemble()
+    t0 = time.perf_counter()
+    o.simulate(steps)
+    return lb.big ** 2 * steps / (time.perf_counter() - t0) / 1e9
+
+
 def cpu_baseline(wl, steps):
     """The numpy oracle (plain FP64 method) on the same workload, single core."""
     import oracle
@@ -274,7 +327,8 @@ def main():
     ap.add_argument('--forms', default='fp64,fp32,fp32_kahan')
     ap.add_argument('--cpu-steps', type=int, default=20)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--sections', default='acoustic2d,batched,thermal')
+    ap.add_argument('--sections', default='acoustic2d,batched,thermal,learned')
+    ap.add_argument('--learned-iters', type=int, default=1500)
     ap.add_argument('--batch-steps', type=int, default=1000)
     ap.add_argument('--batch-forms', default='fp32,fp32_kahan')
     ap.add_argument('--thermal-steps', type=int, default=2000)
@@ -371,6 +425,8 @@ def main():
                                       unit='Gcell-updates/s', cores=1, kind='port',
                                       sample='numpy oracle, Thermal2D 4096x4096, 3 steps')
         line['thermal2d'] = th
+    if 'learned' in sections:
+        line['learned_bc'] = learned_section(args.no_cpu, iters=args.learned_iters)
     print(json.dumps(line), flush=True)
 
 

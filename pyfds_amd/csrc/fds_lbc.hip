@@ -267,13 +267,27 @@ __global__ __launch_bounds__(256) void k_lbc_step(LbcStep A) {
   constexpr int R = kLbcTaps / 2;
   constexpr int NX = kLbcSeg + 4 * R;   // 72 input positions
   constexpr int NH = 80;                // hidden rows (5 MFMA row tiles, 68 used)
+  constexpr int NW1 = kLbcHidden * kLbcTaps * kLbcCin;
+  constexpr int NW2 = kLbcCout * kLbcTaps * kLbcHidden;
   __shared__ __attribute__((aligned(16))) uint16_t xin[NX * kLbcCin];
   __shared__ __attribute__((aligned(16))) uint16_t hid[NH * kLbcHidden];
+  __shared__ __attribute__((aligned(16))) uint16_t sw1[NW1];
+  __shared__ __attribute__((aligned(16))) uint16_t sw2[NW2];
   __shared__ float outs[kLbcSeg * kLbcCout];
   const LbcStep &st = A;
   const int G = st.g, L = st.l;
   const int s = blockIdx.y, a0 = blockIdx.x * kLbcSeg;
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, m = l & 15, g = l >> 4;
+
+  // this side's weights into LDS first: one round trip instead of one per MFMA
+  {
+    const s4 *g1 = reinterpret_cast<const s4 *>(st.w1 + (size_t)s * NW1);
+    const s4 *g2 = reinterpret_cast<const s4 *>(st.w2 + (size_t)s * NW2);
+    s4 *l1 = reinterpret_cast<s4 *>(sw1);
+    s4 *l2 = reinterpret_cast<s4 *>(sw2);
+    for (int e = tid; e < NW1 / 4; e += 256) l1[e] = g1[e];
+    for (int e = tid; e < NW2 / 4; e += 256) l2[e] = g2[e];
+  }
 
   for (int e = tid; e < NX * kLbcCin; e += 256) {
     const int row = e / kLbcCin, ci = e % kLbcCin;
@@ -289,7 +303,7 @@ __global__ __launch_bounds__(256) void k_lbc_step(LbcStep A) {
   __syncthreads();
 
   // layer 1: hid[hp][co], hidden position a = a0 - R + hp (zero outside the strip)
-  const uint16_t *w1 = st.w1 + (size_t)s * kLbcHidden * kLbcTaps * kLbcCin;
+  const uint16_t *w1 = sw1;
   for (int t = wave; t < (NH / 16) * (kLbcHidden / 16); t += 4) {
     const int mt = t / (kLbcHidden / 16), nt = t % (kLbcHidden / 16);
     f4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -310,7 +324,7 @@ __global__ __launch_bounds__(256) void k_lbc_step(LbcStep A) {
   __syncthreads();
 
   // layer 2: outs[op][co] at a = a0 + op
-  const uint16_t *w2 = st.w2 + (size_t)s * kLbcCout * kLbcTaps * kLbcHidden;
+  const uint16_t *w2 = sw2;
   for (int t = wave; t < (kLbcSeg / 16) * (kLbcCout / 16); t += 4) {
     const int mt = t / (kLbcCout / 16), nt = t % (kLbcCout / 16);
     f4 acc = {0.f, 0.f, 0.f, 0.f};
