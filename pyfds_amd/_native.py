@@ -53,6 +53,13 @@ def load():
         raise NativeError('pyfds_amd HIP library not built (%s missing); run '
                           '`python -m pyfds_amd.build`' % LIB_PATH)
     try:
+        # PyTorch-ROCm ships its own libamdhip64 under the same soname as /opt/rocm's.  Whichever
+        # is loaded first serves the whole process, and torch reports no GPU when it finds the
+        # other one already loaded; so when torch is installed, let it load first and share it.
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    try:
         lib = ct.CDLL(LIB_PATH)
     except OSError as exc:
         raise NativeError('cannot load %s: %s' % (LIB_PATH, exc)) from exc
