@@ -137,11 +137,17 @@ def batched_fields(ns, form, n_inst=256, seed=2, steps=1000):
     return fields
 
 
-def time_batched(ns, form, steps, warmup, chunk=50):
+def time_batched(ns, form, steps, warmup, chunk=50, tune=True):
     import torch
     from pyfds_amd.batch import BatchRunner
-    fields = batched_fields(ns, form, steps=steps + warmup)
-    batch = BatchRunner(fields, chunk=chunk)
+    from pyfds_amd import autotune
+    fields = batched_fields(ns, form, steps=steps + warmup + 64)
+    table = None
+    tile = None
+    if tune:
+        tile, chunk, table = autotune.tune_batch(fields, steps=10, chunks=(chunk,),
+                                                 use_cache=False)
+    batch = BatchRunner(fields, tile=tile, chunk=chunk)
     if warmup:
         batch.simulate(warmup)
     torch.cuda.synchronize()
@@ -153,7 +159,8 @@ def time_batched(ns, form, steps, warmup, chunk=50):
     kernel_s = batch.last_run_ms() / 1e3 / steps
     achieved = BYTES_PER_CELL[form] * cells / kernel_s / 1e9
     res = dict(gcells=cells * steps / wall / 1e9, ms_per_step=wall / steps * 1e3,
-               kernel_us=kernel_s * 1e6, achieved_gbs=achieved, cells=cells)
+               kernel_us=kernel_s * 1e6, achieved_gbs=achieved, cells=cells, tile=tile,
+               tune_us_per_step=table)
     batch.close()
     return res, fields
 
@@ -393,7 +400,8 @@ def main():
                           kernel_us=round(r['kernel_us'], 2),
                           achieved_gbs=round(r['achieved_gbs'], 1),
                           roofline_frac=round(r['achieved_gbs'] / HBM_PEAK_GBS, 4),
-                          bytes_per_cell=BYTES_PER_CELL[f], cells=r['cells'])
+                          bytes_per_cell=BYTES_PER_CELL[f], cells=r['cells'],
+                          tile=r['tile'], autotune_us_per_step=r['tune_us_per_step'])
         bat['config'] = ('256 Acoustic2D/Acoustic3DAxi instances, sides 128-1024, %d steps, '
                          'HIP-graph chunks of 50' % args.batch_steps)
         if not args.no_cpu:

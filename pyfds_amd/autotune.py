@@ -1,0 +1,148 @@
+"""Autotuner: picks the step kernels' tile shape and the HIP-graph chunk length by timing.
+
+For a field (or a batch of fields) it times a few steps of every admissible tile shape of the
+compiled tile table (``fds_tile_count``/``fds_tile_shape``) and graph-chunk length with the
+GPU's own clock (HIP events around the launches, ``fds_last_run_ms`` /
+``fds_batch_last_run_ms``), keeps the fastest and caches the choice in a JSON file keyed by
+device, model, form, grid and lossy/Axi variant.  Timing runs advance a scratch copy of the
+state: the caller's field is restored to its values before tuning.
+"""
+import ctypes as ct
+import json
+import os
+
+import numpy as np
+
+from . import _native as nat
+
+CACHE = os.environ.get('PYFDS_AMD_TUNE_CACHE',
+                       os.path.join(os.path.expanduser('~'), '.cache', 'pyfds_amd_tune.json'))
+
+
+def _device_name():
+    try:
+        import torch
+        return torch.cuda.get_device_name(0)
+    except Exception:
+        return 'hip-device'
+
+
+def _load_cache():
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save_cache(cache):
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        with open(CACHE, 'w') as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+    except OSError:
+        pass
+
+
+def _key(fields):
+    f = fields[0]
+    lossy = any(getattr(x, '_coefs', {}).get(4) is not None for x in fields)
+    cells = sum(x.num_points for x in fields)
+    return '|'.join(str(v) for v in (_device_name(), type(f).__name__, f.form, len(fields),
+                                     cells, f._device_grid(), lossy))
+
+
+def _snapshot(fields):
+    out = []
+    for f in fields:
+        comps = [getattr(f, n) for n in f._component_names if n]
+        out.append(([np.array(c.values, copy=True) for c in comps],
+                    [[[list(s) for s in o.signals] for o in c.outputs] for c in comps],
+                    f.step))
+    return out
+
+
+def _restore(fields, snap):
+    for f, (vals, sigs, step) in zip(fields, snap):
+        comps = [getattr(f, n) for n in f._component_names if n]
+        for c, v, s in zip(comps, vals, sigs):
+            c.values = v
+            for o, so in zip(c.outputs, s):
+                o.signals = [list(x) for x in so]
+        f.step = step
+
+
+def tune_field(fld, steps=40, chunks=(50,), use_cache=True):
+    """Choose ``fld.tile`` / ``fld.graph_chunk`` for one field; returns (tile, chunk, table)
+    where table maps (tile, chunk) -> microseconds per step."""
+    return _tune([fld], steps, chunks, use_cache, batch=False)
+
+
+def tune_batch(fields, steps=20, chunks=(50,), use_cache=True):
+    """Choose one tile shape and chunk for a ``BatchRunner`` over ``fields``."""
+    return _tune(fields, steps, chunks, use_cache, batch=True)
+
+
+def _tune(fields, steps, chunks, use_cache, batch):
+    lib = nat.load()
+    key = _key(fields) + ('|batch' if batch else '')
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        tile, chunk = cache[key]['tile'], cache[key]['chunk']
+        _apply(fields, tile, chunk)
+        return tile, chunk, cache[key]['table']
+    for f in fields:
+        if not f.matrices_assembled:
+            f.assemble_matrices()
+    snap = _snapshot(fields)
+    shapes = nat.tile_shapes()
+    one_row = fields[0]._device_grid()[1] == 1
+    cand = [i for i, (tx, ty) in enumerate(shapes) if (ty == 1) == one_row]
+    table = {}
+    for tile in cand:
+        for chunk in chunks:
+            try:
+                if batch:
+                    from .batch import BatchRunner
+                    br = BatchRunner(fields, tile=tile, chunk=chunk)
+                    br.simulate(min(steps, 5))
+                    br.simulate(steps)
+                    ms = br.last_run_ms()
+                    br.close()
+                else:
+                    f = fields[0]
+                    r = f._runner()
+                    r.sync(f)
+                    nat.check(lib.fds_set_tile(r.ctx, tile))
+                    r.chunk = chunk
+                    r.run_steps(f, f.step, min(steps, 5))
+                    r.run_steps(f, f.step, steps)
+                    out = ct.c_double()
+                    nat.check(lib.fds_last_run_ms(r.ctx, ct.byref(out)))
+                    ms = out.value
+            except ValueError:        # tile does not fit this field's LDS budget
+                continue
+            finally:
+                _restore(fields, snap)
+            table['%d,%d' % (tile, chunk)] = ms * 1e3 / steps
+    if not table:
+        raise RuntimeError('no tile shape could be timed')
+    best = min(table, key=table.get)
+    tile, chunk = (int(v) for v in best.split(','))
+    _apply(fields, tile, chunk)
+    cache[key] = dict(tile=tile, chunk=chunk, table=table)
+    if use_cache:
+        _save_cache(cache)
+    return tile, chunk, table
+
+
+def _apply(fields, tile, chunk):
+    lib = nat.load()
+    for f in fields:
+        f.tile = tile
+        f.graph_chunk = chunk
+        r = f.__dict__.get('_device_runner')
+        if r is not None:
+            r.tile, r.chunk = tile, chunk
+            if r.ctx is not None:
+                nat.check(lib.fds_set_tile(r.ctx, tile))
