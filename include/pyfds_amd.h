@@ -137,11 +137,15 @@ int fds_last_run_ms(fds_ctx *ctx, double *ms);
  * dst[nrows][m] (row-major, one row per step). */
 int fds_read_outputs(fds_ctx *ctx, int comp, int64_t row0, int64_t nrows, double *dst);
 
-/* Tile shape of the 2D step kernels (autotuner hook): index into the compiled tile table,
- * or -1 for the default.  fds_tile_count reports the table size, fds_tile_shape its entries. */
+/* Tile shape of the 2D step kernels (autotuner hook): index into the compiled shape table,
+ * or -1 for the default.  fds_tile_count reports the table size, fds_tile_shape its entries
+ * (tx columns x ty rows).  fds_tile_engine returns 0 for an LDS-tile shape and 1 for a
+ * row-streaming shape (a workgroup of tx threads marching down a strip of ty rows); streaming
+ * shapes serve the lossless 2D models, other fields given one run on the default tile shape. */
 int fds_set_tile(fds_ctx *ctx, int tile_index);
 int fds_tile_count(void);
 int fds_tile_shape(int tile_index, int *tx, int *ty);
+int fds_tile_engine(int tile_index);
 
 /* Batched sweeps (no reference counterpart: pyfds runs one field per Python loop).  A batch
  * groups contexts that are fully set up (coefficients committed, state, boundaries, outputs)

@@ -22,7 +22,7 @@ NUM_COEFF, NUM_SCALARS = 8, 8
 EXPORTS = ('fds_create', 'fds_destroy', 'fds_set_scalars', 'fds_set_coeff', 'fds_commit',
            'fds_set_state', 'fds_get_state', 'fds_set_boundaries', 'fds_set_outputs',
            'fds_run', 'fds_read_outputs', 'fds_set_tile', 'fds_tile_count', 'fds_tile_shape',
-           'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_batch_create', 'fds_batch_run',
+           'fds_tile_engine', 'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_batch_create', 'fds_batch_run',
            'fds_batch_last_run_ms', 'fds_batch_destroy', 'fds_set_learned_boundary',
            'fds_lbc_conv_fwd', 'fds_lbc_conv_bwd_data', 'fds_lbc_conv_bwd_weight',
            'fds_lbc_reduce_weight', 'fds_lbc_mse', 'fds_lbc_adam', 'fds_lbc_pack',
@@ -98,6 +98,7 @@ def load():
         'fds_output_device_ptr': (ct.c_int, [_p, ct.c_int, ct.POINTER(_p), ct.POINTER(_i64),
                                              ct.POINTER(_i64)]),
         'fds_tile_count': (ct.c_int, []),
+        'fds_tile_engine': (ct.c_int, [ct.c_int]),
         'fds_tile_shape': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         'fds_last_error': (ct.c_char_p, []),
         'fds_version': (ct.c_char_p, []),
@@ -150,3 +151,10 @@ def tile_shapes():
         check(lib.fds_tile_shape(i, ct.byref(tx), ct.byref(ty)))
         out.append((tx.value, ty.value))
     return out
+
+
+def tile_engines():
+    """Per shape index: 'tile' (LDS tiles) or 'stream' (row streaming, lossless 2D only)."""
+    lib = load()
+    return ['stream' if lib.fds_tile_engine(i) == 1 else 'tile'
+            for i in range(lib.fds_tile_count())]

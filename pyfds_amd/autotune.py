@@ -95,9 +95,11 @@ def _tune(fields, steps, chunks, use_cache, batch):
         if not f.matrices_assembled:
             f.assemble_matrices()
     snap = _snapshot(fields)
-    shapes = nat.tile_shapes()
+    shapes, engines = nat.tile_shapes(), nat.tile_engines()
     one_row = fields[0]._device_grid()[1] == 1
-    cand = [i for i, (tx, ty) in enumerate(shapes) if (ty == 1) == one_row]
+    lossy = any(getattr(x, '_coefs', {}).get(4) is not None for x in fields)
+    cand = [i for i, (tx, ty) in enumerate(shapes)
+            if (ty == 1) == one_row and not (engines[i] == 'stream' and (lossy or one_row))]
     table = {}
     for tile in cand:
         for chunk in chunks:

@@ -131,7 +131,9 @@ struct fds_ctx {
   double s[FDS_NUM_SCALARS] = {};
   bool lossy = false, iso_b = true, iso_k = true;
   bool committed = false;
-  int tile = -1;
+  int tile = -1;          // requested shape index (-1: default)
+  int shape = 0;          // shape index in use (tile table, then the streaming table)
+  bool streaming = false; // row-streaming engine in use
   int tx = 0, ty = 0, tiles_x = 0, tiles_y = 0, ntiles = 0;
   bool events_dirty = true;
   DevBuf dyn;
@@ -156,6 +158,8 @@ struct fds_batch {
   size_t elem = 8;
   hipStream_t stream = nullptr;
   const fds::KernelSet *ks = nullptr;
+  int shape = 0;
+  bool streaming = false;
   int tx = 0, ty = 0, ntiles = 0;
   size_t lds = 0;
   DevBuf descs[2][2];          // [parity][store]
@@ -254,15 +258,37 @@ void build_output_events(const fds_ctx *c, int k, Events &ev) {
   for (int t = 0; t < c->ntiles; ++t) ev.ptr[t + 1] += ev.ptr[t];
 }
 
+// Shape index -> (tx, ty); indices from kNumTiles on name the streaming shapes.
+void shape_of(int t, int *tx, int *ty) {
+  if (t < fds::kNumTiles) {
+    *tx = fds::kTileShapes[t][0];
+    *ty = fds::kTileShapes[t][1];
+  } else {
+    *tx = fds::kStreamShapes[t - fds::kNumTiles][0];
+    *ty = fds::kStreamShapes[t - fds::kNumTiles][1];
+  }
+}
+
+// The streaming engine covers the lossless 2D models; everything else runs on tiles.
+bool can_stream(const fds_ctx *c) {
+  return c->family != kNonlinear && c->ny > 1 && !c->lossy;
+}
+
 int choose_tiles(fds_ctx *c) {
+  c->streaming = false;
   if (c->family == kNonlinear) {
     c->tx = fds::kNlTile;
     c->ty = 1;
+    c->shape = 0;
   } else {
     int t = c->tile;
-    if (t < 0) t = (c->ny == 1) ? fds::kTileRow : fds::kDefaultTile2D;
-    c->tx = fds::kTileShapes[t][0];
-    c->ty = fds::kTileShapes[t][1];
+    if (t < 0) t = (c->ny == 1) ? fds::kTileRow : (c->lossy ? fds::kDefaultTile2D
+                                                             : fds::kDefaultLossless2D);
+    if (t >= fds::kNumTiles && !can_stream(c))
+      t = (c->ny == 1) ? fds::kTileRow : fds::kDefaultTile2D;
+    c->shape = t;
+    c->streaming = t >= fds::kNumTiles;
+    shape_of(t, &c->tx, &c->ty);
   }
   c->tiles_x = (int)((c->nx + c->tx - 1) / c->tx);
   c->tiles_y = (int)((c->ny + c->ty - 1) / c->ty);
@@ -275,7 +301,7 @@ int choose_tiles(fds_ctx *c) {
 }
 
 size_t lds_bytes(const fds_ctx *c) {
-  if (c->family == kNonlinear) return 0;
+  if (c->family == kNonlinear || c->streaming) return 0;   // static LDS only
   const bool sep_k = c->axi || !c->iso_k;
   fds::Lds2D L = fds::lds2d_layout(c->tx, c->ty, !c->iso_b, sep_k, is_kahan(c));
   return (size_t)L.total * c->elem;
@@ -361,12 +387,16 @@ void fill_args(const fds_ctx *c, fds::StepArgs<T> &a, int parity) {
   for (int s = 0; s < FDS_NUM_SCALARS; ++s) a.s[s] = c->s[s];
   a.dyn = c->dyn.as<const int64_t>();
   a.flags = (c->iso_b ? fds::kFlagIsoB : 0) | (c->iso_k ? fds::kFlagIsoK : 0);
+  a.strip_h = c->ty;
 }
 
 fds::LaunchFn pick_launch(const fds_ctx *c) {
-  int t = 0;
-  for (int i = 0; i < fds::kNumTiles; ++i)
-    if (fds::kTileShapes[i][0] == c->tx && fds::kTileShapes[i][1] == c->ty) t = i;
+  const int t = c->shape;
+  if (c->streaming) {
+    const int s = t - fds::kNumTiles;
+    return c->family == kThermal ? c->ks->stream_thermal[s][c->axi ? 1 : 0]
+                                 : c->ks->stream_acoustic[s][c->axi ? 1 : 0];
+  }
   switch (c->family) {
     case kAcoustic: return c->ks->acoustic[t][c->axi ? 1 : 0][c->lossy ? 1 : 0];
     case kThermal: return c->ks->thermal[t][c->axi ? 1 : 0];
@@ -480,12 +510,18 @@ const char *fds_last_error(void) { return g_last_error.c_str(); }
 
 const char *fds_version(void) { return "pyfds_amd 0.1 (gfx950)"; }
 
-int fds_tile_count(void) { return fds::kNumTiles; }
+int fds_tile_count(void) { return fds::kNumTiles + fds::kNumStream; }
+
+int fds_tile_engine(int i) {
+  if (i < 0 || i >= fds::kNumTiles + fds::kNumStream)
+    return fail(FDS_E_ARG, "tile index %d out of range", i);
+  return i >= fds::kNumTiles ? 1 : 0;
+}
 
 int fds_tile_shape(int i, int *tx, int *ty) {
-  if (i < 0 || i >= fds::kNumTiles) return fail(FDS_E_ARG, "tile index %d out of range", i);
-  *tx = fds::kTileShapes[i][0];
-  *ty = fds::kTileShapes[i][1];
+  if (i < 0 || i >= fds::kNumTiles + fds::kNumStream)
+    return fail(FDS_E_ARG, "tile index %d out of range", i);
+  shape_of(i, tx, ty);
   return 0;
 }
 
@@ -495,8 +531,10 @@ int fds_create(fds_ctx **out, int model, int form, int64_t nx, int64_t ny, int d
   *out = nullptr;
   if (form < FDS_FP64 || form > FDS_FP32_KAHAN) return fail(FDS_E_ARG, "unknown form %d", form);
   if (nx < 1 || ny < 1) return fail(FDS_E_ARG, "grid must be at least 1x1");
-  if (nx * ny > (int64_t)std::numeric_limits<int32_t>::max())
-    return fail(FDS_E_ARG, "grid of %lld cells exceeds 2^31", (long long)(nx * ny));
+  // kernels index cells with 32-bit ints, including look-ahead rows past the last strip
+  if (nx * (ny + 40) + 1024 > (int64_t)std::numeric_limits<int32_t>::max())
+    return fail(FDS_E_ARG, "grid of %lld cells exceeds the 32-bit cell index",
+                (long long)(nx * ny));
   auto *c = new fds_ctx();
   c->model = model;
   c->form = form;
@@ -633,6 +671,8 @@ int fds_commit(fds_ctx *c) {
   c->iso_b = !c->one_d && !c->coef_set[FDS_C_BY];   // (thermal: QY)
   c->iso_k = !c->one_d && !c->coef_set[FDS_C_KY];   // (thermal: GY)
   if (c->family != kNonlinear) {
+    // the engine depends on the lossy flag just derived
+    if (int rc = choose_tiles(c)) return rc;
     const size_t lds = lds_bytes(c);
     if (lds > 65536) {
       // fall back to the default shape, which always fits
@@ -741,7 +781,8 @@ int fds_set_outputs(fds_ctx *c, int comp, int64_t m, const int64_t *cells) {
 
 int fds_set_tile(fds_ctx *c, int t) {
   if (int rc = check_ctx(c)) return rc;
-  if (t < -1 || t >= fds::kNumTiles) return fail(FDS_E_ARG, "tile index %d out of range", t);
+  if (t < -1 || t >= fds::kNumTiles + fds::kNumStream)
+    return fail(FDS_E_ARG, "tile index %d out of range", t);
   if (c->family == kNonlinear) return 0;
   const int old = c->tile;
   c->tile = t;
@@ -934,10 +975,10 @@ int batch_launch(fds_batch *b, int rel_parity, int k, bool store) {
   L.inst_base = b->inst_base.as<const int32_t>();
   L.ntiles = b->ntiles;
   L.k = k;
-  int t = 0;
-  for (int i = 0; i < fds::kNumTiles; ++i)
-    if (fds::kTileShapes[i][0] == b->tx && fds::kTileShapes[i][1] == b->ty) t = i;
-  b->ks->batch[t](L, fds::xcd_grid(b->ntiles), b->lds, b->stream);
+  if (b->streaming)
+    b->ks->stream_batch[b->shape - fds::kNumTiles](L, fds::xcd_grid(b->ntiles), 0, b->stream);
+  else
+    b->ks->batch[b->shape](L, fds::xcd_grid(b->ntiles), b->lds, b->stream);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1003,8 +1044,16 @@ int fds_batch_create(fds_batch **out, fds_ctx *const *ctxs, int n, int tile) {
   hipError_t e = hipSetDevice(b->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete b; return fail(FDS_E_HIP, "stream: %s", hipGetErrorString(e)); }
-  // one tile shape for every member (re-derives each member's events)
-  const int t = tile < 0 ? fds::kDefaultTile2D : tile;
+  // one tile shape for every member (re-derives each member's events); the streaming
+  // engine only when it covers every member
+  bool all_stream = true;
+  for (auto *c : b->ctxs) all_stream = all_stream && can_stream(c);
+  int t = tile < 0 ? (all_stream ? fds::kDefaultLossless2D : fds::kDefaultTile2D) : tile;
+  if (t >= fds::kNumTiles + fds::kNumStream || t < 0) {
+    fds_batch_destroy(b);
+    return fail(FDS_E_ARG, "tile index %d out of range", t);
+  }
+  if (t >= fds::kNumTiles && !all_stream) t = fds::kDefaultTile2D;
   std::vector<int32_t> tinst, base;
   size_t lds = 0;
   for (int i = 0; i < n; ++i) {
@@ -1020,8 +1069,9 @@ int fds_batch_create(fds_batch **out, fds_ctx *const *ctxs, int n, int tile) {
     fds_batch_destroy(b);
     return fail(FDS_E_ARG, "tile %d needs more than 64 KiB of LDS for some member", t);
   }
-  b->tx = fds::kTileShapes[t][0];
-  b->ty = fds::kTileShapes[t][1];
+  b->shape = t;
+  b->streaming = t >= fds::kNumTiles;
+  shape_of(t, &b->tx, &b->ty);
   b->lds = lds;
   b->ntiles = (int)tinst.size();
   if (upload(b->tile_inst, tinst, b->stream) != hipSuccess ||

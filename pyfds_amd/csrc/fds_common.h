@@ -51,6 +51,7 @@ struct StepArgs {
   int32_t k;            // step index within the launch sequence
   int32_t flags;        // bit0: store assigned components (thermal flux) this step
   int32_t steps;        // resident 1D kernels: steps to run in this launch
+  int32_t strip_h;      // row-streaming kernels: rows per strip
 };
 
 // Map workgroup b to a tile so that each XCD (b % 8) gets one contiguous band of tiles:

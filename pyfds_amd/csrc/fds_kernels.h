@@ -10,8 +10,14 @@ namespace fds {
 constexpr int kNumTiles = 6;
 constexpr int kTileShapes[kNumTiles][2] = {
     {64, 16}, {64, 8}, {128, 8}, {256, 4}, {128, 4}, {256, 1}};
-constexpr int kDefaultTile2D = 0;
+constexpr int kDefaultTile2D = 1;       // default of the tile engine (lossy models)
+constexpr int kDefaultLossless2D = 1;   // default for lossless 2D models (either engine)
 constexpr int kTileRow = 5;
+
+// Row-streaming engine (lossless 2D models): TX threads = columns, strips of TY rows.  Shape
+// indices continue the tile table: index kNumTiles + i is kStreamShapes[i].
+constexpr int kNumStream = 5;
+constexpr int kStreamShapes[kNumStream][2] = {{256, 16}, {256, 32}, {128, 16}, {128, 32}, {256, 8}};
 
 using LaunchFn = void (*)(const void *args, int nblocks, size_t lds_bytes, hipStream_t s);
 
@@ -32,6 +38,9 @@ struct KernelSet {
   LaunchFn thermal[kNumTiles][2];       // [tile][axi]
   LaunchFn nonlinear;
   BatchFn batch[kNumTiles];
+  LaunchFn stream_acoustic[kNumStream][2];   // [shape][axi]
+  LaunchFn stream_thermal[kNumStream][2];
+  BatchFn stream_batch[kNumStream];
 };
 
 const KernelSet &kernels_f64();

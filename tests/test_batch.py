@@ -60,3 +60,50 @@ def test_batch_rejects_mixed_forms():
     b.form = 'fp32'
     with pytest.raises(ValueError):
         BatchRunner([a, b])
+
+
+LOSSLESS = ['a2d_main', 'axi_lossless', 't2d_blocks', 't2d_aniso', 'bounds_semantics', 'taxi',
+            'a2d_wrap']
+
+
+def _stream_index(slot):
+    return [i for i, e in enumerate(pyfds_amd._native.tile_engines()) if e == 'stream'][slot]
+
+
+@pytest.mark.parametrize('slot', [0, 2])
+def test_fp64_stream_batch_matches_oracle(slot):
+    """A batch of lossless members on the row-streaming engine."""
+    members = []
+    for name in LOSSLESS:
+        scene = dict(scenes.SCENES[name](), plan=PLAN)
+        members.append((scene, scenes.build_field(pyfds_amd, pyfds_amd.regions, scene)))
+    batch = BatchRunner([f for _, f in members], tile=_stream_index(slot), chunk=5)
+    for _, n in PLAN:
+        batch.simulate(n)
+    for scene, fld in members:
+        print('member', scene['name'], flush=True)
+        ref = scenes.run_oracle(scene)
+        got = scenes.collect(fld, scene)
+        for k in ref:
+            np.testing.assert_array_equal(got[k], ref[k], err_msg='%s %s' % (scene['name'], k))
+
+
+def test_kahan_stream_batch_equals_single_runs():
+    tile = _stream_index(0)
+    members = []
+    for name in LOSSLESS:
+        scene = dict(scenes.SCENES[name](), plan=PLAN)
+        fld = scenes.build_field(pyfds_amd, pyfds_amd.regions, scene)
+        fld.form = 'fp32_kahan'
+        members.append((scene, fld))
+    batch = BatchRunner([f for _, f in members], tile=tile, chunk=4)
+    for _, n in PLAN:
+        batch.simulate(n)
+    for scene, fld in members:
+        single = scenes.build_field(pyfds_amd, pyfds_amd.regions, scene)
+        single.form = 'fp32_kahan'
+        single.tile = tile
+        scenes.run_plan(single, scene)
+        a, b = scenes.collect(fld, scene), scenes.collect(single, scene)
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k], err_msg='%s %s' % (scene['name'], k))

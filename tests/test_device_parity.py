@@ -91,6 +91,61 @@ def test_fp64_tile_shapes_bit_identical(tile):
             np.testing.assert_array_equal(res[k], gold[k], err_msg=k)
 
 
+# Row-streaming engine: shape indices after the LDS-tile table (lossless 2D models only; lossy
+# fields asked for a streaming shape run on the default tile shape).
+def _stream_indices():
+    return [i for i, e in enumerate(pyfds_amd._native.tile_engines()) if e == 'stream']
+
+
+LOSSLESS_2D = ['a2d_main', 'a2d_wrap', 'axi_lossless', 'bounds_semantics', 't2d_aniso',
+               't2d_blocks', 'taxi']
+
+
+@pytest.mark.parametrize('slot', range(5))
+@pytest.mark.parametrize('name', LOSSLESS_2D + ['a2d_lossy', 'axi_lossy'])
+def test_fp64_stream_shapes_match_reference_golden(name, slot):
+    idx = _stream_indices()
+    if slot >= len(idx):
+        pytest.skip('fewer streaming shapes compiled')
+    scene = scenes.SCENES[name]()
+    gold = np.load(os.path.join(GOLDEN, name + '.npz'))
+    print('scene', name, 'shape', idx[slot], flush=True)
+    res = run_device(scene, tile=idx[slot])
+    for k in gold.files:
+        if k != 'fingerprint':
+            np.testing.assert_array_equal(res[k], gold[k], err_msg='%s %s' % (name, k))
+
+
+@pytest.mark.parametrize('form', ['fp32', 'fp32_kahan'])
+@pytest.mark.parametrize('name', LOSSLESS_2D)
+def test_fp32_stream_engine_equals_tile_engine(name, form):
+    """Both engines evaluate the same expressions in the same order, so the FP32 forms agree
+    bit for bit as well."""
+    scene = scenes.SCENES[name]()
+    print('scene', name, form, flush=True)
+    a = run_device(scene, form=form, tile=1)
+    for idx in _stream_indices()[:2]:
+        b = run_device(scene, form=form, tile=idx)
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k], err_msg='%s %s %d' % (name, k, idx))
+
+
+@pytest.mark.parametrize('model,nx,ny,slot', [('Acoustic2D', 256, 200, 0),
+                                              ('Acoustic2D', 300, 75, 2),
+                                              ('Acoustic3DAxi', 300, 130, 1),
+                                              ('Acoustic3DAxi', 513, 171, 3),
+                                              ('Acoustic2D', 130, 101, 4)])
+def test_fp64_stream_multi_strip_grids_match_oracle_bitwise(model, nx, ny, slot):
+    """Partial column blocks at the right edge and partial strips at the bottom."""
+    idx = _stream_indices()[slot]
+    scene = _big_acoustic(nx, ny, 120, model=model)
+    ref = scenes.run_oracle(scene)
+    print('grid', model, nx, ny, 'shape', idx, flush=True)
+    res = run_device(scene, tile=idx)
+    for k in ref:
+        np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
+
+
 @pytest.mark.parametrize('chunk', [0, 1, 7, 50])
 def test_graph_chunks_bit_identical(chunk):
     """HIP-graph replayed step chunks (and plain launches) give identical results."""
@@ -137,8 +192,9 @@ def test_fp64_multi_tile_grids_match_oracle_bitwise(model, mu, nx, ny):
         np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
 
 
+@pytest.mark.parametrize('tile', [None, 'stream0', 'stream3'])
 @pytest.mark.parametrize('model', ['Thermal2D', 'Thermal3DAxi'])
-def test_fp64_thermal_multi_tile_matches_oracle_bitwise(model):
+def test_fp64_thermal_multi_tile_matches_oracle_bitwise(model, tile):
     nx, ny, dx = 190, 150, 1e-3
     rng = np.random.default_rng(5)
     scene = dict(
@@ -157,6 +213,7 @@ def test_fp64_thermal_multi_tile_matches_oracle_bitwise(model):
                  'heat_flux_y': [[nx * 70 + 3]]},
         plan=[('simulate', 77), ('simulate', 73)])
     ref = scenes.run_oracle(scene)
-    res = run_device(scene)
+    attrs = {} if tile is None else dict(tile=_stream_indices()[int(tile[6:])])
+    res = run_device(scene, **attrs)
     for k in ref:
         np.testing.assert_array_equal(res[k], ref[k], err_msg=k)

@@ -31,8 +31,15 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_tile_table():
-    shapes = _native.tile_shapes()
-    assert (64, 16) in shapes and shapes[-1] == (256, 1)
+    shapes, engines = _native.tile_shapes(), _native.tile_engines()
+    assert (64, 16) in shapes and (256, 1) in shapes
+    assert len(engines) == len(shapes) and 'stream' in engines
+    # LDS-tile shapes come first, the streaming shapes after them
+    first = engines.index('stream')
+    assert all(e == 'tile' for e in engines[:first]) and all(e == 'stream' for e in engines[first:])
+    assert all(ty > 1 for _, ty in shapes[first:])
+    lib = _native.load()
+    assert lib.fds_tile_engine(len(shapes)) < 0 and lib.fds_tile_engine(-1) < 0
 
 
 def test_simulation_without_gpu_fails_loudly():
