@@ -80,10 +80,19 @@ def build_field(ns, wl, form='fp64', t_samples=None):
     return fld
 
 
+def tuned(fld):
+    """Pick the shape / strip height / graph chunk by timing (state restored afterwards)."""
+    from pyfds_amd import autotune
+    tile, chunk, table = autotune.tune_field(fld, steps=40, use_cache=False)
+    return dict(tile=tile, strip=getattr(fld, 'strip_height', 0) or 0, chunk=chunk,
+                table={k: round(v, 2) for k, v in table.items()})
+
+
 def time_form(ns, wl, form, steps, warmup):
     import torch
-    fld = build_field(ns, wl, form, t_samples=steps + warmup)
+    fld = build_field(ns, wl, form, t_samples=steps + warmup + 50)
     fld.assemble_matrices()
+    tune = tuned(fld)
     if warmup:
         fld.simulate(warmup)
     runner = fld._runner()
@@ -100,7 +109,7 @@ def time_form(ns, wl, form, steps, warmup):
     achieved = BYTES_PER_CELL[form] * cells / kernel_s / 1e9
     p = np.array(fld.pressure.values)
     return dict(gcells=cells * steps / wall / 1e9, wall_s=wall, ms_per_step=wall / steps * 1e3,
-                kernel_us=kernel_s * 1e6, achieved_gbs=achieved, p=p)
+                kernel_us=kernel_s * 1e6, achieved_gbs=achieved, p=p, tune=tune)
 
 
 def batched_fields(ns, form, n_inst=256, seed=2, steps=1000):
@@ -187,8 +196,9 @@ def thermal_field(ns, form, steps, nx=4096, seed=3):
 
 def time_thermal(ns, form, steps, warmup):
     import torch
-    fld = thermal_field(ns, form, steps + warmup)
+    fld = thermal_field(ns, form, steps + warmup + 50)
     fld.assemble_matrices()
+    tune = tuned(fld)
     if warmup:
         fld.simulate(warmup)
     runner = fld._runner()
@@ -205,7 +215,7 @@ def time_thermal(ns, form, steps, warmup):
     achieved = THERMAL_BYTES_PER_CELL[form] * cells / kernel_s / 1e9
     return dict(gcells=cells * steps / wall / 1e9, ms_per_step=wall / steps * 1e3,
                 kernel_us=kernel_s * 1e6, achieved_gbs=achieved,
-                t=np.array(fld.temperature.values))
+                t=np.array(fld.temperature.values), tune=tune)
 
 
 def cpu_baseline_batched(steps, n_inst=32):
@@ -357,7 +367,7 @@ def main():
         entry = dict(value=round(r['gcells'], 3), ms_per_step=round(r['ms_per_step'], 4),
                      kernel_us=round(r['kernel_us'], 2),
                      roofline_frac=round(r['achieved_gbs'] / HBM_PEAK_GBS, 4),
-                     bytes_per_cell=BYTES_PER_CELL[f])
+                     bytes_per_cell=BYTES_PER_CELL[f], autotune=r['tune'])
         if f != main_form and main_form == 'fp64':
             den = np.linalg.norm(ref['p'])
             entry['rel_l2_vs_fp64'] = float(np.linalg.norm(r['p'] - ref['p']) / den)
@@ -420,7 +430,7 @@ def main():
                          kernel_us=round(r['kernel_us'], 2),
                          achieved_gbs=round(r['achieved_gbs'], 1),
                          roofline_frac=round(r['achieved_gbs'] / HBM_PEAK_GBS, 4),
-                         bytes_per_cell=THERMAL_BYTES_PER_CELL[f])
+                         bytes_per_cell=THERMAL_BYTES_PER_CELL[f], autotune=r['tune'])
         if 'fp64' in tvals:
             for f, t in tvals.items():
                 if f != 'fp64':

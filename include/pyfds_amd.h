@@ -146,6 +146,9 @@ int fds_set_tile(fds_ctx *ctx, int tile_index);
 int fds_tile_count(void);
 int fds_tile_shape(int tile_index, int *tx, int *ty);
 int fds_tile_engine(int tile_index);
+/* Strip height (rows per workgroup) of a streaming shape; 0 restores the shape's own height.
+ * Ignored by LDS-tile shapes. */
+int fds_set_strip_height(fds_ctx *ctx, int rows);
 
 /* Batched sweeps (no reference counterpart: pyfds runs one field per Python loop).  A batch
  * groups contexts that are fully set up (coefficients committed, state, boundaries, outputs)

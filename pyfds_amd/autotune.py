@@ -1,7 +1,8 @@
-"""Autotuner: picks the step kernels' tile shape and the HIP-graph chunk length by timing.
+"""Autotuner: picks the step kernels' shape, strip height and HIP-graph chunk by timing.
 
-For a field (or a batch of fields) it times a few steps of every admissible tile shape of the
-compiled tile table (``fds_tile_count``/``fds_tile_shape``) and graph-chunk length with the
+For a field (or a batch of fields) it times a few steps of every admissible shape of the
+compiled shape table (``fds_tile_count``/``fds_tile_shape``/``fds_tile_engine``; streaming
+shapes also at a few strip heights, ``fds_set_strip_height``) and graph-chunk length with the
 GPU's own clock (HIP events around the launches, ``fds_last_run_ms`` /
 ``fds_batch_last_run_ms``), keeps the fastest and caches the choice in a JSON file keyed by
 device, model, form, grid and lossy/Axi variant.  Timing runs advance a scratch copy of the
@@ -49,7 +50,7 @@ def _key(fields):
     lossy = any(getattr(x, '_coefs', {}).get(4) is not None for x in fields)
     cells = sum(x.num_points for x in fields)
     return '|'.join(str(v) for v in (_device_name(), type(f).__name__, f.form, len(fields),
-                                     cells, f._device_grid(), lossy))
+                                     cells, f._device_grid(), lossy, 'v2'))
 
 
 def _snapshot(fields):
@@ -72,9 +73,14 @@ def _restore(fields, snap):
         f.step = step
 
 
+# strip heights tried for the 256-wide streaming shape (other streaming shapes: their own)
+STRIP_HEIGHTS = (6, 12, 24)
+
+
 def tune_field(fld, steps=40, chunks=(50,), use_cache=True):
-    """Choose ``fld.tile`` / ``fld.graph_chunk`` for one field; returns (tile, chunk, table)
-    where table maps (tile, chunk) -> microseconds per step."""
+    """Choose ``fld.tile`` / ``fld.strip_height`` / ``fld.graph_chunk`` for one field; returns
+    (tile, chunk, table) where table maps 'tile,strip,chunk' -> microseconds per step (strip 0:
+    the shape's own height)."""
     return _tune([fld], steps, chunks, use_cache, batch=False)
 
 
@@ -89,7 +95,7 @@ def _tune(fields, steps, chunks, use_cache, batch):
     cache = _load_cache() if use_cache else {}
     if key in cache:
         tile, chunk = cache[key]['tile'], cache[key]['chunk']
-        _apply(fields, tile, chunk)
+        _apply(fields, tile, chunk, cache[key].get('strip', 0))
         return tile, chunk, cache[key]['table']
     for f in fields:
         if not f.matrices_assembled:
@@ -98,12 +104,23 @@ def _tune(fields, steps, chunks, use_cache, batch):
     shapes, engines = nat.tile_shapes(), nat.tile_engines()
     one_row = fields[0]._device_grid()[1] == 1
     lossy = any(getattr(x, '_coefs', {}).get(4) is not None for x in fields)
-    cand = [i for i, (tx, ty) in enumerate(shapes)
-            if (ty == 1) == one_row and not (engines[i] == 'stream' and (lossy or one_row))]
+    cand = []
+    for i, (tx, ty) in enumerate(shapes):
+        if (ty == 1) != one_row:
+            continue
+        if engines[i] == 'stream':
+            if lossy or one_row:
+                continue
+            cand.append((i, 0))
+            if tx == 256 and ty == 16:
+                cand += [(i, h) for h in STRIP_HEIGHTS]
+        else:
+            cand.append((i, 0))
     table = {}
-    for tile in cand:
+    for tile, strip in cand:
         for chunk in chunks:
             try:
+                _set_strip(fields, strip)
                 if batch:
                     from .batch import BatchRunner
                     br = BatchRunner(fields, tile=tile, chunk=chunk)
@@ -126,20 +143,32 @@ def _tune(fields, steps, chunks, use_cache, batch):
                 continue
             finally:
                 _restore(fields, snap)
-            table['%d,%d' % (tile, chunk)] = ms * 1e3 / steps
+            table['%d,%d,%d' % (tile, strip, chunk)] = ms * 1e3 / steps
     if not table:
         raise RuntimeError('no tile shape could be timed')
     best = min(table, key=table.get)
-    tile, chunk = (int(v) for v in best.split(','))
-    _apply(fields, tile, chunk)
-    cache[key] = dict(tile=tile, chunk=chunk, table=table)
+    tile, strip, chunk = (int(v) for v in best.split(','))
+    _apply(fields, tile, chunk, strip)
+    cache[key] = dict(tile=tile, strip=strip, chunk=chunk, table=table)
     if use_cache:
         _save_cache(cache)
     return tile, chunk, table
 
 
-def _apply(fields, tile, chunk):
+def _set_strip(fields, strip):
     lib = nat.load()
+    for f in fields:
+        f.strip_height = strip
+        r = f.__dict__.get('_device_runner')
+        if r is not None:
+            r.strip_height = strip
+            if r.ctx is not None:
+                nat.check(lib.fds_set_strip_height(r.ctx, strip))
+
+
+def _apply(fields, tile, chunk, strip=0):
+    lib = nat.load()
+    _set_strip(fields, strip)
     for f in fields:
         f.tile = tile
         f.graph_chunk = chunk

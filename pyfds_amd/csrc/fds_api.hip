@@ -134,6 +134,10 @@ struct fds_ctx {
   int tile = -1;          // requested shape index (-1: default)
   int shape = 0;          // shape index in use (tile table, then the streaming table)
   bool streaming = false; // row-streaming engine in use
+  int strip_rows = 0;     // streaming strip height override (0: the shape's own)
+  int fuse = 1;           // time steps per launch of the streaming shape
+  int ghost = 0;          // ghost columns / rows per side of a streaming tile (fuse - 1)
+  int sx = 0;             // x stride of the tiles (tx - 2*ghost)
   int tx = 0, ty = 0, tiles_x = 0, tiles_y = 0, ntiles = 0;
   bool events_dirty = true;
   DevBuf dyn;
@@ -160,6 +164,7 @@ struct fds_batch {
   const fds::KernelSet *ks = nullptr;
   int shape = 0;
   bool streaming = false;
+  int fuse = 1;
   int tx = 0, ty = 0, ntiles = 0;
   size_t lds = 0;
   DevBuf descs[2][2];          // [parity][store]
@@ -193,32 +198,36 @@ Region region_of(const fds_ctx *c, int k) {
     if (k == 1) return Region{0, 0, 0, tx, tx + 1};
     return Region{0, 0, 0, tx - 1, tx};
   }
-  if (k == 0) return Region{-1, ty, -1, tx, tx + 2};
-  if (k == 1) return Region{0, ty - 1, 0, tx, tx + 1};
-  return Region{0, ty, 0, tx - 1, tx};
+  // rows of all fused stages: G ghost rows above and below the strip
+  const int G = c->ghost;
+  if (k == 0) return Region{-G - 1, ty + G, -1, tx, tx + 2};
+  if (k == 1) return Region{-G, ty - 1 + G, 0, tx, tx + 1};
+  return Region{-G, ty + G, 0, tx - 1, tx};
 }
 
 void build_bound_events(const fds_ctx *c, int k, Events &ev) {
   const Comp &cp = c->comp[k];
   const Region R = region_of(c, k);
-  const int64_t nx = c->nx, tx = c->tx, ty = c->ty;
+  // tile (txi, tyi) has its column 0 at absolute column txi*sx - g and row 0 at tyi*ty
+  const int64_t nx = c->nx, sx = c->sx, ty = c->ty, g = c->ghost;
   std::vector<std::pair<int32_t, std::pair<int32_t, int32_t>>> all;   // tile, (pos, u)
-  const int64_t cmax_abs = (int64_t)(c->tiles_x - 1) * tx + R.cmax;
+  const int64_t cmin_abs = R.cmin - g;
+  const int64_t cmax_abs = (int64_t)(c->tiles_x - 1) * sx + R.cmax - g;
   const int64_t rmax_abs = (int64_t)(c->tiles_y - 1) * ty + R.rmax;
   for (size_t u = 0; u < cp.cells.size(); ++u) {
     const int64_t j = cp.cells[u];
-    // absolute row Rw and column C with Rw*nx + C = j, C in [cmin, cmax_abs]
+    // absolute row Rw and column C with Rw*nx + C = j, C in [cmin_abs, cmax_abs]
     const int64_t rlo = std::max<int64_t>(R.rmin, ceildiv(j - cmax_abs, nx));
-    const int64_t rhi = std::min<int64_t>(rmax_abs, floordiv(j - R.cmin, nx));
+    const int64_t rhi = std::min<int64_t>(rmax_abs, floordiv(j - cmin_abs, nx));
     for (int64_t Rw = rlo; Rw <= rhi; ++Rw) {
       const int64_t C = j - Rw * nx;
       const int64_t ty_lo = std::max<int64_t>(0, ceildiv(Rw - R.rmax, ty));
       const int64_t ty_hi = std::min<int64_t>(c->tiles_y - 1, floordiv(Rw - R.rmin, ty));
-      const int64_t tx_lo = std::max<int64_t>(0, ceildiv(C - R.cmax, tx));
-      const int64_t tx_hi = std::min<int64_t>(c->tiles_x - 1, floordiv(C - R.cmin, tx));
+      const int64_t tx_lo = std::max<int64_t>(0, ceildiv(C + g - R.cmax, sx));
+      const int64_t tx_hi = std::min<int64_t>(c->tiles_x - 1, floordiv(C + g - R.cmin, sx));
       for (int64_t tyi = ty_lo; tyi <= ty_hi; ++tyi)
         for (int64_t txi = tx_lo; txi <= tx_hi; ++txi) {
-          const int r = (int)(Rw - tyi * ty), col = (int)(C - txi * tx);
+          const int r = (int)(Rw - tyi * ty), col = (int)(C - txi * sx + g);
           all.push_back({(int32_t)(tyi * c->tiles_x + txi), {R.pos(r, col), (int32_t)u}});
         }
     }
@@ -241,9 +250,10 @@ void build_output_events(const fds_ctx *c, int k, Events &ev) {
   std::vector<std::pair<int32_t, std::pair<int32_t, int32_t>>> all;
   for (size_t s = 0; s < cp.out_cells.size(); ++s) {
     const int64_t j = cp.out_cells[s];
+    // the tile that owns the cell (owned columns of a streaming tile start at its ghost)
     const int64_t iy = j / c->nx, ix = j % c->nx;
-    const int64_t tyi = iy / c->ty, txi = ix / c->tx;
-    const int r = (int)(iy - tyi * c->ty), col = (int)(ix - txi * c->tx);
+    const int64_t tyi = iy / c->ty, txi = ix / c->sx;
+    const int r = (int)(iy - tyi * c->ty), col = (int)(ix - txi * c->sx + c->ghost);
     all.push_back({(int32_t)(tyi * c->tiles_x + txi), {R.pos(r, col), (int32_t)s}});
   }
   std::sort(all.begin(), all.end());
@@ -276,6 +286,8 @@ bool can_stream(const fds_ctx *c) {
 
 int choose_tiles(fds_ctx *c) {
   c->streaming = false;
+  c->fuse = 1;
+  c->ghost = 0;
   if (c->family == kNonlinear) {
     c->tx = fds::kNlTile;
     c->ty = 1;
@@ -289,8 +301,14 @@ int choose_tiles(fds_ctx *c) {
     c->shape = t;
     c->streaming = t >= fds::kNumTiles;
     shape_of(t, &c->tx, &c->ty);
+    if (c->streaming) {
+      if (c->strip_rows > 0) c->ty = c->strip_rows;
+      c->fuse = fds::kStreamShapes[t - fds::kNumTiles][2];
+      c->ghost = c->fuse - 1;
+    }
   }
-  c->tiles_x = (int)((c->nx + c->tx - 1) / c->tx);
+  c->sx = c->tx - 2 * c->ghost;
+  c->tiles_x = (int)((c->nx + c->sx - 1) / c->sx);
   c->tiles_y = (int)((c->ny + c->ty - 1) / c->ty);
   const int64_t nt = (int64_t)c->tiles_x * c->tiles_y;
   if (nt > (int64_t)std::numeric_limits<int32_t>::max() / 2)
@@ -388,14 +406,15 @@ void fill_args(const fds_ctx *c, fds::StepArgs<T> &a, int parity) {
   a.dyn = c->dyn.as<const int64_t>();
   a.flags = (c->iso_b ? fds::kFlagIsoB : 0) | (c->iso_k ? fds::kFlagIsoK : 0);
   a.strip_h = c->ty;
+  a.ghost = c->ghost;
 }
 
-fds::LaunchFn pick_launch(const fds_ctx *c) {
+fds::LaunchFn pick_launch(const fds_ctx *c, bool fused) {
   const int t = c->shape;
   if (c->streaming) {
-    const int s = t - fds::kNumTiles;
-    return c->family == kThermal ? c->ks->stream_thermal[s][c->axi ? 1 : 0]
-                                 : c->ks->stream_acoustic[s][c->axi ? 1 : 0];
+    const int s = t - fds::kNumTiles, f = fused ? 1 : 0;
+    return c->family == kThermal ? c->ks->stream_thermal[s][c->axi ? 1 : 0][f]
+                                 : c->ks->stream_acoustic[s][c->axi ? 1 : 0][f];
   }
   switch (c->family) {
     case kAcoustic: return c->ks->acoustic[t][c->axi ? 1 : 0][c->lossy ? 1 : 0];
@@ -431,14 +450,20 @@ void launch_lbc<float>(fds_ctx *c, const fds::StepArgs<float> &a) {
   fds::launch_lbc_step(st, c->stream);
 }
 
+// Time steps per launch: the streaming shapes' fused stage count (one step per launch when a
+// per-step learned boundary follows the step kernel).
+int step_group(const fds_ctx *c) { return (c->streaming && !c->lbc_w1) ? c->fuse : 1; }
+
+// Launch one step (or, fused, step_group(c) steps) starting at step k of the launch sequence:
+// reads state buffer `parity`, writes parity^1.
 template <typename T>
-int launch_step(fds_ctx *c, int parity, int k, bool store) {
+int launch_step(fds_ctx *c, int parity, int k, bool store, bool fused) {
   fds::StepArgs<T> a;
   fill_args<T>(c, a, parity);
   a.k = k;
   if (store) a.flags |= fds::kFlagStore;
   const int nb = fds::xcd_grid(c->ntiles);
-  pick_launch(c)(&a, nb, lds_bytes(c), c->stream);
+  pick_launch(c, fused)(&a, nb, lds_bytes(c), c->stream);
   HIP_TRY(hipGetLastError());
   if (c->lbc_w1) {
     launch_lbc<T>(c, a);
@@ -447,9 +472,9 @@ int launch_step(fds_ctx *c, int parity, int k, bool store) {
   return 0;
 }
 
-int launch_step_any(fds_ctx *c, int parity, int k, bool store) {
-  return c->elem == 8 ? launch_step<double>(c, parity, k, store)
-                      : launch_step<float>(c, parity, k, store);
+int launch_step_any(fds_ctx *c, int parity, int k, bool store, bool fused = false) {
+  return c->elem == 8 ? launch_step<double>(c, parity, k, store, fused)
+                      : launch_step<float>(c, parity, k, store, fused);
 }
 
 int get_graph(fds_ctx *c, int chunk, int parity, hipGraphExec_t *out) {
@@ -465,7 +490,9 @@ int get_graph(fds_ctx *c, int chunk, int parity, hipGraphExec_t *out) {
   }
   HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
-  for (int k = 0; k < chunk && rc == 0; ++k) rc = launch_step_any(c, parity ^ (k & 1), k, false);
+  const int grp = step_group(c);   // chunk is a multiple of grp
+  for (int k = 0; k < chunk && rc == 0; k += grp)
+    rc = launch_step_any(c, parity ^ ((k / grp) & 1), k, false, grp > 1);
   if (rc == 0) {
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->dyn.as<int64_t>(),
                        (int64_t)chunk);
@@ -796,6 +823,15 @@ int fds_set_tile(fds_ctx *c, int t) {
   return 0;
 }
 
+int fds_set_strip_height(fds_ctx *c, int rows) {
+  if (int rc = check_ctx(c)) return rc;
+  if (rows < 0 || rows > 65536) return fail(FDS_E_ARG, "strip height %d out of range", rows);
+  c->strip_rows = rows;
+  if (int rc = choose_tiles(c)) return rc;
+  touch(c);
+  return 0;
+}
+
 int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
   if (int rc = check_ctx(c)) return rc;
   if (!c->committed) return fail(FDS_E_STATE, "coefficients not committed");
@@ -830,18 +866,26 @@ int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
   if (!c->ev1) HIP_TRY(hipEventCreate(&c->ev1));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   int64_t done = 0;
+  const int g = step_group(c);
   if (chunk > 0) {
-    // graph-replayed full chunks, never including the run's last step (which stores the
-    // assigned components)
+    // graph-replayed full chunks (whole launches of g steps), never including the run's last
+    // step (which stores the assigned components)
+    chunk = std::max(g, chunk - chunk % g);
     while (nsteps - done > chunk) {
       hipGraphExec_t ex;
       if (int rc = get_graph(c, chunk, c->parity, &ex)) return rc;
       HIP_TRY(hipGraphLaunch(ex, c->stream));
-      c->parity ^= (chunk & 1);
+      c->parity ^= ((chunk / g) & 1);
       done += chunk;
     }
   }
-  for (int64_t k = 0; done + k < nsteps; ++k) {
+  // the rest as plain launches (k counts from the device step counter's current base)
+  int64_t k = 0;
+  for (; g > 1 && nsteps - done - k >= g; k += g) {
+    if (int rc = launch_step_any(c, c->parity, (int)k, done + k + g == nsteps, true)) return rc;
+    c->parity ^= 1;
+  }
+  for (; done + k < nsteps; ++k) {
     const bool last = (done + k == nsteps - 1);
     if (int rc = launch_step_any(c, c->parity, (int)k, last)) return rc;
     c->parity ^= 1;
@@ -968,7 +1012,7 @@ int batch_build(fds_batch *b, int64_t nsteps) {
   return 0;
 }
 
-int batch_launch(fds_batch *b, int rel_parity, int k, bool store) {
+int batch_launch(fds_batch *b, int rel_parity, int k, bool store, bool fused = false) {
   fds::BatchLaunch L;
   L.descs = b->descs[rel_parity][store ? 1 : 0].p;
   L.tile_inst = b->tile_inst.as<const int32_t>();
@@ -976,7 +1020,8 @@ int batch_launch(fds_batch *b, int rel_parity, int k, bool store) {
   L.ntiles = b->ntiles;
   L.k = k;
   if (b->streaming)
-    b->ks->stream_batch[b->shape - fds::kNumTiles](L, fds::xcd_grid(b->ntiles), 0, b->stream);
+    b->ks->stream_batch[b->shape - fds::kNumTiles][fused ? 1 : 0](L, fds::xcd_grid(b->ntiles), 0,
+                                                                 b->stream);
   else
     b->ks->batch[b->shape](L, fds::xcd_grid(b->ntiles), b->lds, b->stream);
   HIP_TRY(hipGetLastError());
@@ -992,7 +1037,9 @@ int batch_graph(fds_batch *b, int chunk, int rel_parity, hipGraphExec_t *out) {
   }
   HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
-  for (int k = 0; k < chunk && rc == 0; ++k) rc = batch_launch(b, rel_parity ^ (k & 1), k, false);
+  const int grp = b->fuse;   // chunk is a multiple of grp
+  for (int k = 0; k < chunk && rc == 0; k += grp)
+    rc = batch_launch(b, rel_parity ^ ((k / grp) & 1), k, false, grp > 1);
   if (rc == 0)
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, b->stream, b->dyn.as<int64_t>(),
                        (int64_t)chunk);
@@ -1071,6 +1118,7 @@ int fds_batch_create(fds_batch **out, fds_ctx *const *ctxs, int n, int tile) {
   }
   b->shape = t;
   b->streaming = t >= fds::kNumTiles;
+  b->fuse = b->streaming ? fds::kStreamShapes[t - fds::kNumTiles][2] : 1;
   shape_of(t, &b->tx, &b->ty);
   b->lds = lds;
   b->ntiles = (int)tinst.size();
@@ -1119,16 +1167,23 @@ int fds_batch_run(fds_batch *b, int64_t start, int64_t nsteps, int chunk) {
   HIP_TRY(hipEventRecord(b->ev0, b->stream));
   int rel = b->flip;
   int64_t done = 0;
+  const int g = b->fuse;
   if (chunk > 0) {
+    chunk = std::max(g, chunk - chunk % g);
     while (nsteps - done > chunk) {
       hipGraphExec_t ex;
       if (int rc2 = batch_graph(b, chunk, rel, &ex)) return rc2;
       HIP_TRY(hipGraphLaunch(ex, b->stream));
-      rel ^= (chunk & 1);
+      rel ^= ((chunk / g) & 1);
       done += chunk;
     }
   }
-  for (int64_t k = 0; done + k < nsteps; ++k) {
+  int64_t k = 0;
+  for (; g > 1 && nsteps - done - k >= g; k += g) {
+    if (int rc2 = batch_launch(b, rel, (int)k, done + k + g == nsteps, true)) return rc2;
+    rel ^= 1;
+  }
+  for (; done + k < nsteps; ++k) {
     if (int rc2 = batch_launch(b, rel, (int)k, done + k == nsteps - 1)) return rc2;
     rel ^= 1;
   }
@@ -1137,8 +1192,8 @@ int fds_batch_run(fds_batch *b, int64_t start, int64_t nsteps, int chunk) {
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->last_run_ms = ms;
-  // members advanced nsteps steps in lock step
-  if (nsteps & 1) {
+  // members advanced nsteps steps in lock step, flipping buffers once per launch
+  if (rel != b->flip) {
     for (auto *c : b->ctxs) c->parity ^= 1;
     b->flip ^= 1;
   }

@@ -52,6 +52,7 @@ struct StepArgs {
   int32_t flags;        // bit0: store assigned components (thermal flux) this step
   int32_t steps;        // resident 1D kernels: steps to run in this launch
   int32_t strip_h;      // row-streaming kernels: rows per strip
+  int32_t ghost;        // row-streaming kernels: ghost columns / rows per side (fused steps - 1)
 };
 
 // Map workgroup b to a tile so that each XCD (b % 8) gets one contiguous band of tiles:

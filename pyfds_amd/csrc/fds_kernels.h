@@ -11,13 +11,16 @@ constexpr int kNumTiles = 6;
 constexpr int kTileShapes[kNumTiles][2] = {
     {64, 16}, {64, 8}, {128, 8}, {256, 4}, {128, 4}, {256, 1}};
 constexpr int kDefaultTile2D = 1;       // default of the tile engine (lossy models)
-constexpr int kDefaultLossless2D = 1;   // default for lossless 2D models (either engine)
+constexpr int kDefaultLossless2D = kNumTiles;   // lossless 2D: streaming 256 x 16
 constexpr int kTileRow = 5;
 
-// Row-streaming engine (lossless 2D models): TX threads = columns, strips of TY rows.  Shape
-// indices continue the tile table: index kNumTiles + i is kStreamShapes[i].
-constexpr int kNumStream = 5;
-constexpr int kStreamShapes[kNumStream][2] = {{256, 16}, {256, 32}, {128, 16}, {128, 32}, {256, 8}};
+// Row-streaming engine (lossless 2D models): TX threads = columns, strips of TY rows, S time
+// steps fused per launch.  Shape indices continue the tile table: index kNumTiles + i is
+// kStreamShapes[i] = {TX, TY, S}.
+constexpr int kNumStream = 8;
+constexpr int kStreamShapes[kNumStream][3] = {{256, 16, 1}, {256, 32, 1}, {128, 16, 1},
+                                              {256, 8, 1},  {256, 16, 2}, {256, 32, 2},
+                                              {128, 32, 2}, {256, 32, 3}};
 
 using LaunchFn = void (*)(const void *args, int nblocks, size_t lds_bytes, hipStream_t s);
 
@@ -38,9 +41,9 @@ struct KernelSet {
   LaunchFn thermal[kNumTiles][2];       // [tile][axi]
   LaunchFn nonlinear;
   BatchFn batch[kNumTiles];
-  LaunchFn stream_acoustic[kNumStream][2];   // [shape][axi]
-  LaunchFn stream_thermal[kNumStream][2];
-  BatchFn stream_batch[kNumStream];
+  LaunchFn stream_acoustic[kNumStream][2][2];   // [shape][axi][fused]
+  LaunchFn stream_thermal[kNumStream][2][2];
+  BatchFn stream_batch[kNumStream][2];          // [shape][fused]
 };
 
 const KernelSet &kernels_f64();

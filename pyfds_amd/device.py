@@ -131,6 +131,7 @@ class DeviceRunner:
         self.owners = [None, None, None]
         self.chunk = getattr(field, 'graph_chunk', 50)
         self.tile = getattr(field, 'tile', None)
+        self.strip_height = getattr(field, 'strip_height', None)
 
     # -- lifetime -------------------------------------------------------------------------
     def _create(self, field):
@@ -146,6 +147,8 @@ class DeviceRunner:
         self.n = nx * ny
         if self.tile is not None:
             nat.check(self.lib.fds_set_tile(self.ctx, int(self.tile)))
+        if self.strip_height is not None:
+            nat.check(self.lib.fds_set_strip_height(self.ctx, int(self.strip_height)))
 
     def close(self):
         if self.ctx is not None and self.lib is not None:

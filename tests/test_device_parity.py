@@ -101,7 +101,7 @@ LOSSLESS_2D = ['a2d_main', 'a2d_wrap', 'axi_lossless', 'bounds_semantics', 't2d_
                't2d_blocks', 'taxi']
 
 
-@pytest.mark.parametrize('slot', range(5))
+@pytest.mark.parametrize('slot', range(8))
 @pytest.mark.parametrize('name', LOSSLESS_2D + ['a2d_lossy', 'axi_lossy'])
 def test_fp64_stream_shapes_match_reference_golden(name, slot):
     idx = _stream_indices()
@@ -124,7 +124,7 @@ def test_fp32_stream_engine_equals_tile_engine(name, form):
     scene = scenes.SCENES[name]()
     print('scene', name, form, flush=True)
     a = run_device(scene, form=form, tile=1)
-    for idx in _stream_indices()[:2]:
+    for idx in [_stream_indices()[i] for i in (0, 4, 7)]:
         b = run_device(scene, form=form, tile=idx)
         for k in a:
             np.testing.assert_array_equal(a[k], b[k], err_msg='%s %s %d' % (name, k, idx))
@@ -134,7 +134,11 @@ def test_fp32_stream_engine_equals_tile_engine(name, form):
                                               ('Acoustic2D', 300, 75, 2),
                                               ('Acoustic3DAxi', 300, 130, 1),
                                               ('Acoustic3DAxi', 513, 171, 3),
-                                              ('Acoustic2D', 130, 101, 4)])
+                                              ('Acoustic2D', 130, 101, 4),
+                                              ('Acoustic2D', 700, 90, 5),
+                                              ('Acoustic3DAxi', 260, 133, 6),
+                                              ('Acoustic2D', 509, 97, 7),
+                                              ('Acoustic3DAxi', 251, 64, 7)])
 def test_fp64_stream_multi_strip_grids_match_oracle_bitwise(model, nx, ny, slot):
     """Partial column blocks at the right edge and partial strips at the bottom."""
     idx = _stream_indices()[slot]
@@ -146,12 +150,16 @@ def test_fp64_stream_multi_strip_grids_match_oracle_bitwise(model, nx, ny, slot)
         np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
 
 
+@pytest.mark.parametrize('slot', [None, 4, 7])
 @pytest.mark.parametrize('chunk', [0, 1, 7, 50])
-def test_graph_chunks_bit_identical(chunk):
-    """HIP-graph replayed step chunks (and plain launches) give identical results."""
+def test_graph_chunks_bit_identical(chunk, slot):
+    """HIP-graph replayed step chunks (and plain launches) give identical results, also with
+    several steps fused per launch (chunks rounded to whole launches, remainders run as
+    single steps)."""
     scene = scenes.scene_bounds_semantics()
     gold = np.load(os.path.join(GOLDEN, 'bounds_semantics.npz'))
-    res = run_device(scene, graph_chunk=chunk)
+    attrs = {} if slot is None else dict(tile=_stream_indices()[slot])
+    res = run_device(scene, graph_chunk=chunk, **attrs)
     for k in gold.files:
         if k != 'fingerprint':
             np.testing.assert_array_equal(res[k], gold[k], err_msg=k)
@@ -162,7 +170,7 @@ def _big_acoustic(nx, ny, steps, form='fp64', seed=1, model='Acoustic2D', mu=0.0
     dt = 0.5 * dx / 5900.0
     t = np.arange(steps) * dt
     rng = np.random.default_rng(seed)
-    w, h = rng.integers(nx // 8, nx // 4, size=2)
+    w, h = rng.integers(nx // 8, nx // 4), rng.integers(ny // 8, ny // 4)
     x0, y0 = rng.integers(0, nx - w), rng.integers(0, ny - h)
     incl = scenes.rect_indices(nx, int(x0), int(y0), int(w), int(h))
     return dict(
@@ -192,7 +200,7 @@ def test_fp64_multi_tile_grids_match_oracle_bitwise(model, mu, nx, ny):
         np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
 
 
-@pytest.mark.parametrize('tile', [None, 'stream0', 'stream3'])
+@pytest.mark.parametrize('tile', [None, 'stream0', 'stream3', 'stream4', 'stream7'])
 @pytest.mark.parametrize('model', ['Thermal2D', 'Thermal3DAxi'])
 def test_fp64_thermal_multi_tile_matches_oracle_bitwise(model, tile):
     nx, ny, dx = 190, 150, 1e-3
