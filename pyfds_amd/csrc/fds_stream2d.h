@@ -22,6 +22,8 @@
 // the stages' rows; every stage walks them with its own cursor and its own step index.
 #pragma once
 
+#include <type_traits>
+
 #include "fds_common.h"
 #include "fds_step2d.h"
 #include "../../include/pyfds_amd.h"
@@ -81,9 +83,40 @@ __device__ __forceinline__ void stream_record(const OutView &o, Cursor &q, int w
   q.cur = e;
 }
 
-template <typename T>
+// Cell j of an array of n cells, zero outside [0, n) (the reference's zero ghosts at the array
+// ends).  CHK = false for workgroups whose reads all fall inside the array.
+template <bool CHK, typename T>
 __device__ __forceinline__ T ld(const T *p, int j, int n) {
+  if (!CHK) return p[j];
   return (j >= 0 && j < n) ? p[j] : T(0);
+}
+
+// Cell rb + off with rb uniform across the workgroup (a row base): unchecked loads address a
+// uniform row pointer plus the per-thread offset.
+template <bool CHK, typename T>
+__device__ __forceinline__ T ldr(const T *p, int rb, int off, int n) {
+  if (!CHK) return (p + rb)[off];
+  return ld<true>(p, rb + off, n);
+}
+
+// Whether every read of a streaming tile (rows y0-G-1 .. y0+H+G, columns x0-1 .. x0+TX) falls
+// inside the array, so its loads need no bounds checks.
+__device__ __forceinline__ bool strip_inside(int x0, int y0, int H, int G, int TX, int nx,
+                                             int64_t n) {
+  const int64_t lo = (int64_t)(y0 - G - 1) * nx + x0 - 1;
+  const int64_t hi = (int64_t)(y0 + H + G) * nx + x0 + TX;
+  return lo >= 0 && hi < n;
+}
+
+// Whether tile `tile` has boundary or output events in any component.
+template <typename T>
+__device__ __forceinline__ bool tile_has_events(const StepArgs<T> &a, int tile) {
+  bool any = false;
+  for (int k = 0; k < 3; ++k) {
+    any = any || a.bev[k].ptr[tile + 1] > a.bev[k].ptr[tile];
+    any = any || a.outp[k].ev.ptr[tile + 1] > a.outp[k].ev.ptr[tile];
+  }
+  return any;
 }
 
 // x modulo nx for any x (ghost columns of the first tile are negative, those of the last may
@@ -102,12 +135,61 @@ struct StageRows {
   __device__ __forceinline__ StageRows(int G, int H, int sg) : lo(sg - G), hi(H - 1 + G - sg) {}
 };
 
+// Event cursors of one stage (one per component for boundaries and for outputs).
+struct StageCursors {
+  Cursor b[3], o[3];
+};
+
+template <typename T>
+__device__ __forceinline__ StageCursors stage_cursors(const StepArgs<T> &a, int tile) {
+  StageCursors c;
+  for (int k = 0; k < 3; ++k) {
+    c.b[k] = cursor(a.bev[k], tile);
+    c.o[k] = cursor(a.outp[k].ev, tile);
+  }
+  return c;
+}
+
+// f(integral_constant<int, s>) for s = 0 .. N-1 (or N-1 .. 0), each s a compile-time constant.
+template <int N, bool REV = false, typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, REV ? N - 1 - I : I>{}), ...);
+}
+
+template <int N, bool REV = false, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl<N, REV>(f, std::make_integer_sequence<int, N>{});
+}
+
+// The main loop runs iterations i = 0 .. iters-1 as pairs so that every value carried from one
+// iteration to the next lives in a register pair indexed by the iteration's parity (no copies).
+template <typename F>
+__device__ __forceinline__ void run_pairs(int iters, F &&iter) {
+  int i = 0;
+  for (; i + 1 < iters; i += 2) {
+    iter(std::integral_constant<int, 0>{}, i);
+    iter(std::integral_constant<int, 1>{}, i + 1);
+  }
+  if (i < iters) iter(std::integral_constant<int, 0>{}, i);
+}
+
 // ------------------------------------------------------------------------------------------
 // Acoustic (lossless): v -= Dbwd(B) p, then p -= Dfwd(K) v
+//
+// Stage s at an iteration of parity Q reads: stage 0 the row loaded into nin[Q]; stage s > 0
+// p from pn[s-1][Q^1] (finished by stage s-1 one iteration ago), v from vh[s-1][Q] (two
+// iterations ago) and its previous row's p from pn[s-1][Q]; its coefficients it loads itself
+// (one iteration ahead, from cache).  Consumers run their post-barrier part before producers,
+// so a producer overwrites a slot only after its consumer is done with it.
 // ------------------------------------------------------------------------------------------
 template <typename T>
-struct AcIn {          // one input row of one column
+struct AcRow {         // one row of one column: state and coefficients
   T p, cp, vx, cvx, vy, cvy, bx, by, kx, ky;
+};
+
+template <typename T>
+struct AcCoef {
+  T bx, by, kx, ky;
 };
 
 template <typename T>
@@ -116,111 +198,15 @@ struct AcHalo {        // left thread: p, B of column -1; right thread: p, B, vx
 };
 
 template <typename T>
-struct AcStage {
-  AcIn<T> in;          // row being processed
-  AcIn<T> hold;        // v_new and coefficients of the previous row (handed to the next stage)
-  T bpx, bpy, bpy_prev, p_prev, cp_prev, kvy_prev;
-  Cursor pc, vc1, vc2, oc0, oc1, oc2;
+struct AcVel {
+  T vx, cvx, vy, cvy;
 };
 
-template <typename T>
-__device__ __forceinline__ void ac_stage_init(AcStage<T> &st, const StepArgs<T> &a, int tile) {
-  st.pc = cursor(a.bev[0], tile);
-  st.vc1 = cursor(a.bev[1], tile);
-  st.vc2 = cursor(a.bev[2], tile);
-  st.oc0 = cursor(a.outp[0].ev, tile);
-  st.oc1 = cursor(a.outp[1].ev, tile);
-  st.oc2 = cursor(a.outp[2].ev, tile);
-  st.bpx = st.bpy = st.bpy_prev = st.p_prev = st.cp_prev = st.kvy_prev = T(0);
-  st.in = AcIn<T>{};
-  st.hold = AcIn<T>{};
-}
-
-// Before the barrier: boundaries of p in row rho, record p, publish Bx*p.
-template <typename T, bool KAHAN, int TX, bool HALO>
-__device__ __forceinline__ void ac_pre(AcStage<T> &st, AcHalo<T> &h, const StepArgs<T> &a,
-                                       int G, int rho, const StageRows &R, int64_t step,
-                                       int64_t orow, T *sbp) {
-  const int t = threadIdx.x;
-  const bool lh = (t == 0), rh = (t == TX - 1);
-  stream_events<T, KAHAN>(a.bev[0], a.prog[0], st.pc, TX + 2, -G - 1, -1, rho, t, step, st.in.p,
-                          st.in.cp, HALO && (lh || rh), lh ? -1 : TX, h.p, h.cp);
-  stream_record<T, KAHAN>(a.outp[0], st.oc0, TX + 2, -G - 1, -1, rho, t, orow, st.in.p,
-                          st.in.cp);
-  st.bpx = st.in.bx * st.in.p;
-  st.bpy = st.in.by * st.in.p;
-  if (rho <= R.hi) {
-    sbp[t + 1] = st.bpx;
-    if (HALO && lh) sbp[0] = h.b * h.p;
-  }
-}
-
-// After the barrier: v of row rho, x products, p of row rho-1.  Returns the new p of row rho-1
-// in (pn, cpn) and the new v of row rho in (vxn, cvxn, vyn, cvyn).
-template <typename T, bool KAHAN, int TX, bool AXI, bool HALO>
-__device__ __forceinline__ void ac_post(AcStage<T> &st, const AcHalo<T> &h, const StepArgs<T> &a,
-                                        int G, int rho, const StageRows &R, int64_t step,
-                                        int64_t orow, const T *sbp, T *skv_cur,
-                                        const T *skv_prev, T rad, T radh, T &pn, T &cpn,
-                                        T &vxn, T &cvxn, T &vyn, T &cvyn) {
-  const int t = threadIdx.x;
-  const bool rh = (t == TX - 1);
-  vxn = cvxn = vyn = cvyn = T(0);
-  T hvxn = T(0), hcvxn = T(0);
-  if (rho >= R.lo && rho <= R.hi) {
-    const T d = (-sbp[t]) + st.bpx;
-    vxn = st.in.vx;
-    cvxn = st.in.cvx;
-    if (KAHAN) kahan_sub(vxn, cvxn, d); else vxn = vxn - d;
-    if (HALO && rh) {
-      const T dh = (-st.bpx) + h.b * h.p;
-      hvxn = h.vx;
-      hcvxn = h.cvx;
-      if (KAHAN) kahan_sub(hvxn, hcvxn, dh); else hvxn = hvxn - dh;
-    }
-    stream_events<T, KAHAN>(a.bev[1], a.prog[1], st.vc1, TX + 1, -G, 0, rho, t, step, vxn, cvxn,
-                            HALO && rh, TX, hvxn, hcvxn);
-    stream_record<T, KAHAN>(a.outp[1], st.oc1, TX + 1, -G, 0, rho, t, orow, vxn, cvxn);
-    T kvx;
-    if (AXI) kvx = (st.in.kx / rad) * (vxn * rad);
-    else kvx = st.in.kx * vxn;
-    skv_cur[t] = kvx;
-    if (HALO && rh) {
-      if (AXI) skv_cur[TX] = (h.k / radh) * (hvxn * radh);
-      else skv_cur[TX] = h.k * hvxn;
-    }
-  }
-  T kvy = T(0);
-  if (rho >= R.lo) {
-    const T d = (-st.bpy_prev) + st.bpy;
-    vyn = st.in.vy;
-    cvyn = st.in.cvy;
-    if (KAHAN) kahan_sub(vyn, cvyn, d); else vyn = vyn - d;
-    T dummy = T(0), dummyc = T(0);
-    stream_events<T, KAHAN>(a.bev[2], a.prog[2], st.vc2, TX, -G, 0, rho, t, step, vyn, cvyn,
-                            false, 0, dummy, dummyc);
-    stream_record<T, KAHAN>(a.outp[2], st.oc2, TX, -G, 0, rho, t, orow, vyn, cvyn);
-    kvy = st.in.ky * vyn;
-  }
-  pn = cpn = T(0);
-  if (rho > R.lo) {    // p of row rho-1 (its x products were published an iteration ago)
-    const T dxt = (-skv_prev[t]) + skv_prev[t + 1];
-    const T dyt = (-st.kvy_prev) + kvy;
-    const T d = dxt + dyt;
-    pn = st.p_prev;
-    cpn = st.cp_prev;
-    if (KAHAN) kahan_sub(pn, cpn, d); else pn = pn - d;
-  }
-  st.p_prev = st.in.p;
-  st.cp_prev = st.in.cp;
-  st.kvy_prev = kvy;
-  st.bpy_prev = st.bpy;
-}
-
-// S stages over one strip (S <= G + 1).
-template <typename T, bool KAHAN, int TX, bool AXI, int S>
-__device__ void acoustic_stream(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
-                                StreamLds<T, TX, S> &L) {
+// S stages over one strip (S <= G + 1).  EV: the tile has events; CHK: some reads fall
+// outside the arrays.
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool EV, bool CHK>
+__device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
+                               StreamLds<T, TX, S> &L) {
   auto &sbp = L.bwd;    // Bx*p of the row, columns -1..TX
   auto &skv = L.fwd;    // forward x products of the row, columns 0..TX
   const int H = a.strip_h, G = a.ghost;
@@ -252,122 +238,196 @@ __device__ void acoustic_stream(const StepArgs<T> &a, int tile, int64_t step, in
     }
   }
 
-  AcStage<T> st[S];
+  StageCursors cur[S];
+  if constexpr (EV) {
 #pragma unroll
-  for (int s = 0; s < S; ++s) ac_stage_init(st[s], a, tile);
-  AcHalo<T> h{}, nh{};
-  AcIn<T> nin{};
+    for (int s = 0; s < S; ++s) cur[s] = stage_cursors(a, tile);
+  }
+  AcRow<T> nin[2] = {};        // stage 0 input rows
+  AcHalo<T> nh[2] = {};        // stage 0 halo
+  T pp[2] = {}, cpp[2] = {};   // stage 0 input p after its events (previous-row p)
+  AcCoef<T> cf[S][2] = {};     // stages > 0: coefficients
+  T bpy[S][2] = {}, kvy[S][2] = {};
+  T pn[S][2] = {}, cpn[S][2] = {};   // new p of the row above the stage's current row
+  AcVel<T> vh[S][2] = {};            // new v of the stage's current row
 
-  // stage 1 input row rho into (nin, nh)
-  auto load_row = [&](int rho) {
-    const int j = (y0 + rho) * nx + x0 + t;
-    nin.p = ld(pin, j, n);
-    nin.vx = ld(vxin, j, n);
-    nin.vy = ld(vyin, j, n);
-    nin.bx = ld(Bx, j, n);
-    nin.by = ld(By, j, n);
-    nin.kx = ld(Kx, j, n);
-    nin.ky = ld(Ky, j, n);
+  auto load_row = [&](int rho, AcRow<T> &d, AcHalo<T> &h) __attribute__((always_inline)) {
+    const int rb = (y0 + rho) * nx + x0;
+    d.p = ldr<CHK>(pin, rb, t, n);
+    d.vx = ldr<CHK>(vxin, rb, t, n);
+    d.vy = ldr<CHK>(vyin, rb, t, n);
+    d.bx = ldr<CHK>(Bx, rb, t, n);
+    d.by = ldr<CHK>(By, rb, t, n);
+    d.kx = ldr<CHK>(Kx, rb, t, n);
+    d.ky = ldr<CHK>(Ky, rb, t, n);
     if (KAHAN) {
-      nin.cp = ld(a.cin[0], j, n);
-      nin.cvx = ld(a.cin[1], j, n);
-      nin.cvy = ld(a.cin[2], j, n);
+      d.cp = ldr<CHK>(a.cin[0], rb, t, n);
+      d.cvx = ldr<CHK>(a.cin[1], rb, t, n);
+      d.cvy = ldr<CHK>(a.cin[2], rb, t, n);
     }
     if (lh) {
-      nh.p = ld(pin, j - 1, n);
-      nh.b = ld(Bx, j - 1, n);
-      if (KAHAN) nh.cp = ld(a.cin[0], j - 1, n);
+      h.p = ldr<CHK>(pin, rb, t - 1, n);
+      h.b = ldr<CHK>(Bx, rb, t - 1, n);
+      if (KAHAN) h.cp = ldr<CHK>(a.cin[0], rb, t - 1, n);
     }
     if (rh) {
-      nh.p = ld(pin, j + 1, n);
-      nh.b = ld(Bx, j + 1, n);
-      nh.vx = ld(vxin, j + 1, n);
-      nh.k = ld(Kx, j + 1, n);
+      h.p = ldr<CHK>(pin, rb, t + 1, n);
+      h.b = ldr<CHK>(Bx, rb, t + 1, n);
+      h.vx = ldr<CHK>(vxin, rb, t + 1, n);
+      h.k = ldr<CHK>(Kx, rb, t + 1, n);
       if (KAHAN) {
-        nh.cp = ld(a.cin[0], j + 1, n);
-        nh.cvx = ld(a.cin[1], j + 1, n);
+        h.cp = ldr<CHK>(a.cin[0], rb, t + 1, n);
+        h.cvx = ldr<CHK>(a.cin[1], rb, t + 1, n);
       }
     }
   };
+  auto load_coef = [&](int rho, AcCoef<T> &d) __attribute__((always_inline)) {
+    const int rb = (y0 + rho) * nx + x0;
+    d.bx = ldr<CHK>(Bx, rb, t, n);
+    d.by = ldr<CHK>(By, rb, t, n);
+    d.kx = ldr<CHK>(Kx, rb, t, n);
+    d.ky = ldr<CHK>(Ky, rb, t, n);
+  };
 
-  load_row(-G - 1);
+  load_row(-G - 1, nin[0], nh[0]);
   const int iters = H + 2 * G + S + 1;
-  for (int i = 0; i < iters; ++i) {
-    const int buf = i & 1;
-    // ---- before the barrier (stage 1 takes the prefetched row and prefetches the next)
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
+  run_pairs(iters, [&](auto Qc, int i) __attribute__((always_inline)) {
+    constexpr int Q = decltype(Qc)::value;
+    T bpx[S];
+    // ---- before the barrier
+    static_for<S>([&](auto Sc) __attribute__((always_inline)) {
+      constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
       const StageRows R(G, H, s);
-      if (rho < R.lo - 1 || rho > R.hi + 1) continue;
-      if (s == 0) {
-        st[0].in = nin;
-        h = nh;
-        if (rho < R.hi + 1) load_row(rho + 1);
-        ac_pre<T, KAHAN, TX, true>(st[0], h, a, G, rho, R, step, row, sbp[0][buf]);
-      } else {
-        ac_pre<T, KAHAN, TX, false>(st[s], h, a, G, rho, R, step + s, row + s, sbp[s][buf]);
+      if (s > 0 && rho + 1 >= R.lo - 1 && rho + 1 <= R.hi + 1) load_coef(rho + 1, cf[s][Q ^ 1]);
+      if (rho < R.lo - 1 || rho > R.hi + 1) return;
+      if (s == 0 && rho < R.hi + 1) load_row(rho + 1, nin[Q ^ 1], nh[Q ^ 1]);
+      T &p = s == 0 ? nin[Q].p : pn[s > 0 ? s - 1 : 0][Q ^ 1];
+      T &cp = s == 0 ? nin[Q].cp : cpn[s > 0 ? s - 1 : 0][Q ^ 1];
+      const T cbx = s == 0 ? nin[Q].bx : cf[s][Q].bx;
+      const T cby = s == 0 ? nin[Q].by : cf[s][Q].by;
+      AcHalo<T> &h = nh[Q];
+      if constexpr (EV) {
+        stream_events<T, KAHAN>(a.bev[0], a.prog[0], cur[s].b[0], TX + 2, -G - 1, -1, rho, t,
+                                step + s, p, cp, s == 0 && (lh || rh), lh ? -1 : TX, h.p, h.cp);
+        stream_record<T, KAHAN>(a.outp[0], cur[s].o[0], TX + 2, -G - 1, -1, rho, t, row + s, p,
+                                cp);
       }
-    }
+      bpx[s] = cbx * p;
+      bpy[s][Q] = cby * p;
+      if (rho <= R.hi) {
+        sbp[s][Q][t + 1] = bpx[s];
+        if (s == 0 && lh) sbp[s][Q][0] = h.b * h.p;
+      }
+      if (s == 0) {
+        pp[Q] = p;
+        cpp[Q] = cp;
+      }
+    });
     __syncthreads();
-    // ---- after the barrier: consumers first, so a stage hands its finished row to the next
-    // stage once that stage is done with its current one
-#pragma unroll
-    for (int s = S - 1; s >= 0; --s) {
+    // ---- after the barrier, consumers first
+    static_for<S, true>([&](auto Sc) __attribute__((always_inline)) {
+      constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
       const StageRows R(G, H, s);
-      if (rho < R.lo - 1 || rho > R.hi + 1) continue;
-      T pn, cpn, vxn, cvxn, vyn, cvyn;
-      if (s == 0)
-        ac_post<T, KAHAN, TX, AXI, true>(st[0], h, a, G, rho, R, step, row, sbp[0][buf],
-                                         skv[0][buf], skv[0][buf ^ 1], rad, radh, pn, cpn, vxn,
-                                         cvxn, vyn, cvyn);
-      else
-        ac_post<T, KAHAN, TX, AXI, false>(st[s], h, a, G, rho, R, step + s, row + s,
-                                          sbp[s][buf], skv[s][buf], skv[s][buf ^ 1], rad, radh,
-                                          pn, cpn, vxn, cvxn, vyn, cvyn);
+      if (rho < R.lo - 1 || rho > R.hi + 1) return;
+      const int sp = s > 0 ? s - 1 : 0;
+      const T ivx = s == 0 ? nin[Q].vx : vh[sp][Q].vx;
+      const T icvx = s == 0 ? nin[Q].cvx : vh[sp][Q].cvx;
+      const T ivy = s == 0 ? nin[Q].vy : vh[sp][Q].vy;
+      const T icvy = s == 0 ? nin[Q].cvy : vh[sp][Q].cvy;
+      const T ckx = s == 0 ? nin[Q].kx : cf[s][Q].kx;
+      const T cky = s == 0 ? nin[Q].ky : cf[s][Q].ky;
+      const T pprev = s == 0 ? pp[Q ^ 1] : pn[sp][Q];
+      const T cpprev = s == 0 ? cpp[Q ^ 1] : cpn[sp][Q];
+      const AcHalo<T> &h = nh[Q];
+      T vxn = T(0), cvxn = T(0), vyn = T(0), cvyn = T(0);
+      if (rho >= R.lo && rho <= R.hi) {
+        const T d = (-sbp[s][Q][t]) + bpx[s];
+        vxn = ivx;
+        cvxn = icvx;
+        if (KAHAN) kahan_sub(vxn, cvxn, d); else vxn = vxn - d;
+        T hvxn = T(0), hcvxn = T(0);
+        if (s == 0 && rh) {
+          const T dh = (-bpx[s]) + h.b * h.p;
+          hvxn = h.vx;
+          hcvxn = h.cvx;
+          if (KAHAN) kahan_sub(hvxn, hcvxn, dh); else hvxn = hvxn - dh;
+        }
+        if constexpr (EV) {
+          stream_events<T, KAHAN>(a.bev[1], a.prog[1], cur[s].b[1], TX + 1, -G, 0, rho, t,
+                                  step + s, vxn, cvxn, s == 0 && rh, TX, hvxn, hcvxn);
+          stream_record<T, KAHAN>(a.outp[1], cur[s].o[1], TX + 1, -G, 0, rho, t, row + s, vxn,
+                                  cvxn);
+        }
+        if (AXI) skv[s][Q][t] = (ckx / rad) * (vxn * rad);
+        else skv[s][Q][t] = ckx * vxn;
+        if (s == 0 && rh) {
+          if (AXI) skv[s][Q][TX] = (h.k / radh) * (hvxn * radh);
+          else skv[s][Q][TX] = h.k * hvxn;
+        }
+      }
+      kvy[s][Q] = T(0);
+      if (rho >= R.lo) {
+        const T d = (-bpy[s][Q ^ 1]) + bpy[s][Q];
+        vyn = ivy;
+        cvyn = icvy;
+        if (KAHAN) kahan_sub(vyn, cvyn, d); else vyn = vyn - d;
+        if constexpr (EV) {
+          T dummy = T(0), dummyc = T(0);
+          stream_events<T, KAHAN>(a.bev[2], a.prog[2], cur[s].b[2], TX, -G, 0, rho, t,
+                                  step + s, vyn, cvyn, false, 0, dummy, dummyc);
+          stream_record<T, KAHAN>(a.outp[2], cur[s].o[2], TX, -G, 0, rho, t, row + s, vyn,
+                                  cvyn);
+        }
+        kvy[s][Q] = cky * vyn;
+      }
+      T pnew = T(0), cpnew = T(0);
+      if (rho > R.lo) {    // p of row rho-1 (its x products were published an iteration ago)
+        const T dxt = (-skv[s][Q ^ 1][t]) + skv[s][Q ^ 1][t + 1];
+        const T dyt = (-kvy[s][Q ^ 1]) + kvy[s][Q];
+        const T d = dxt + dyt;
+        pnew = pprev;
+        cpnew = cpprev;
+        if (KAHAN) kahan_sub(pnew, cpnew, d); else pnew = pnew - d;
+      }
       if (s == S - 1) {
         // last stage: write the owned cells
         if (owned && rho >= 1 && rho - 1 < H && y0 + rho - 1 < a.ny) {
-          const int jp = (y0 + rho - 1) * nx + x0 + t;
-          a.out[0][jp] = pn;
-          if (KAHAN) a.cout[0][jp] = cpn;
+          const int rb = (y0 + rho - 1) * nx + x0;
+          (a.out[0] + rb)[t] = pnew;
+          if (KAHAN) (a.cout[0] + rb)[t] = cpnew;
         }
         if (owned && rho >= 0 && rho < H && y0 + rho < a.ny) {
-          const int jv = (y0 + rho) * nx + x0 + t;
-          a.out[1][jv] = vxn;
-          a.out[2][jv] = vyn;
+          const int rb = (y0 + rho) * nx + x0;
+          (a.out[1] + rb)[t] = vxn;
+          (a.out[2] + rb)[t] = vyn;
           if (KAHAN) {
-            a.cout[1][jv] = cvxn;
-            a.cout[2][jv] = cvyn;
+            (a.cout[1] + rb)[t] = cvxn;
+            (a.cout[2] + rb)[t] = cvyn;
           }
         }
       } else {
-        // hand row rho-1 (p just updated; v and coefficients held) to the next stage
-        AcIn<T> &nxt = st[s + 1 < S ? s + 1 : s].in;
-        nxt = st[s].hold;
-        nxt.p = pn;
-        nxt.cp = cpn;
+        pn[s][Q] = pnew;
+        cpn[s][Q] = cpnew;
+        vh[s][Q] = AcVel<T>{vxn, cvxn, vyn, cvyn};
       }
-      AcIn<T> &hd = st[s].hold;
-      hd.vx = vxn;
-      hd.cvx = cvxn;
-      hd.vy = vyn;
-      hd.cvy = cvyn;
-      hd.bx = st[s].in.bx;
-      hd.by = st[s].in.by;
-      hd.kx = st[s].in.kx;
-      hd.ky = st[s].in.ky;
-    }
-  }
+    });
+  });
 }
 
 // ------------------------------------------------------------------------------------------
-// Thermal: q = -(Dbwd(Q) T) assigned, T -= Dfwd(G) q
+// Thermal: q = -(Dbwd(Q) T) assigned, T -= Dfwd(G) q (same stage protocol; only T is handed
+// on, the assigned flux needs no input)
 // ------------------------------------------------------------------------------------------
 template <typename T>
-struct ThIn {
+struct ThRow {
   T t, ct, qx, qy, gx, gy;
+};
+
+template <typename T>
+struct ThCoef {
+  T qx, qy, gx, gy;
 };
 
 template <typename T>
@@ -375,96 +435,9 @@ struct ThHalo {        // left thread: T, Q of column -1; right thread: T, Q, G 
   T t, ct, q, g;
 };
 
-template <typename T>
-struct ThStage {
-  ThIn<T> in, hold;
-  T qtx, qty, qty_prev, t_prev, ct_prev, gqy_prev;
-  Cursor pc, vc1, vc2, oc0, oc1, oc2;
-};
-
-template <typename T>
-__device__ __forceinline__ void th_stage_init(ThStage<T> &st, const StepArgs<T> &a, int tile) {
-  st.pc = cursor(a.bev[0], tile);
-  st.vc1 = cursor(a.bev[1], tile);
-  st.vc2 = cursor(a.bev[2], tile);
-  st.oc0 = cursor(a.outp[0].ev, tile);
-  st.oc1 = cursor(a.outp[1].ev, tile);
-  st.oc2 = cursor(a.outp[2].ev, tile);
-  st.qtx = st.qty = st.qty_prev = st.t_prev = st.ct_prev = st.gqy_prev = T(0);
-  st.in = ThIn<T>{};
-  st.hold = ThIn<T>{};
-}
-
-template <typename T, bool KAHAN, int TX, bool HALO>
-__device__ __forceinline__ void th_pre(ThStage<T> &st, ThHalo<T> &h, const StepArgs<T> &a,
-                                       int G, int rho, const StageRows &R, int64_t step,
-                                       int64_t orow, T *sqt) {
-  const int t = threadIdx.x;
-  const bool lh = (t == 0), rh = (t == TX - 1);
-  stream_events<T, KAHAN>(a.bev[0], a.prog[0], st.pc, TX + 2, -G - 1, -1, rho, t, step, st.in.t,
-                          st.in.ct, HALO && (lh || rh), lh ? -1 : TX, h.t, h.ct);
-  stream_record<T, KAHAN>(a.outp[0], st.oc0, TX + 2, -G - 1, -1, rho, t, orow, st.in.t,
-                          st.in.ct);
-  st.qtx = st.in.qx * st.in.t;
-  st.qty = st.in.qy * st.in.t;
-  if (rho <= R.hi) {
-    sqt[t + 1] = st.qtx;
-    if (HALO && lh) sqt[0] = h.q * h.t;
-  }
-}
-
-template <typename T, bool KAHAN, int TX, bool AXI, bool HALO>
-__device__ __forceinline__ void th_post(ThStage<T> &st, const ThHalo<T> &h, const StepArgs<T> &a,
-                                        int G, int rho, const StageRows &R, int64_t step,
-                                        int64_t orow, const T *sqt, T *sgq_cur,
-                                        const T *sgq_prev, T rad, T radh, T &tn, T &ctn, T &fx,
-                                        T &fy) {
-  const int t = threadIdx.x;
-  const bool rh = (t == TX - 1);
-  T zero = T(0), zeroc = T(0), dummy = T(0), dummyc = T(0);
-  fx = fy = T(0);
-  T hfx = T(0);
-  if (rho >= R.lo && rho <= R.hi) {
-    fx = -((-sqt[t]) + st.qtx);
-    if (HALO && rh) hfx = -((-st.qtx) + h.q * h.t);
-    stream_events<T, false>(a.bev[1], a.prog[1], st.vc1, TX + 1, -G, 0, rho, t, step, fx, zeroc,
-                            HALO && rh, TX, hfx, zero);
-    stream_record<T, false>(a.outp[1], st.oc1, TX + 1, -G, 0, rho, t, orow, fx, zeroc);
-    T gqx;
-    if (AXI) gqx = (st.in.gx / rad) * (fx * rad);
-    else gqx = st.in.gx * fx;
-    sgq_cur[t] = gqx;
-    if (HALO && rh) {
-      if (AXI) sgq_cur[TX] = (h.g / radh) * (hfx * radh);
-      else sgq_cur[TX] = h.g * hfx;
-    }
-  }
-  T gqy = T(0);
-  if (rho >= R.lo) {
-    fy = -((-st.qty_prev) + st.qty);
-    stream_events<T, false>(a.bev[2], a.prog[2], st.vc2, TX, -G, 0, rho, t, step, fy, zeroc,
-                            false, 0, dummy, dummyc);
-    stream_record<T, false>(a.outp[2], st.oc2, TX, -G, 0, rho, t, orow, fy, zeroc);
-    gqy = st.in.gy * fy;
-  }
-  tn = ctn = T(0);
-  if (rho > R.lo) {
-    const T dxt = (-sgq_prev[t]) + sgq_prev[t + 1];
-    const T dyt = (-st.gqy_prev) + gqy;
-    const T d = dxt + dyt;
-    tn = st.t_prev;
-    ctn = st.ct_prev;
-    if (KAHAN) kahan_sub(tn, ctn, d); else tn = tn - d;
-  }
-  st.t_prev = st.in.t;
-  st.ct_prev = st.in.ct;
-  st.gqy_prev = gqy;
-  st.qty_prev = st.qty;
-}
-
-template <typename T, bool KAHAN, int TX, bool AXI, int S>
-__device__ void thermal_stream(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
-                               StreamLds<T, TX, S> &L) {
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool EV, bool CHK>
+__device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
+                              StreamLds<T, TX, S> &L) {
   auto &sqt = L.bwd;    // Qx*T of the row
   auto &sgq = L.fwd;    // forward x products of the row
   const int H = a.strip_h, G = a.ghost;
@@ -497,104 +470,213 @@ __device__ void thermal_stream(const StepArgs<T> &a, int tile, int64_t step, int
     }
   }
 
-  ThStage<T> st[S];
+  StageCursors cur[S];
+  if constexpr (EV) {
 #pragma unroll
-  for (int s = 0; s < S; ++s) th_stage_init(st[s], a, tile);
-  ThHalo<T> h{}, nh{};
-  ThIn<T> nin{};
+    for (int s = 0; s < S; ++s) cur[s] = stage_cursors(a, tile);
+  }
+  ThRow<T> nin[2] = {};
+  ThHalo<T> nh[2] = {};
+  T tp[2] = {}, ctp[2] = {};
+  ThCoef<T> cf[S][2] = {};
+  T qty[S][2] = {}, gqy[S][2] = {};
+  T tn[S][2] = {}, ctn[S][2] = {};
 
-  auto load_row = [&](int rho) {
-    const int j = (y0 + rho) * nx + x0 + t;
-    nin.t = ld(tin, j, n);
-    nin.qx = ld(Qx, j, n);
-    nin.qy = ld(Qy, j, n);
-    nin.gx = ld(Gx, j, n);
-    nin.gy = ld(Gy, j, n);
-    if (KAHAN) nin.ct = ld(a.cin[0], j, n);
+  auto load_row = [&](int rho, ThRow<T> &d, ThHalo<T> &h) __attribute__((always_inline)) {
+    const int rb = (y0 + rho) * nx + x0;
+    d.t = ldr<CHK>(tin, rb, t, n);
+    d.qx = ldr<CHK>(Qx, rb, t, n);
+    d.qy = ldr<CHK>(Qy, rb, t, n);
+    d.gx = ldr<CHK>(Gx, rb, t, n);
+    d.gy = ldr<CHK>(Gy, rb, t, n);
+    if (KAHAN) d.ct = ldr<CHK>(a.cin[0], rb, t, n);
     if (lh) {
-      nh.t = ld(tin, j - 1, n);
-      nh.q = ld(Qx, j - 1, n);
-      if (KAHAN) nh.ct = ld(a.cin[0], j - 1, n);
+      h.t = ldr<CHK>(tin, rb, t - 1, n);
+      h.q = ldr<CHK>(Qx, rb, t - 1, n);
+      if (KAHAN) h.ct = ldr<CHK>(a.cin[0], rb, t - 1, n);
     }
     if (rh) {
-      nh.t = ld(tin, j + 1, n);
-      nh.q = ld(Qx, j + 1, n);
-      nh.g = ld(Gx, j + 1, n);
-      if (KAHAN) nh.ct = ld(a.cin[0], j + 1, n);
+      h.t = ldr<CHK>(tin, rb, t + 1, n);
+      h.q = ldr<CHK>(Qx, rb, t + 1, n);
+      h.g = ldr<CHK>(Gx, rb, t + 1, n);
+      if (KAHAN) h.ct = ldr<CHK>(a.cin[0], rb, t + 1, n);
     }
   };
+  auto load_coef = [&](int rho, ThCoef<T> &d) __attribute__((always_inline)) {
+    const int rb = (y0 + rho) * nx + x0;
+    d.qx = ldr<CHK>(Qx, rb, t, n);
+    d.qy = ldr<CHK>(Qy, rb, t, n);
+    d.gx = ldr<CHK>(Gx, rb, t, n);
+    d.gy = ldr<CHK>(Gy, rb, t, n);
+  };
 
-  load_row(-G - 1);
+  load_row(-G - 1, nin[0], nh[0]);
   const int iters = H + 2 * G + S + 1;
-  for (int i = 0; i < iters; ++i) {
-    const int buf = i & 1;
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
+  run_pairs(iters, [&](auto Qc, int i) __attribute__((always_inline)) {
+    constexpr int Q = decltype(Qc)::value;
+    T qtx[S];
+    static_for<S>([&](auto Sc) __attribute__((always_inline)) {
+      constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
       const StageRows R(G, H, s);
-      if (rho < R.lo - 1 || rho > R.hi + 1) continue;
-      if (s == 0) {
-        st[0].in = nin;
-        h = nh;
-        if (rho < R.hi + 1) load_row(rho + 1);
-        th_pre<T, KAHAN, TX, true>(st[0], h, a, G, rho, R, step, row, sqt[0][buf]);
-      } else {
-        th_pre<T, KAHAN, TX, false>(st[s], h, a, G, rho, R, step + s, row + s, sqt[s][buf]);
+      if (s > 0 && rho + 1 >= R.lo - 1 && rho + 1 <= R.hi + 1) load_coef(rho + 1, cf[s][Q ^ 1]);
+      if (rho < R.lo - 1 || rho > R.hi + 1) return;
+      if (s == 0 && rho < R.hi + 1) load_row(rho + 1, nin[Q ^ 1], nh[Q ^ 1]);
+      T &tv = s == 0 ? nin[Q].t : tn[s > 0 ? s - 1 : 0][Q ^ 1];
+      T &ct = s == 0 ? nin[Q].ct : ctn[s > 0 ? s - 1 : 0][Q ^ 1];
+      const T cqx = s == 0 ? nin[Q].qx : cf[s][Q].qx;
+      const T cqy = s == 0 ? nin[Q].qy : cf[s][Q].qy;
+      ThHalo<T> &h = nh[Q];
+      if constexpr (EV) {
+        stream_events<T, KAHAN>(a.bev[0], a.prog[0], cur[s].b[0], TX + 2, -G - 1, -1, rho, t,
+                                step + s, tv, ct, s == 0 && (lh || rh), lh ? -1 : TX, h.t, h.ct);
+        stream_record<T, KAHAN>(a.outp[0], cur[s].o[0], TX + 2, -G - 1, -1, rho, t, row + s, tv,
+                                ct);
       }
-    }
+      qtx[s] = cqx * tv;
+      qty[s][Q] = cqy * tv;
+      if (rho <= R.hi) {
+        sqt[s][Q][t + 1] = qtx[s];
+        if (s == 0 && lh) sqt[s][Q][0] = h.q * h.t;
+      }
+      if (s == 0) {
+        tp[Q] = tv;
+        ctp[Q] = ct;
+      }
+    });
     __syncthreads();
-#pragma unroll
-    for (int s = S - 1; s >= 0; --s) {
+    static_for<S, true>([&](auto Sc) __attribute__((always_inline)) {
+      constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
       const StageRows R(G, H, s);
-      if (rho < R.lo - 1 || rho > R.hi + 1) continue;
-      T tn, ctn, fx, fy;
-      if (s == 0)
-        th_post<T, KAHAN, TX, AXI, true>(st[0], h, a, G, rho, R, step, row, sqt[0][buf],
-                                         sgq[0][buf], sgq[0][buf ^ 1], rad, radh, tn, ctn, fx,
-                                         fy);
-      else
-        th_post<T, KAHAN, TX, AXI, false>(st[s], h, a, G, rho, R, step + s, row + s,
-                                          sqt[s][buf], sgq[s][buf], sgq[s][buf ^ 1], rad, radh,
-                                          tn, ctn, fx, fy);
+      if (rho < R.lo - 1 || rho > R.hi + 1) return;
+      const int sp = s > 0 ? s - 1 : 0;
+      const T cgx = s == 0 ? nin[Q].gx : cf[s][Q].gx;
+      const T cgy = s == 0 ? nin[Q].gy : cf[s][Q].gy;
+      const T tprev = s == 0 ? tp[Q ^ 1] : tn[sp][Q];
+      const T ctprev = s == 0 ? ctp[Q ^ 1] : ctn[sp][Q];
+      const ThHalo<T> &h = nh[Q];
+      T zero = T(0), zeroc = T(0);
+      T fx = T(0), fy = T(0);
+      if (rho >= R.lo && rho <= R.hi) {
+        fx = -((-sqt[s][Q][t]) + qtx[s]);
+        T hfx = T(0);
+        if (s == 0 && rh) hfx = -((-qtx[s]) + h.q * h.t);
+        if constexpr (EV) {
+          stream_events<T, false>(a.bev[1], a.prog[1], cur[s].b[1], TX + 1, -G, 0, rho, t,
+                                  step + s, fx, zeroc, s == 0 && rh, TX, hfx, zero);
+          stream_record<T, false>(a.outp[1], cur[s].o[1], TX + 1, -G, 0, rho, t, row + s, fx,
+                                  zeroc);
+        }
+        if (AXI) sgq[s][Q][t] = (cgx / rad) * (fx * rad);
+        else sgq[s][Q][t] = cgx * fx;
+        if (s == 0 && rh) {
+          if (AXI) sgq[s][Q][TX] = (h.g / radh) * (hfx * radh);
+          else sgq[s][Q][TX] = h.g * hfx;
+        }
+      }
+      gqy[s][Q] = T(0);
+      if (rho >= R.lo) {
+        fy = -((-qty[s][Q ^ 1]) + qty[s][Q]);
+        if constexpr (EV) {
+          T dummy = T(0), dummyc = T(0);
+          stream_events<T, false>(a.bev[2], a.prog[2], cur[s].b[2], TX, -G, 0, rho, t,
+                                  step + s, fy, zeroc, false, 0, dummy, dummyc);
+          stream_record<T, false>(a.outp[2], cur[s].o[2], TX, -G, 0, rho, t, row + s, fy,
+                                  zeroc);
+        }
+        gqy[s][Q] = cgy * fy;
+      }
+      T tnew = T(0), ctnew = T(0);
+      if (rho > R.lo) {
+        const T dxt = (-sgq[s][Q ^ 1][t]) + sgq[s][Q ^ 1][t + 1];
+        const T dyt = (-gqy[s][Q ^ 1]) + gqy[s][Q];
+        const T d = dxt + dyt;
+        tnew = tprev;
+        ctnew = ctprev;
+        if (KAHAN) kahan_sub(tnew, ctnew, d); else tnew = tnew - d;
+      }
       if (s == S - 1) {
         if (owned && rho >= 1 && rho - 1 < H && y0 + rho - 1 < a.ny) {
-          const int jp = (y0 + rho - 1) * nx + x0 + t;
-          a.out[0][jp] = tn;
-          if (KAHAN) a.cout[0][jp] = ctn;
+          const int rb = (y0 + rho - 1) * nx + x0;
+          (a.out[0] + rb)[t] = tnew;
+          if (KAHAN) (a.cout[0] + rb)[t] = ctnew;
         }
         if (store && owned && rho >= 0 && rho < H && y0 + rho < a.ny) {
-          const int jv = (y0 + rho) * nx + x0 + t;
-          a.out[1][jv] = fx;
-          a.out[2][jv] = fy;
+          const int rb = (y0 + rho) * nx + x0;
+          (a.out[1] + rb)[t] = fx;
+          (a.out[2] + rb)[t] = fy;
         }
       } else {
-        ThIn<T> &nxt = st[s + 1 < S ? s + 1 : s].in;
-        nxt = st[s].hold;
-        nxt.t = tn;
-        nxt.ct = ctn;
+        tn[s][Q] = tnew;
+        ctn[s][Q] = ctnew;
       }
-      ThIn<T> &hd = st[s].hold;
-      hd.qx = st[s].in.qx;
-      hd.qy = st[s].in.qy;
-      hd.gx = st[s].in.gx;
-      hd.gy = st[s].in.gy;
-    }
+    });
+  });
+}
+
+// Path of a tile: bit 1 = it has events, bit 0 = all its reads fall inside the arrays.  The
+// value is uniform; variants are selected with one switch (a flat switch keeps each variant's
+// registers to itself, nested branches over the four inlined strips do not).
+template <typename T, int TX>
+__device__ __forceinline__ int strip_path(const StepArgs<T> &a, int tile) {
+  const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+  const int G = a.ghost;
+  const bool inside = strip_inside(tx * (TX - 2 * G) - G, ty * a.strip_h, a.strip_h, G, TX,
+                                   a.nx, a.n);
+  return __builtin_amdgcn_readfirstlane((tile_has_events(a, tile) ? 2 : 0) | (inside ? 1 : 0));
+}
+
+template <typename T, bool KAHAN, int TX, bool AXI, int S>
+__device__ __forceinline__ void acoustic_stream(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
+                                StreamLds<T, TX, S> &L) {
+  switch (strip_path<T, TX>(a, tile)) {
+    case 0: acoustic_strip<T, KAHAN, TX, AXI, S, false, true>(a, tile, step, row, L); break;
+    case 1: acoustic_strip<T, KAHAN, TX, AXI, S, false, false>(a, tile, step, row, L); break;
+    case 2: acoustic_strip<T, KAHAN, TX, AXI, S, true, true>(a, tile, step, row, L); break;
+    default: acoustic_strip<T, KAHAN, TX, AXI, S, true, false>(a, tile, step, row, L); break;
   }
 }
 
-// One tile of instance `a` in a batched streaming launch (lossless models only).
+template <typename T, bool KAHAN, int TX, bool AXI, int S>
+__device__ __forceinline__ void thermal_stream(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
+                               StreamLds<T, TX, S> &L) {
+  switch (strip_path<T, TX>(a, tile)) {
+    case 0: thermal_strip<T, KAHAN, TX, AXI, S, false, true>(a, tile, step, row, L); break;
+    case 1: thermal_strip<T, KAHAN, TX, AXI, S, false, false>(a, tile, step, row, L); break;
+    case 2: thermal_strip<T, KAHAN, TX, AXI, S, true, true>(a, tile, step, row, L); break;
+    default: thermal_strip<T, KAHAN, TX, AXI, S, true, false>(a, tile, step, row, L); break;
+  }
+}
+
+// One tile of instance `a` in a batched streaming launch (lossless models only): one switch
+// over model variant and path.
 template <typename T, bool KAHAN, int TX, int S>
 __device__ __forceinline__ void any_stream(const StepArgs<T> &a, int tile, int64_t step,
                                            int64_t row, StreamLds<T, TX, S> &L) {
   const int f = a.flags;
-  if (f & kFlagThermal) {
-    if (f & kFlagAxi) thermal_stream<T, KAHAN, TX, true, S>(a, tile, step, row, L);
-    else thermal_stream<T, KAHAN, TX, false, S>(a, tile, step, row, L);
-  } else if (f & kFlagAxi) {
-    acoustic_stream<T, KAHAN, TX, true, S>(a, tile, step, row, L);
-  } else {
-    acoustic_stream<T, KAHAN, TX, false, S>(a, tile, step, row, L);
+  const int model = ((f & kFlagThermal) ? 2 : 0) | ((f & kFlagAxi) ? 1 : 0);
+  switch (__builtin_amdgcn_readfirstlane(model * 4 + strip_path<T, TX>(a, tile))) {
+#define FDS_STREAM_CASE(M, V, FN, AXI, EV, CHK)                                                   \
+  case M * 4 + V: FN<T, KAHAN, TX, AXI, S, EV, CHK>(a, tile, step, row, L); break;
+    FDS_STREAM_CASE(0, 0, acoustic_strip, false, false, true)
+    FDS_STREAM_CASE(0, 1, acoustic_strip, false, false, false)
+    FDS_STREAM_CASE(0, 2, acoustic_strip, false, true, true)
+    FDS_STREAM_CASE(0, 3, acoustic_strip, false, true, false)
+    FDS_STREAM_CASE(1, 0, acoustic_strip, true, false, true)
+    FDS_STREAM_CASE(1, 1, acoustic_strip, true, false, false)
+    FDS_STREAM_CASE(1, 2, acoustic_strip, true, true, true)
+    FDS_STREAM_CASE(1, 3, acoustic_strip, true, true, false)
+    FDS_STREAM_CASE(2, 0, thermal_strip, false, false, true)
+    FDS_STREAM_CASE(2, 1, thermal_strip, false, false, false)
+    FDS_STREAM_CASE(2, 2, thermal_strip, false, true, true)
+    FDS_STREAM_CASE(2, 3, thermal_strip, false, true, false)
+    FDS_STREAM_CASE(3, 0, thermal_strip, true, false, true)
+    FDS_STREAM_CASE(3, 1, thermal_strip, true, false, false)
+    FDS_STREAM_CASE(3, 2, thermal_strip, true, true, true)
+    FDS_STREAM_CASE(3, 3, thermal_strip, true, true, false)
+#undef FDS_STREAM_CASE
+    default: break;
   }
 }
 
