@@ -11,16 +11,19 @@ constexpr int kNumTiles = 6;
 constexpr int kTileShapes[kNumTiles][2] = {
     {64, 16}, {64, 8}, {128, 8}, {256, 4}, {128, 4}, {256, 1}};
 constexpr int kDefaultTile2D = 1;       // default of the tile engine (lossy models)
-constexpr int kDefaultLossless2D = kNumTiles;   // lossless 2D: streaming 256 x 16
+constexpr int kDefaultLossless2D = kNumTiles + 2;   // lossless 2D: streaming 256 x 16, 2 steps
 constexpr int kTileRow = 5;
 
 // Row-streaming engine (lossless 2D models): TX threads = columns, strips of TY rows, S time
-// steps fused per launch.  Shape indices continue the tile table: index kNumTiles + i is
-// kStreamShapes[i] = {TX, TY, S}.
+// steps fused per launch, NT = non-temporal state reads and writes.  Shape indices continue the
+// tile table: index kNumTiles + i is kStreamShapes[i] = {TX, TY, S, NT}.  Every fused shape has
+// a one-stage shape of the same width and NT (its remainder kernel).
 constexpr int kNumStream = 8;
-constexpr int kStreamShapes[kNumStream][3] = {{256, 16, 1}, {256, 32, 1}, {128, 16, 1},
-                                              {256, 8, 1},  {256, 16, 2}, {256, 32, 2},
-                                              {128, 32, 2}, {256, 32, 3}};
+constexpr int kStreamShapes[kNumStream][4] = {{256, 16, 1, 0}, {128, 16, 1, 0}, {256, 16, 2, 0},
+                                              {128, 32, 2, 0}, {256, 32, 3, 0}, {256, 16, 1, 1},
+                                              {256, 16, 2, 1}, {256, 32, 3, 1}};
+// X-macro over the streaming shape indices (each shape is its own translation unit).
+#define FDS_STREAM_SHAPES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
 
 using LaunchFn = void (*)(const void *args, int nblocks, size_t lds_bytes, hipStream_t s);
 

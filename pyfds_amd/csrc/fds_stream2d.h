@@ -99,6 +99,23 @@ __device__ __forceinline__ T ldr(const T *p, int rb, int off, int n) {
   return ld<true>(p, rb + off, n);
 }
 
+// Read of a state value that the launch reads once (NT: non-temporal, it does not displace
+// the coefficients and ghost rows that later stages and neighbouring strips re-read).
+template <bool NT, bool CHK, typename T>
+__device__ __forceinline__ T ldn(const T *p, int rb, int off, int n) {
+  if (!NT) return ldr<CHK>(p, rb, off, n);
+  if (!CHK) return __builtin_nontemporal_load(p + rb + off);
+  const int j = rb + off;
+  return (j >= 0 && j < n) ? __builtin_nontemporal_load(p + j) : T(0);
+}
+
+// Store of a new state value (NT: non-temporal, nothing reads it again within the launch).
+template <bool NT, typename T>
+__device__ __forceinline__ void stn(T *p, int rb, int off, T v) {
+  if (NT) __builtin_nontemporal_store(v, p + rb + off);
+  else (p + rb)[off] = v;
+}
+
 // Whether every read of a streaming tile (rows y0-G-1 .. y0+H+G, columns x0-1 .. x0+TX) falls
 // inside the array, so its loads need no bounds checks.
 __device__ __forceinline__ bool strip_inside(int x0, int y0, int H, int G, int TX, int nx,
@@ -161,22 +178,30 @@ __device__ __forceinline__ void static_for(F &&f) {
   static_for_impl<N, REV>(f, std::make_integer_sequence<int, N>{});
 }
 
-// The main loop runs iterations i = 0 .. iters-1 as pairs so that every value carried from one
-// iteration to the next lives in a register pair indexed by the iteration's parity (no copies).
-template <typename F>
-__device__ __forceinline__ void run_pairs(int iters, F &&iter) {
-  int i = 0;
-  for (; i + 1 < iters; i += 2) {
-    iter(std::integral_constant<int, 0>{}, i);
-    iter(std::integral_constant<int, 1>{}, i + 1);
-  }
-  if (i < iters) iter(std::integral_constant<int, 0>{}, i);
+// The main loop runs iterations i = 0 .. iters-1 in blocks of U = lcm(2, RS) copies of the body,
+// each copy with its iteration's parity Q = i % 2 and ring slot W = i % RS as compile-time
+// constants, so values carried between iterations live in registers indexed by Q (or W) and
+// never need copying.  Iterations past the end are skipped (a uniform branch).
+template <int RS, typename F, int... J>
+__device__ __forceinline__ void run_block(F &f, int i, int iters,
+                                          std::integer_sequence<int, J...>) {
+  ((i + J < iters
+        ? (f(std::integral_constant<int, J % 2>{}, std::integral_constant<int, J % RS>{}, i + J),
+           0)
+        : 0),
+   ...);
+}
+
+template <int RS, typename F>
+__device__ __forceinline__ void run_loop(int iters, F &&f) {
+  constexpr int U = RS % 2 == 0 ? RS : 2 * RS;
+  for (int i = 0; i < iters; i += U) run_block<RS>(f, i, iters, std::make_integer_sequence<int, U>{});
 }
 
 // ------------------------------------------------------------------------------------------
 // Acoustic (lossless): v -= Dbwd(B) p, then p -= Dfwd(K) v
 //
-// Stage s at an iteration of parity Q reads: stage 0 the row loaded into nin[Q]; stage s > 0
+// Stage s at an iteration of parity Q reads: stage 0 the row loaded into its ring slot; stage s > 0
 // p from pn[s-1][Q^1] (finished by stage s-1 one iteration ago), v from vh[s-1][Q] (two
 // iterations ago) and its previous row's p from pn[s-1][Q]; its coefficients it loads itself
 // (one iteration ahead, from cache).  Consumers run their post-barrier part before producers,
@@ -202,9 +227,10 @@ struct AcVel {
   T vx, cvx, vy, cvy;
 };
 
-// S stages over one strip (S <= G + 1).  EV: the tile has events; CHK: some reads fall
-// outside the arrays.
-template <typename T, bool KAHAN, int TX, bool AXI, int S, bool EV, bool CHK>
+// S stages over one strip (S <= G + 1).  NT: state reads and writes bypass the caches'
+// temporal allocation (pays off when the working set is far larger than the caches); EV: the
+// tile has events; CHK: some reads fall outside the arrays.
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool EV, bool CHK>
 __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
                                StreamLds<T, TX, S> &L) {
   auto &sbp = L.bwd;    // Bx*p of the row, columns -1..TX
@@ -243,8 +269,9 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
 #pragma unroll
     for (int s = 0; s < S; ++s) cur[s] = stage_cursors(a, tile);
   }
-  AcRow<T> nin[2] = {};        // stage 0 input rows
-  AcHalo<T> nh[2] = {};        // stage 0 halo
+  constexpr int PF = 1, RS = PF + 1;   // stage 0 loads its rows one iteration ahead
+  AcRow<T> nin[RS] = {};       // stage 0 input rows (ring)
+  AcHalo<T> nh[RS] = {};       // stage 0 halo
   T pp[2] = {}, cpp[2] = {};   // stage 0 input p after its events (previous-row p)
   AcCoef<T> cf[S][2] = {};     // stages > 0: coefficients
   T bpy[S][2] = {}, kvy[S][2] = {};
@@ -253,17 +280,17 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
 
   auto load_row = [&](int rho, AcRow<T> &d, AcHalo<T> &h) __attribute__((always_inline)) {
     const int rb = (y0 + rho) * nx + x0;
-    d.p = ldr<CHK>(pin, rb, t, n);
-    d.vx = ldr<CHK>(vxin, rb, t, n);
-    d.vy = ldr<CHK>(vyin, rb, t, n);
+    d.p = ldn<NT, CHK>(pin, rb, t, n);
+    d.vx = ldn<NT, CHK>(vxin, rb, t, n);
+    d.vy = ldn<NT, CHK>(vyin, rb, t, n);
     d.bx = ldr<CHK>(Bx, rb, t, n);
     d.by = ldr<CHK>(By, rb, t, n);
     d.kx = ldr<CHK>(Kx, rb, t, n);
     d.ky = ldr<CHK>(Ky, rb, t, n);
     if (KAHAN) {
-      d.cp = ldr<CHK>(a.cin[0], rb, t, n);
-      d.cvx = ldr<CHK>(a.cin[1], rb, t, n);
-      d.cvy = ldr<CHK>(a.cin[2], rb, t, n);
+      d.cp = ldn<NT, CHK>(a.cin[0], rb, t, n);
+      d.cvx = ldn<NT, CHK>(a.cin[1], rb, t, n);
+      d.cvy = ldn<NT, CHK>(a.cin[2], rb, t, n);
     }
     if (lh) {
       h.p = ldr<CHK>(pin, rb, t - 1, n);
@@ -289,10 +316,11 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
     d.ky = ldr<CHK>(Ky, rb, t, n);
   };
 
-  load_row(-G - 1, nin[0], nh[0]);
+#pragma unroll
+  for (int r = 0; r < PF; ++r) load_row(-G - 1 + r, nin[r], nh[r]);
   const int iters = H + 2 * G + S + 1;
-  run_pairs(iters, [&](auto Qc, int i) __attribute__((always_inline)) {
-    constexpr int Q = decltype(Qc)::value;
+  run_loop<RS>(iters, [&](auto Qc, auto Wc, int i) __attribute__((always_inline)) {
+    constexpr int Q = decltype(Qc)::value, W = decltype(Wc)::value;
     T bpx[S];
     // ---- before the barrier
     static_for<S>([&](auto Sc) __attribute__((always_inline)) {
@@ -301,12 +329,12 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       const StageRows R(G, H, s);
       if (s > 0 && rho + 1 >= R.lo - 1 && rho + 1 <= R.hi + 1) load_coef(rho + 1, cf[s][Q ^ 1]);
       if (rho < R.lo - 1 || rho > R.hi + 1) return;
-      if (s == 0 && rho < R.hi + 1) load_row(rho + 1, nin[Q ^ 1], nh[Q ^ 1]);
-      T &p = s == 0 ? nin[Q].p : pn[s > 0 ? s - 1 : 0][Q ^ 1];
-      T &cp = s == 0 ? nin[Q].cp : cpn[s > 0 ? s - 1 : 0][Q ^ 1];
-      const T cbx = s == 0 ? nin[Q].bx : cf[s][Q].bx;
-      const T cby = s == 0 ? nin[Q].by : cf[s][Q].by;
-      AcHalo<T> &h = nh[Q];
+      if (s == 0 && rho + PF <= R.hi + 1) load_row(rho + PF, nin[(W + PF) % RS], nh[(W + PF) % RS]);
+      T &p = s == 0 ? nin[W].p : pn[s > 0 ? s - 1 : 0][Q ^ 1];
+      T &cp = s == 0 ? nin[W].cp : cpn[s > 0 ? s - 1 : 0][Q ^ 1];
+      const T cbx = s == 0 ? nin[W].bx : cf[s][Q].bx;
+      const T cby = s == 0 ? nin[W].by : cf[s][Q].by;
+      AcHalo<T> &h = nh[W];
       if constexpr (EV) {
         stream_events<T, KAHAN>(a.bev[0], a.prog[0], cur[s].b[0], TX + 2, -G - 1, -1, rho, t,
                                 step + s, p, cp, s == 0 && (lh || rh), lh ? -1 : TX, h.p, h.cp);
@@ -332,15 +360,15 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       const StageRows R(G, H, s);
       if (rho < R.lo - 1 || rho > R.hi + 1) return;
       const int sp = s > 0 ? s - 1 : 0;
-      const T ivx = s == 0 ? nin[Q].vx : vh[sp][Q].vx;
-      const T icvx = s == 0 ? nin[Q].cvx : vh[sp][Q].cvx;
-      const T ivy = s == 0 ? nin[Q].vy : vh[sp][Q].vy;
-      const T icvy = s == 0 ? nin[Q].cvy : vh[sp][Q].cvy;
-      const T ckx = s == 0 ? nin[Q].kx : cf[s][Q].kx;
-      const T cky = s == 0 ? nin[Q].ky : cf[s][Q].ky;
+      const T ivx = s == 0 ? nin[W].vx : vh[sp][Q].vx;
+      const T icvx = s == 0 ? nin[W].cvx : vh[sp][Q].cvx;
+      const T ivy = s == 0 ? nin[W].vy : vh[sp][Q].vy;
+      const T icvy = s == 0 ? nin[W].cvy : vh[sp][Q].cvy;
+      const T ckx = s == 0 ? nin[W].kx : cf[s][Q].kx;
+      const T cky = s == 0 ? nin[W].ky : cf[s][Q].ky;
       const T pprev = s == 0 ? pp[Q ^ 1] : pn[sp][Q];
       const T cpprev = s == 0 ? cpp[Q ^ 1] : cpn[sp][Q];
-      const AcHalo<T> &h = nh[Q];
+      const AcHalo<T> &h = nh[W];
       T vxn = T(0), cvxn = T(0), vyn = T(0), cvyn = T(0);
       if (rho >= R.lo && rho <= R.hi) {
         const T d = (-sbp[s][Q][t]) + bpx[s];
@@ -395,16 +423,16 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
         // last stage: write the owned cells
         if (owned && rho >= 1 && rho - 1 < H && y0 + rho - 1 < a.ny) {
           const int rb = (y0 + rho - 1) * nx + x0;
-          (a.out[0] + rb)[t] = pnew;
-          if (KAHAN) (a.cout[0] + rb)[t] = cpnew;
+          stn<NT>(a.out[0], rb, t, pnew);
+          if (KAHAN) stn<NT>(a.cout[0], rb, t, cpnew);
         }
         if (owned && rho >= 0 && rho < H && y0 + rho < a.ny) {
           const int rb = (y0 + rho) * nx + x0;
-          (a.out[1] + rb)[t] = vxn;
-          (a.out[2] + rb)[t] = vyn;
+          stn<NT>(a.out[1], rb, t, vxn);
+          stn<NT>(a.out[2], rb, t, vyn);
           if (KAHAN) {
-            (a.cout[1] + rb)[t] = cvxn;
-            (a.cout[2] + rb)[t] = cvyn;
+            stn<NT>(a.cout[1], rb, t, cvxn);
+            stn<NT>(a.cout[2], rb, t, cvyn);
           }
         }
       } else {
@@ -435,7 +463,7 @@ struct ThHalo {        // left thread: T, Q of column -1; right thread: T, Q, G 
   T t, ct, q, g;
 };
 
-template <typename T, bool KAHAN, int TX, bool AXI, int S, bool EV, bool CHK>
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool EV, bool CHK>
 __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
                               StreamLds<T, TX, S> &L) {
   auto &sqt = L.bwd;    // Qx*T of the row
@@ -475,8 +503,9 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
 #pragma unroll
     for (int s = 0; s < S; ++s) cur[s] = stage_cursors(a, tile);
   }
-  ThRow<T> nin[2] = {};
-  ThHalo<T> nh[2] = {};
+  constexpr int PF = 1, RS = PF + 1;
+  ThRow<T> nin[RS] = {};
+  ThHalo<T> nh[RS] = {};
   T tp[2] = {}, ctp[2] = {};
   ThCoef<T> cf[S][2] = {};
   T qty[S][2] = {}, gqy[S][2] = {};
@@ -484,12 +513,12 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
 
   auto load_row = [&](int rho, ThRow<T> &d, ThHalo<T> &h) __attribute__((always_inline)) {
     const int rb = (y0 + rho) * nx + x0;
-    d.t = ldr<CHK>(tin, rb, t, n);
+    d.t = ldn<NT, CHK>(tin, rb, t, n);
     d.qx = ldr<CHK>(Qx, rb, t, n);
     d.qy = ldr<CHK>(Qy, rb, t, n);
     d.gx = ldr<CHK>(Gx, rb, t, n);
     d.gy = ldr<CHK>(Gy, rb, t, n);
-    if (KAHAN) d.ct = ldr<CHK>(a.cin[0], rb, t, n);
+    if (KAHAN) d.ct = ldn<NT, CHK>(a.cin[0], rb, t, n);
     if (lh) {
       h.t = ldr<CHK>(tin, rb, t - 1, n);
       h.q = ldr<CHK>(Qx, rb, t - 1, n);
@@ -510,10 +539,11 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
     d.gy = ldr<CHK>(Gy, rb, t, n);
   };
 
-  load_row(-G - 1, nin[0], nh[0]);
+#pragma unroll
+  for (int r = 0; r < PF; ++r) load_row(-G - 1 + r, nin[r], nh[r]);
   const int iters = H + 2 * G + S + 1;
-  run_pairs(iters, [&](auto Qc, int i) __attribute__((always_inline)) {
-    constexpr int Q = decltype(Qc)::value;
+  run_loop<RS>(iters, [&](auto Qc, auto Wc, int i) __attribute__((always_inline)) {
+    constexpr int Q = decltype(Qc)::value, W = decltype(Wc)::value;
     T qtx[S];
     static_for<S>([&](auto Sc) __attribute__((always_inline)) {
       constexpr int s = decltype(Sc)::value;
@@ -521,12 +551,12 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
       const StageRows R(G, H, s);
       if (s > 0 && rho + 1 >= R.lo - 1 && rho + 1 <= R.hi + 1) load_coef(rho + 1, cf[s][Q ^ 1]);
       if (rho < R.lo - 1 || rho > R.hi + 1) return;
-      if (s == 0 && rho < R.hi + 1) load_row(rho + 1, nin[Q ^ 1], nh[Q ^ 1]);
-      T &tv = s == 0 ? nin[Q].t : tn[s > 0 ? s - 1 : 0][Q ^ 1];
-      T &ct = s == 0 ? nin[Q].ct : ctn[s > 0 ? s - 1 : 0][Q ^ 1];
-      const T cqx = s == 0 ? nin[Q].qx : cf[s][Q].qx;
-      const T cqy = s == 0 ? nin[Q].qy : cf[s][Q].qy;
-      ThHalo<T> &h = nh[Q];
+      if (s == 0 && rho + PF <= R.hi + 1) load_row(rho + PF, nin[(W + PF) % RS], nh[(W + PF) % RS]);
+      T &tv = s == 0 ? nin[W].t : tn[s > 0 ? s - 1 : 0][Q ^ 1];
+      T &ct = s == 0 ? nin[W].ct : ctn[s > 0 ? s - 1 : 0][Q ^ 1];
+      const T cqx = s == 0 ? nin[W].qx : cf[s][Q].qx;
+      const T cqy = s == 0 ? nin[W].qy : cf[s][Q].qy;
+      ThHalo<T> &h = nh[W];
       if constexpr (EV) {
         stream_events<T, KAHAN>(a.bev[0], a.prog[0], cur[s].b[0], TX + 2, -G - 1, -1, rho, t,
                                 step + s, tv, ct, s == 0 && (lh || rh), lh ? -1 : TX, h.t, h.ct);
@@ -551,11 +581,11 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
       const StageRows R(G, H, s);
       if (rho < R.lo - 1 || rho > R.hi + 1) return;
       const int sp = s > 0 ? s - 1 : 0;
-      const T cgx = s == 0 ? nin[Q].gx : cf[s][Q].gx;
-      const T cgy = s == 0 ? nin[Q].gy : cf[s][Q].gy;
+      const T cgx = s == 0 ? nin[W].gx : cf[s][Q].gx;
+      const T cgy = s == 0 ? nin[W].gy : cf[s][Q].gy;
       const T tprev = s == 0 ? tp[Q ^ 1] : tn[sp][Q];
       const T ctprev = s == 0 ? ctp[Q ^ 1] : ctn[sp][Q];
-      const ThHalo<T> &h = nh[Q];
+      const ThHalo<T> &h = nh[W];
       T zero = T(0), zeroc = T(0);
       T fx = T(0), fy = T(0);
       if (rho >= R.lo && rho <= R.hi) {
@@ -599,13 +629,13 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
       if (s == S - 1) {
         if (owned && rho >= 1 && rho - 1 < H && y0 + rho - 1 < a.ny) {
           const int rb = (y0 + rho - 1) * nx + x0;
-          (a.out[0] + rb)[t] = tnew;
-          if (KAHAN) (a.cout[0] + rb)[t] = ctnew;
+          stn<NT>(a.out[0], rb, t, tnew);
+          if (KAHAN) stn<NT>(a.cout[0], rb, t, ctnew);
         }
         if (store && owned && rho >= 0 && rho < H && y0 + rho < a.ny) {
           const int rb = (y0 + rho) * nx + x0;
-          (a.out[1] + rb)[t] = fx;
-          (a.out[2] + rb)[t] = fy;
+          stn<NT>(a.out[1], rb, t, fx);
+          stn<NT>(a.out[2], rb, t, fy);
         }
       } else {
         tn[s][Q] = tnew;
@@ -627,38 +657,38 @@ __device__ __forceinline__ int strip_path(const StepArgs<T> &a, int tile) {
   return __builtin_amdgcn_readfirstlane((tile_has_events(a, tile) ? 2 : 0) | (inside ? 1 : 0));
 }
 
-template <typename T, bool KAHAN, int TX, bool AXI, int S>
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT>
 __device__ __forceinline__ void acoustic_stream(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
                                 StreamLds<T, TX, S> &L) {
   switch (strip_path<T, TX>(a, tile)) {
-    case 0: acoustic_strip<T, KAHAN, TX, AXI, S, false, true>(a, tile, step, row, L); break;
-    case 1: acoustic_strip<T, KAHAN, TX, AXI, S, false, false>(a, tile, step, row, L); break;
-    case 2: acoustic_strip<T, KAHAN, TX, AXI, S, true, true>(a, tile, step, row, L); break;
-    default: acoustic_strip<T, KAHAN, TX, AXI, S, true, false>(a, tile, step, row, L); break;
+    case 0: acoustic_strip<T, KAHAN, TX, AXI, S, NT, false, true>(a, tile, step, row, L); break;
+    case 1: acoustic_strip<T, KAHAN, TX, AXI, S, NT, false, false>(a, tile, step, row, L); break;
+    case 2: acoustic_strip<T, KAHAN, TX, AXI, S, NT, true, true>(a, tile, step, row, L); break;
+    default: acoustic_strip<T, KAHAN, TX, AXI, S, NT, true, false>(a, tile, step, row, L); break;
   }
 }
 
-template <typename T, bool KAHAN, int TX, bool AXI, int S>
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT>
 __device__ __forceinline__ void thermal_stream(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
                                StreamLds<T, TX, S> &L) {
   switch (strip_path<T, TX>(a, tile)) {
-    case 0: thermal_strip<T, KAHAN, TX, AXI, S, false, true>(a, tile, step, row, L); break;
-    case 1: thermal_strip<T, KAHAN, TX, AXI, S, false, false>(a, tile, step, row, L); break;
-    case 2: thermal_strip<T, KAHAN, TX, AXI, S, true, true>(a, tile, step, row, L); break;
-    default: thermal_strip<T, KAHAN, TX, AXI, S, true, false>(a, tile, step, row, L); break;
+    case 0: thermal_strip<T, KAHAN, TX, AXI, S, NT, false, true>(a, tile, step, row, L); break;
+    case 1: thermal_strip<T, KAHAN, TX, AXI, S, NT, false, false>(a, tile, step, row, L); break;
+    case 2: thermal_strip<T, KAHAN, TX, AXI, S, NT, true, true>(a, tile, step, row, L); break;
+    default: thermal_strip<T, KAHAN, TX, AXI, S, NT, true, false>(a, tile, step, row, L); break;
   }
 }
 
 // One tile of instance `a` in a batched streaming launch (lossless models only): one switch
 // over model variant and path.
-template <typename T, bool KAHAN, int TX, int S>
+template <typename T, bool KAHAN, int TX, int S, bool NT>
 __device__ __forceinline__ void any_stream(const StepArgs<T> &a, int tile, int64_t step,
                                            int64_t row, StreamLds<T, TX, S> &L) {
   const int f = a.flags;
   const int model = ((f & kFlagThermal) ? 2 : 0) | ((f & kFlagAxi) ? 1 : 0);
   switch (__builtin_amdgcn_readfirstlane(model * 4 + strip_path<T, TX>(a, tile))) {
 #define FDS_STREAM_CASE(M, V, FN, AXI, EV, CHK)                                                   \
-  case M * 4 + V: FN<T, KAHAN, TX, AXI, S, EV, CHK>(a, tile, step, row, L); break;
+  case M * 4 + V: FN<T, KAHAN, TX, AXI, S, NT, EV, CHK>(a, tile, step, row, L); break;
     FDS_STREAM_CASE(0, 0, acoustic_strip, false, false, true)
     FDS_STREAM_CASE(0, 1, acoustic_strip, false, false, false)
     FDS_STREAM_CASE(0, 2, acoustic_strip, false, true, true)

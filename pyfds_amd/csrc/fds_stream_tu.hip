@@ -1,0 +1,106 @@
+// Streaming step kernels of one numeric form and one shape of the streaming table.  The build
+// compiles this file once per form (FDS_FORM 0: float64, 1: float32, 2: float32 + Kahan) and
+// per shape index (FDS_SHAPE), so the shapes compile in parallel.
+#include "fds_kernels.h"
+#include "fds_stream2d.h"
+
+#if !defined(FDS_FORM) || !defined(FDS_SHAPE)
+#error "compile with -DFDS_FORM=<0|1|2> -DFDS_SHAPE=<index>"
+#endif
+
+#if FDS_FORM == 0
+#define FDS_FORM_NS f64
+#define FDS_T double
+#define FDS_KAHAN false
+#elif FDS_FORM == 1
+#define FDS_FORM_NS f32
+#define FDS_T float
+#define FDS_KAHAN false
+#else
+#define FDS_FORM_NS k32
+#define FDS_T float
+#define FDS_KAHAN true
+#endif
+
+#define FDS_CAT2(a, b) a##b
+#define FDS_CAT(a, b) FDS_CAT2(a, b)
+
+namespace fds {
+namespace FDS_FORM_NS {
+
+using T = FDS_T;
+constexpr bool K = FDS_KAHAN;
+
+template <int TX, bool AXI, int S, bool NT>
+__global__ __launch_bounds__(TX) void k_stream_acoustic(StepArgs<T> a) {
+  __shared__ StreamLds<T, TX, S> L;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x, a.ntiles);
+  if (tile < 0) return;
+  acoustic_stream<T, K, TX, AXI, S, NT>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
+}
+
+template <int TX, bool AXI, int S, bool NT>
+__global__ __launch_bounds__(TX) void k_stream_thermal(StepArgs<T> a) {
+  __shared__ StreamLds<T, TX, S> L;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x, a.ntiles);
+  if (tile < 0) return;
+  thermal_stream<T, K, TX, AXI, S, NT>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
+}
+
+template <int TX, int S, bool NT>
+__global__ __launch_bounds__(TX) void k_stream_batch(const StepArgs<T> *__restrict__ descs,
+                                                     const int32_t *__restrict__ tile_inst,
+                                                     const int32_t *__restrict__ inst_base,
+                                                     int ntiles, int k) {
+  __shared__ StreamLds<T, TX, S> L;
+  const int g = xcd_tile(blockIdx.x, gridDim.x, ntiles);
+  if (g < 0) return;
+  const int inst = tile_inst[g];
+  const StepArgs<T> &a = descs[inst];
+  any_stream<T, K, TX, S, NT>(a, g - inst_base[inst], a.dyn[0] + k, a.dyn[1] + k, L);
+}
+
+template <int TX, bool AXI, int S, bool NT>
+void launch_stream_acoustic(const void *args, int nblocks, size_t, hipStream_t s) {
+  hipLaunchKernelGGL((k_stream_acoustic<TX, AXI, S, NT>), dim3(nblocks), dim3(TX), 0, s,
+                     *static_cast<const StepArgs<T> *>(args));
+}
+
+template <int TX, bool AXI, int S, bool NT>
+void launch_stream_thermal(const void *args, int nblocks, size_t, hipStream_t s) {
+  hipLaunchKernelGGL((k_stream_thermal<TX, AXI, S, NT>), dim3(nblocks), dim3(TX), 0, s,
+                     *static_cast<const StepArgs<T> *>(args));
+}
+
+template <int TX, int S, bool NT>
+void launch_stream_batch(const BatchLaunch &b, int nblocks, size_t, hipStream_t s) {
+  hipLaunchKernelGGL((k_stream_batch<TX, S, NT>), dim3(nblocks), dim3(TX), 0, s,
+                     static_cast<const StepArgs<T> *>(b.descs), b.tile_inst, b.inst_base,
+                     b.ntiles, b.k);
+}
+
+// Fill entry [.][1] (the shape's fused stage count) and, for a one-stage shape, also [.][0]
+// (the one-stage remainder kernel, which fused shapes of the same width and cache policy share;
+// filled in by the kernel-set builder).
+void FDS_CAT(fill_stream_, FDS_SHAPE)(KernelSet &ks) {
+  constexpr int I = FDS_SHAPE;
+  static_assert(I < kNumStream, "shape index out of range");
+  constexpr int TX = kStreamShapes[I][0];
+  constexpr int S = kStreamShapes[I][2];
+  constexpr bool NT = kStreamShapes[I][3] != 0;
+  ks.stream_acoustic[I][0][1] = launch_stream_acoustic<TX, false, S, NT>;
+  ks.stream_acoustic[I][1][1] = launch_stream_acoustic<TX, true, S, NT>;
+  ks.stream_thermal[I][0][1] = launch_stream_thermal<TX, false, S, NT>;
+  ks.stream_thermal[I][1][1] = launch_stream_thermal<TX, true, S, NT>;
+  ks.stream_batch[I][1] = launch_stream_batch<TX, S, NT>;
+  if constexpr (S == 1) {
+    ks.stream_acoustic[I][0][0] = ks.stream_acoustic[I][0][1];
+    ks.stream_acoustic[I][1][0] = ks.stream_acoustic[I][1][1];
+    ks.stream_thermal[I][0][0] = ks.stream_thermal[I][0][1];
+    ks.stream_thermal[I][1][0] = ks.stream_thermal[I][1][1];
+    ks.stream_batch[I][0] = ks.stream_batch[I][1];
+  }
+}
+
+}  // namespace FDS_FORM_NS
+}  // namespace fds
