@@ -146,6 +146,8 @@ int fds_set_tile(fds_ctx *ctx, int tile_index);
 int fds_tile_count(void);
 int fds_tile_shape(int tile_index, int *tx, int *ty);
 int fds_tile_engine(int tile_index);
+/* Time steps per launch and cache policy (1: non-temporal state traffic) of a shape. */
+int fds_tile_traits(int tile_index, int *steps, int *nontemporal);
 /* Strip height (rows per workgroup) of a streaming shape; 0 restores the shape's own height.
  * Ignored by LDS-tile shapes. */
 int fds_set_strip_height(fds_ctx *ctx, int rows);

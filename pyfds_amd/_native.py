@@ -22,7 +22,7 @@ NUM_COEFF, NUM_SCALARS = 8, 8
 EXPORTS = ('fds_create', 'fds_destroy', 'fds_set_scalars', 'fds_set_coeff', 'fds_commit',
            'fds_set_state', 'fds_get_state', 'fds_set_boundaries', 'fds_set_outputs',
            'fds_run', 'fds_read_outputs', 'fds_set_tile', 'fds_tile_count', 'fds_tile_shape',
-           'fds_tile_engine', 'fds_set_strip_height', 'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_batch_create', 'fds_batch_run',
+           'fds_tile_engine', 'fds_tile_traits', 'fds_set_strip_height', 'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_batch_create', 'fds_batch_run',
            'fds_batch_last_run_ms', 'fds_batch_destroy', 'fds_set_learned_boundary',
            'fds_lbc_conv_fwd', 'fds_lbc_conv_bwd_data', 'fds_lbc_conv_bwd_weight',
            'fds_lbc_reduce_weight', 'fds_lbc_mse', 'fds_lbc_adam', 'fds_lbc_pack',
@@ -99,6 +99,7 @@ def load():
                                              ct.POINTER(_i64)]),
         'fds_tile_count': (ct.c_int, []),
         'fds_tile_engine': (ct.c_int, [ct.c_int]),
+        'fds_tile_traits': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         'fds_set_strip_height': (ct.c_int, [_p, ct.c_int]),
         'fds_tile_shape': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         'fds_last_error': (ct.c_char_p, []),
@@ -151,6 +152,19 @@ def tile_shapes():
         tx, ty = ct.c_int(), ct.c_int()
         check(lib.fds_tile_shape(i, ct.byref(tx), ct.byref(ty)))
         out.append((tx.value, ty.value))
+    return out
+
+
+def tile_table():
+    """Every shape of the step kernels: dicts with index, engine, tx, ty, steps (per launch)
+    and nontemporal."""
+    lib = load()
+    out = []
+    for i, ((tx, ty), eng) in enumerate(zip(tile_shapes(), tile_engines())):
+        st, nt = ct.c_int(), ct.c_int()
+        check(lib.fds_tile_traits(i, ct.byref(st), ct.byref(nt)))
+        out.append(dict(index=i, engine=eng, tx=tx, ty=ty, steps=st.value,
+                        nontemporal=bool(nt.value)))
     return out
 
 

@@ -73,7 +73,7 @@ def _restore(fields, snap):
         f.step = step
 
 
-# strip heights tried for the 256-wide streaming shape (other streaming shapes: their own)
+# strip heights also tried for the 16-row streaming shapes (the others: their own height)
 STRIP_HEIGHTS = (6, 12, 24)
 
 
@@ -112,7 +112,7 @@ def _tune(fields, steps, chunks, use_cache, batch):
             if lossy or one_row:
                 continue
             cand.append((i, 0))
-            if tx == 256 and ty == 16:
+            if ty == 16:
                 cand += [(i, h) for h in STRIP_HEIGHTS]
         else:
             cand.append((i, 0))

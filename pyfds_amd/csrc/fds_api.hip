@@ -413,8 +413,10 @@ fds::LaunchFn pick_launch(const fds_ctx *c, bool fused) {
   const int t = c->shape;
   if (c->streaming) {
     const int s = t - fds::kNumTiles, f = fused ? 1 : 0;
-    return c->family == kThermal ? c->ks->stream_thermal[s][c->axi ? 1 : 0][f]
-                                 : c->ks->stream_acoustic[s][c->axi ? 1 : 0][f];
+    // variant: axial-symmetric, isotropic (y coefficient slots alias the x ones), plain
+    const int v = c->axi ? 1 : (c->iso_b && c->iso_k) ? 2 : 0;
+    return c->family == kThermal ? c->ks->stream_thermal[s][v][f]
+                                 : c->ks->stream_acoustic[s][v][f];
   }
   switch (c->family) {
     case kAcoustic: return c->ks->acoustic[t][c->axi ? 1 : 0][c->lossy ? 1 : 0];
@@ -543,6 +545,15 @@ int fds_tile_engine(int i) {
   if (i < 0 || i >= fds::kNumTiles + fds::kNumStream)
     return fail(FDS_E_ARG, "tile index %d out of range", i);
   return i >= fds::kNumTiles ? 1 : 0;
+}
+
+int fds_tile_traits(int i, int *steps, int *nontemporal) {
+  if (i < 0 || i >= fds::kNumTiles + fds::kNumStream)
+    return fail(FDS_E_ARG, "tile index %d out of range", i);
+  const bool st = i >= fds::kNumTiles;
+  *steps = st ? fds::kStreamShapes[i - fds::kNumTiles][2] : 1;
+  *nontemporal = st ? fds::kStreamShapes[i - fds::kNumTiles][3] : 0;
+  return 0;
 }
 
 int fds_tile_shape(int i, int *tx, int *ty) {

@@ -11,19 +11,19 @@ constexpr int kNumTiles = 6;
 constexpr int kTileShapes[kNumTiles][2] = {
     {64, 16}, {64, 8}, {128, 8}, {256, 4}, {128, 4}, {256, 1}};
 constexpr int kDefaultTile2D = 1;       // default of the tile engine (lossy models)
-constexpr int kDefaultLossless2D = kNumTiles + 2;   // lossless 2D: streaming 256 x 16, 2 steps
+constexpr int kDefaultLossless2D = kNumTiles + 1;   // lossless 2D: streaming 256 x 16, 2 steps
 constexpr int kTileRow = 5;
 
 // Row-streaming engine (lossless 2D models): TX threads = columns, strips of TY rows, S time
-// steps fused per launch, NT = non-temporal state reads and writes.  Shape indices continue the
-// tile table: index kNumTiles + i is kStreamShapes[i] = {TX, TY, S, NT}.  Every fused shape has
-// a one-stage shape of the same width and NT (its remainder kernel).
-constexpr int kNumStream = 8;
-constexpr int kStreamShapes[kNumStream][4] = {{256, 16, 1, 0}, {128, 16, 1, 0}, {256, 16, 2, 0},
-                                              {128, 32, 2, 0}, {256, 32, 3, 0}, {256, 16, 1, 1},
-                                              {256, 16, 2, 1}, {256, 32, 3, 1}};
+// steps fused per launch, NT = non-temporal state reads and writes; TX = 64 runs one wave per
+// workgroup and exchanges neighbours by lane shuffles.  Shape indices continue the tile table:
+// index kNumTiles + i is kStreamShapes[i] = {TX, TY, S, NT}.
+constexpr int kNumStream = 10;
+constexpr int kStreamShapes[kNumStream][4] = {
+    {256, 16, 1, 0}, {256, 16, 2, 0}, {128, 32, 2, 0}, {256, 32, 3, 0}, {256, 16, 1, 1},
+    {256, 16, 2, 1}, {256, 32, 3, 1}, {64, 16, 1, 0},  {64, 16, 2, 0},  {64, 32, 3, 0}};
 // X-macro over the streaming shape indices (each shape is its own translation unit).
-#define FDS_STREAM_SHAPES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+#define FDS_STREAM_SHAPES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9)
 
 using LaunchFn = void (*)(const void *args, int nblocks, size_t lds_bytes, hipStream_t s);
 
@@ -44,8 +44,8 @@ struct KernelSet {
   LaunchFn thermal[kNumTiles][2];       // [tile][axi]
   LaunchFn nonlinear;
   BatchFn batch[kNumTiles];
-  LaunchFn stream_acoustic[kNumStream][2][2];   // [shape][axi][fused]
-  LaunchFn stream_thermal[kNumStream][2][2];
+  LaunchFn stream_acoustic[kNumStream][3][2];   // [shape][plain, axi, isotropic][fused]
+  LaunchFn stream_thermal[kNumStream][3][2];
   BatchFn stream_batch[kNumStream][2];          // [shape][fused]
 };
 

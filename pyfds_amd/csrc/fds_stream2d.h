@@ -229,12 +229,14 @@ struct AcVel {
 
 // S stages over one strip (S <= G + 1).  NT: state reads and writes bypass the caches'
 // temporal allocation (pays off when the working set is far larger than the caches); EV: the
-// tile has events; CHK: some reads fall outside the arrays.
-template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool EV, bool CHK>
+// tile has events; CHK: some reads fall outside the arrays; ISO: the y coefficients are the x
+// ones (isotropic material arrays), so they are neither loaded nor held twice.
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool EV, bool CHK, bool ISO>
 __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
                                StreamLds<T, TX, S> &L) {
   auto &sbp = L.bwd;    // Bx*p of the row, columns -1..TX
   auto &skv = L.fwd;    // forward x products of the row, columns 0..TX
+  constexpr bool WAVE = TX == 64;   // one wave: neighbours exchanged by lane shuffles
   const int H = a.strip_h, G = a.ghost;
   const int t = threadIdx.x;
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
@@ -255,6 +257,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     for (int b = 0; b < 2; ++b) {
+      if (WAVE) break;
       sbp[s][b][t + 1] = T(0);
       skv[s][b][t] = T(0);
       if (lh) {
@@ -277,6 +280,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
   T bpy[S][2] = {}, kvy[S][2] = {};
   T pn[S][2] = {}, cpn[S][2] = {};   // new p of the row above the stage's current row
   AcVel<T> vh[S][2] = {};            // new v of the stage's current row
+  T kvd[S][2] = {};                  // one wave: forward x difference of the row
 
   auto load_row = [&](int rho, AcRow<T> &d, AcHalo<T> &h) __attribute__((always_inline)) {
     const int rb = (y0 + rho) * nx + x0;
@@ -284,36 +288,32 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
     d.vx = ldn<NT, CHK>(vxin, rb, t, n);
     d.vy = ldn<NT, CHK>(vyin, rb, t, n);
     d.bx = ldr<CHK>(Bx, rb, t, n);
-    d.by = ldr<CHK>(By, rb, t, n);
+    d.by = ISO ? d.bx : ldr<CHK>(By, rb, t, n);
     d.kx = ldr<CHK>(Kx, rb, t, n);
-    d.ky = ldr<CHK>(Ky, rb, t, n);
+    d.ky = ISO ? d.kx : ldr<CHK>(Ky, rb, t, n);
     if (KAHAN) {
       d.cp = ldn<NT, CHK>(a.cin[0], rb, t, n);
       d.cvx = ldn<NT, CHK>(a.cin[1], rb, t, n);
       d.cvy = ldn<NT, CHK>(a.cin[2], rb, t, n);
     }
-    if (lh) {
-      h.p = ldr<CHK>(pin, rb, t - 1, n);
-      h.b = ldr<CHK>(Bx, rb, t - 1, n);
-      if (KAHAN) h.cp = ldr<CHK>(a.cin[0], rb, t - 1, n);
+    if (lh || rh) {   // one instruction for both sides when they share a wave
+      const int off = lh ? t - 1 : t + 1;
+      h.p = ldr<CHK>(pin, rb, off, n);
+      h.b = ldr<CHK>(Bx, rb, off, n);
+      if (KAHAN) h.cp = ldr<CHK>(a.cin[0], rb, off, n);
     }
     if (rh) {
-      h.p = ldr<CHK>(pin, rb, t + 1, n);
-      h.b = ldr<CHK>(Bx, rb, t + 1, n);
       h.vx = ldr<CHK>(vxin, rb, t + 1, n);
       h.k = ldr<CHK>(Kx, rb, t + 1, n);
-      if (KAHAN) {
-        h.cp = ldr<CHK>(a.cin[0], rb, t + 1, n);
-        h.cvx = ldr<CHK>(a.cin[1], rb, t + 1, n);
-      }
+      if (KAHAN) h.cvx = ldr<CHK>(a.cin[1], rb, t + 1, n);
     }
   };
   auto load_coef = [&](int rho, AcCoef<T> &d) __attribute__((always_inline)) {
     const int rb = (y0 + rho) * nx + x0;
     d.bx = ldr<CHK>(Bx, rb, t, n);
-    d.by = ldr<CHK>(By, rb, t, n);
+    d.by = ISO ? d.bx : ldr<CHK>(By, rb, t, n);
     d.kx = ldr<CHK>(Kx, rb, t, n);
-    d.ky = ldr<CHK>(Ky, rb, t, n);
+    d.ky = ISO ? d.kx : ldr<CHK>(Ky, rb, t, n);
   };
 
 #pragma unroll
@@ -343,7 +343,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       }
       bpx[s] = cbx * p;
       bpy[s][Q] = cby * p;
-      if (rho <= R.hi) {
+      if (!WAVE && rho <= R.hi) {
         sbp[s][Q][t + 1] = bpx[s];
         if (s == 0 && lh) sbp[s][Q][0] = h.b * h.p;
       }
@@ -352,7 +352,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
         cpp[Q] = cp;
       }
     });
-    __syncthreads();
+    if (!WAVE) __syncthreads();
     // ---- after the barrier, consumers first
     static_for<S, true>([&](auto Sc) __attribute__((always_inline)) {
       constexpr int s = decltype(Sc)::value;
@@ -371,7 +371,14 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       const AcHalo<T> &h = nh[W];
       T vxn = T(0), cvxn = T(0), vyn = T(0), cvyn = T(0);
       if (rho >= R.lo && rho <= R.hi) {
-        const T d = (-sbp[s][Q][t]) + bpx[s];
+        T left;
+        if constexpr (WAVE) {
+          left = __shfl_up(bpx[s], 1);
+          if (s == 0 && lh) left = h.b * h.p;
+        } else {
+          left = sbp[s][Q][t];
+        }
+        const T d = (-left) + bpx[s];
         vxn = ivx;
         cvxn = icvx;
         if (KAHAN) kahan_sub(vxn, cvxn, d); else vxn = vxn - d;
@@ -388,11 +395,21 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
           stream_record<T, KAHAN>(a.outp[1], cur[s].o[1], TX + 1, -G, 0, rho, t, row + s, vxn,
                                   cvxn);
         }
-        if (AXI) skv[s][Q][t] = (ckx / rad) * (vxn * rad);
-        else skv[s][Q][t] = ckx * vxn;
+        T kvx;
+        if (AXI) kvx = (ckx / rad) * (vxn * rad);
+        else kvx = ckx * vxn;
+        T kvh = T(0);
         if (s == 0 && rh) {
-          if (AXI) skv[s][Q][TX] = (h.k / radh) * (hvxn * radh);
-          else skv[s][Q][TX] = h.k * hvxn;
+          if (AXI) kvh = (h.k / radh) * (hvxn * radh);
+          else kvh = h.k * hvxn;
+        }
+        if constexpr (WAVE) {
+          T right = __shfl_down(kvx, 1);
+          if (s == 0 && rh) right = kvh;
+          kvd[s][Q] = (-kvx) + right;   // x difference of row rho, used by next iteration
+        } else {
+          skv[s][Q][t] = kvx;
+          if (s == 0 && rh) skv[s][Q][TX] = kvh;
         }
       }
       kvy[s][Q] = T(0);
@@ -412,7 +429,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       }
       T pnew = T(0), cpnew = T(0);
       if (rho > R.lo) {    // p of row rho-1 (its x products were published an iteration ago)
-        const T dxt = (-skv[s][Q ^ 1][t]) + skv[s][Q ^ 1][t + 1];
+        const T dxt = WAVE ? kvd[s][Q ^ 1] : (-skv[s][Q ^ 1][t]) + skv[s][Q ^ 1][t + 1];
         const T dyt = (-kvy[s][Q ^ 1]) + kvy[s][Q];
         const T d = dxt + dyt;
         pnew = pprev;
@@ -463,11 +480,12 @@ struct ThHalo {        // left thread: T, Q of column -1; right thread: T, Q, G 
   T t, ct, q, g;
 };
 
-template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool EV, bool CHK>
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool EV, bool CHK, bool ISO>
 __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
                               StreamLds<T, TX, S> &L) {
   auto &sqt = L.bwd;    // Qx*T of the row
   auto &sgq = L.fwd;    // forward x products of the row
+  constexpr bool WAVE = TX == 64;
   const int H = a.strip_h, G = a.ghost;
   const int t = threadIdx.x;
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
@@ -489,6 +507,7 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     for (int b = 0; b < 2; ++b) {
+      if (WAVE) break;
       sqt[s][b][t + 1] = T(0);
       sgq[s][b][t] = T(0);
       if (lh) {
@@ -510,33 +529,30 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
   ThCoef<T> cf[S][2] = {};
   T qty[S][2] = {}, gqy[S][2] = {};
   T tn[S][2] = {}, ctn[S][2] = {};
+  T gqd[S][2] = {};                   // one wave: forward x difference of the row
 
   auto load_row = [&](int rho, ThRow<T> &d, ThHalo<T> &h) __attribute__((always_inline)) {
     const int rb = (y0 + rho) * nx + x0;
     d.t = ldn<NT, CHK>(tin, rb, t, n);
     d.qx = ldr<CHK>(Qx, rb, t, n);
-    d.qy = ldr<CHK>(Qy, rb, t, n);
+    d.qy = ISO ? d.qx : ldr<CHK>(Qy, rb, t, n);
     d.gx = ldr<CHK>(Gx, rb, t, n);
-    d.gy = ldr<CHK>(Gy, rb, t, n);
+    d.gy = ISO ? d.gx : ldr<CHK>(Gy, rb, t, n);
     if (KAHAN) d.ct = ldn<NT, CHK>(a.cin[0], rb, t, n);
-    if (lh) {
-      h.t = ldr<CHK>(tin, rb, t - 1, n);
-      h.q = ldr<CHK>(Qx, rb, t - 1, n);
-      if (KAHAN) h.ct = ldr<CHK>(a.cin[0], rb, t - 1, n);
+    if (lh || rh) {   // one instruction for both sides when they share a wave
+      const int off = lh ? t - 1 : t + 1;
+      h.t = ldr<CHK>(tin, rb, off, n);
+      h.q = ldr<CHK>(Qx, rb, off, n);
+      if (KAHAN) h.ct = ldr<CHK>(a.cin[0], rb, off, n);
     }
-    if (rh) {
-      h.t = ldr<CHK>(tin, rb, t + 1, n);
-      h.q = ldr<CHK>(Qx, rb, t + 1, n);
-      h.g = ldr<CHK>(Gx, rb, t + 1, n);
-      if (KAHAN) h.ct = ldr<CHK>(a.cin[0], rb, t + 1, n);
-    }
+    if (rh) h.g = ldr<CHK>(Gx, rb, t + 1, n);
   };
   auto load_coef = [&](int rho, ThCoef<T> &d) __attribute__((always_inline)) {
     const int rb = (y0 + rho) * nx + x0;
     d.qx = ldr<CHK>(Qx, rb, t, n);
-    d.qy = ldr<CHK>(Qy, rb, t, n);
+    d.qy = ISO ? d.qx : ldr<CHK>(Qy, rb, t, n);
     d.gx = ldr<CHK>(Gx, rb, t, n);
-    d.gy = ldr<CHK>(Gy, rb, t, n);
+    d.gy = ISO ? d.gx : ldr<CHK>(Gy, rb, t, n);
   };
 
 #pragma unroll
@@ -565,7 +581,7 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
       }
       qtx[s] = cqx * tv;
       qty[s][Q] = cqy * tv;
-      if (rho <= R.hi) {
+      if (!WAVE && rho <= R.hi) {
         sqt[s][Q][t + 1] = qtx[s];
         if (s == 0 && lh) sqt[s][Q][0] = h.q * h.t;
       }
@@ -574,7 +590,7 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
         ctp[Q] = ct;
       }
     });
-    __syncthreads();
+    if (!WAVE) __syncthreads();
     static_for<S, true>([&](auto Sc) __attribute__((always_inline)) {
       constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
@@ -589,7 +605,14 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
       T zero = T(0), zeroc = T(0);
       T fx = T(0), fy = T(0);
       if (rho >= R.lo && rho <= R.hi) {
-        fx = -((-sqt[s][Q][t]) + qtx[s]);
+        T left;
+        if constexpr (WAVE) {
+          left = __shfl_up(qtx[s], 1);
+          if (s == 0 && lh) left = h.q * h.t;
+        } else {
+          left = sqt[s][Q][t];
+        }
+        fx = -((-left) + qtx[s]);
         T hfx = T(0);
         if (s == 0 && rh) hfx = -((-qtx[s]) + h.q * h.t);
         if constexpr (EV) {
@@ -598,11 +621,21 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
           stream_record<T, false>(a.outp[1], cur[s].o[1], TX + 1, -G, 0, rho, t, row + s, fx,
                                   zeroc);
         }
-        if (AXI) sgq[s][Q][t] = (cgx / rad) * (fx * rad);
-        else sgq[s][Q][t] = cgx * fx;
+        T gqx;
+        if (AXI) gqx = (cgx / rad) * (fx * rad);
+        else gqx = cgx * fx;
+        T gqh = T(0);
         if (s == 0 && rh) {
-          if (AXI) sgq[s][Q][TX] = (h.g / radh) * (hfx * radh);
-          else sgq[s][Q][TX] = h.g * hfx;
+          if (AXI) gqh = (h.g / radh) * (hfx * radh);
+          else gqh = h.g * hfx;
+        }
+        if constexpr (WAVE) {
+          T right = __shfl_down(gqx, 1);
+          if (s == 0 && rh) right = gqh;
+          gqd[s][Q] = (-gqx) + right;
+        } else {
+          sgq[s][Q][t] = gqx;
+          if (s == 0 && rh) sgq[s][Q][TX] = gqh;
         }
       }
       gqy[s][Q] = T(0);
@@ -619,7 +652,7 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
       }
       T tnew = T(0), ctnew = T(0);
       if (rho > R.lo) {
-        const T dxt = (-sgq[s][Q ^ 1][t]) + sgq[s][Q ^ 1][t + 1];
+        const T dxt = WAVE ? gqd[s][Q ^ 1] : (-sgq[s][Q ^ 1][t]) + sgq[s][Q ^ 1][t + 1];
         const T dyt = (-gqy[s][Q ^ 1]) + gqy[s][Q];
         const T d = dxt + dyt;
         tnew = tprev;
@@ -657,25 +690,28 @@ __device__ __forceinline__ int strip_path(const StepArgs<T> &a, int tile) {
   return __builtin_amdgcn_readfirstlane((tile_has_events(a, tile) ? 2 : 0) | (inside ? 1 : 0));
 }
 
-template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT>
-__device__ __forceinline__ void acoustic_stream(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
-                                StreamLds<T, TX, S> &L) {
+#define FDS_STRIP_CASES(FN, AXI, ISO, BASE)                                                    \
+  case BASE + 0: FN<T, KAHAN, TX, AXI, S, NT, false, true, ISO>(a, tile, step, row, L); break;  \
+  case BASE + 1: FN<T, KAHAN, TX, AXI, S, NT, false, false, ISO>(a, tile, step, row, L); break; \
+  case BASE + 2: FN<T, KAHAN, TX, AXI, S, NT, true, true, ISO>(a, tile, step, row, L); break;   \
+  case BASE + 3: FN<T, KAHAN, TX, AXI, S, NT, true, false, ISO>(a, tile, step, row, L); break;
+
+// ISO kernels serve fields whose y coefficient arrays are the x ones (the host checks).
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool ISO>
+__device__ __forceinline__ void acoustic_stream(const StepArgs<T> &a, int tile, int64_t step,
+                                                int64_t row, StreamLds<T, TX, S> &L) {
   switch (strip_path<T, TX>(a, tile)) {
-    case 0: acoustic_strip<T, KAHAN, TX, AXI, S, NT, false, true>(a, tile, step, row, L); break;
-    case 1: acoustic_strip<T, KAHAN, TX, AXI, S, NT, false, false>(a, tile, step, row, L); break;
-    case 2: acoustic_strip<T, KAHAN, TX, AXI, S, NT, true, true>(a, tile, step, row, L); break;
-    default: acoustic_strip<T, KAHAN, TX, AXI, S, NT, true, false>(a, tile, step, row, L); break;
+    FDS_STRIP_CASES(acoustic_strip, AXI, ISO, 0)
+    default: break;
   }
 }
 
-template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT>
-__device__ __forceinline__ void thermal_stream(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
-                               StreamLds<T, TX, S> &L) {
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool ISO>
+__device__ __forceinline__ void thermal_stream(const StepArgs<T> &a, int tile, int64_t step,
+                                               int64_t row, StreamLds<T, TX, S> &L) {
   switch (strip_path<T, TX>(a, tile)) {
-    case 0: thermal_strip<T, KAHAN, TX, AXI, S, NT, false, true>(a, tile, step, row, L); break;
-    case 1: thermal_strip<T, KAHAN, TX, AXI, S, NT, false, false>(a, tile, step, row, L); break;
-    case 2: thermal_strip<T, KAHAN, TX, AXI, S, NT, true, true>(a, tile, step, row, L); break;
-    default: thermal_strip<T, KAHAN, TX, AXI, S, NT, true, false>(a, tile, step, row, L); break;
+    FDS_STRIP_CASES(thermal_strip, AXI, ISO, 0)
+    default: break;
   }
 }
 
@@ -687,27 +723,14 @@ __device__ __forceinline__ void any_stream(const StepArgs<T> &a, int tile, int64
   const int f = a.flags;
   const int model = ((f & kFlagThermal) ? 2 : 0) | ((f & kFlagAxi) ? 1 : 0);
   switch (__builtin_amdgcn_readfirstlane(model * 4 + strip_path<T, TX>(a, tile))) {
-#define FDS_STREAM_CASE(M, V, FN, AXI, EV, CHK)                                                   \
-  case M * 4 + V: FN<T, KAHAN, TX, AXI, S, NT, EV, CHK>(a, tile, step, row, L); break;
-    FDS_STREAM_CASE(0, 0, acoustic_strip, false, false, true)
-    FDS_STREAM_CASE(0, 1, acoustic_strip, false, false, false)
-    FDS_STREAM_CASE(0, 2, acoustic_strip, false, true, true)
-    FDS_STREAM_CASE(0, 3, acoustic_strip, false, true, false)
-    FDS_STREAM_CASE(1, 0, acoustic_strip, true, false, true)
-    FDS_STREAM_CASE(1, 1, acoustic_strip, true, false, false)
-    FDS_STREAM_CASE(1, 2, acoustic_strip, true, true, true)
-    FDS_STREAM_CASE(1, 3, acoustic_strip, true, true, false)
-    FDS_STREAM_CASE(2, 0, thermal_strip, false, false, true)
-    FDS_STREAM_CASE(2, 1, thermal_strip, false, false, false)
-    FDS_STREAM_CASE(2, 2, thermal_strip, false, true, true)
-    FDS_STREAM_CASE(2, 3, thermal_strip, false, true, false)
-    FDS_STREAM_CASE(3, 0, thermal_strip, true, false, true)
-    FDS_STREAM_CASE(3, 1, thermal_strip, true, false, false)
-    FDS_STREAM_CASE(3, 2, thermal_strip, true, true, true)
-    FDS_STREAM_CASE(3, 3, thermal_strip, true, true, false)
-#undef FDS_STREAM_CASE
+    FDS_STRIP_CASES(acoustic_strip, false, false, 0)
+    FDS_STRIP_CASES(acoustic_strip, true, false, 4)
+    FDS_STRIP_CASES(thermal_strip, false, false, 8)
+    FDS_STRIP_CASES(thermal_strip, true, false, 12)
     default: break;
   }
 }
+
+#undef FDS_STRIP_CASES
 
 }  // namespace fds

@@ -31,20 +31,20 @@ namespace FDS_FORM_NS {
 using T = FDS_T;
 constexpr bool K = FDS_KAHAN;
 
-template <int TX, bool AXI, int S, bool NT>
+template <int TX, bool AXI, int S, bool NT, bool ISO>
 __global__ __launch_bounds__(TX) void k_stream_acoustic(StepArgs<T> a) {
   __shared__ StreamLds<T, TX, S> L;
   const int tile = xcd_tile(blockIdx.x, gridDim.x, a.ntiles);
   if (tile < 0) return;
-  acoustic_stream<T, K, TX, AXI, S, NT>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
+  acoustic_stream<T, K, TX, AXI, S, NT, ISO>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
 }
 
-template <int TX, bool AXI, int S, bool NT>
+template <int TX, bool AXI, int S, bool NT, bool ISO>
 __global__ __launch_bounds__(TX) void k_stream_thermal(StepArgs<T> a) {
   __shared__ StreamLds<T, TX, S> L;
   const int tile = xcd_tile(blockIdx.x, gridDim.x, a.ntiles);
   if (tile < 0) return;
-  thermal_stream<T, K, TX, AXI, S, NT>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
+  thermal_stream<T, K, TX, AXI, S, NT, ISO>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
 }
 
 template <int TX, int S, bool NT>
@@ -60,15 +60,15 @@ __global__ __launch_bounds__(TX) void k_stream_batch(const StepArgs<T> *__restri
   any_stream<T, K, TX, S, NT>(a, g - inst_base[inst], a.dyn[0] + k, a.dyn[1] + k, L);
 }
 
-template <int TX, bool AXI, int S, bool NT>
+template <int TX, bool AXI, int S, bool NT, bool ISO>
 void launch_stream_acoustic(const void *args, int nblocks, size_t, hipStream_t s) {
-  hipLaunchKernelGGL((k_stream_acoustic<TX, AXI, S, NT>), dim3(nblocks), dim3(TX), 0, s,
+  hipLaunchKernelGGL((k_stream_acoustic<TX, AXI, S, NT, ISO>), dim3(nblocks), dim3(TX), 0, s,
                      *static_cast<const StepArgs<T> *>(args));
 }
 
-template <int TX, bool AXI, int S, bool NT>
+template <int TX, bool AXI, int S, bool NT, bool ISO>
 void launch_stream_thermal(const void *args, int nblocks, size_t, hipStream_t s) {
-  hipLaunchKernelGGL((k_stream_thermal<TX, AXI, S, NT>), dim3(nblocks), dim3(TX), 0, s,
+  hipLaunchKernelGGL((k_stream_thermal<TX, AXI, S, NT, ISO>), dim3(nblocks), dim3(TX), 0, s,
                      *static_cast<const StepArgs<T> *>(args));
 }
 
@@ -79,27 +79,27 @@ void launch_stream_batch(const BatchLaunch &b, int nblocks, size_t, hipStream_t 
                      b.ntiles, b.k);
 }
 
-// Fill entry [.][1] (the shape's fused stage count) and, for a one-stage shape, also [.][0]
-// (the one-stage remainder kernel, which fused shapes of the same width and cache policy share;
-// filled in by the kernel-set builder).
+// Variants [v]: 0 plain, 1 axial-symmetric, 2 isotropic coefficients (y arrays = x arrays, not
+// axial-symmetric).  Fused [.][1]: the shape's stage count; [.][0]: one stage (remainder steps).
+template <int TX, int S, bool NT>
+void fill_variants(KernelSet &ks, int I, int f) {
+  ks.stream_acoustic[I][0][f] = launch_stream_acoustic<TX, false, S, NT, false>;
+  ks.stream_acoustic[I][1][f] = launch_stream_acoustic<TX, true, S, NT, false>;
+  ks.stream_acoustic[I][2][f] = launch_stream_acoustic<TX, false, S, NT, true>;
+  ks.stream_thermal[I][0][f] = launch_stream_thermal<TX, false, S, NT, false>;
+  ks.stream_thermal[I][1][f] = launch_stream_thermal<TX, true, S, NT, false>;
+  ks.stream_thermal[I][2][f] = launch_stream_thermal<TX, false, S, NT, true>;
+  ks.stream_batch[I][f] = launch_stream_batch<TX, S, NT>;
+}
+
 void FDS_CAT(fill_stream_, FDS_SHAPE)(KernelSet &ks) {
   constexpr int I = FDS_SHAPE;
   static_assert(I < kNumStream, "shape index out of range");
   constexpr int TX = kStreamShapes[I][0];
   constexpr int S = kStreamShapes[I][2];
   constexpr bool NT = kStreamShapes[I][3] != 0;
-  ks.stream_acoustic[I][0][1] = launch_stream_acoustic<TX, false, S, NT>;
-  ks.stream_acoustic[I][1][1] = launch_stream_acoustic<TX, true, S, NT>;
-  ks.stream_thermal[I][0][1] = launch_stream_thermal<TX, false, S, NT>;
-  ks.stream_thermal[I][1][1] = launch_stream_thermal<TX, true, S, NT>;
-  ks.stream_batch[I][1] = launch_stream_batch<TX, S, NT>;
-  if constexpr (S == 1) {
-    ks.stream_acoustic[I][0][0] = ks.stream_acoustic[I][0][1];
-    ks.stream_acoustic[I][1][0] = ks.stream_acoustic[I][1][1];
-    ks.stream_thermal[I][0][0] = ks.stream_thermal[I][0][1];
-    ks.stream_thermal[I][1][0] = ks.stream_thermal[I][1][1];
-    ks.stream_batch[I][0] = ks.stream_batch[I][1];
-  }
+  fill_variants<TX, S, NT>(ks, I, 1);
+  fill_variants<TX, 1, NT>(ks, I, 0);
 }
 
 }  // namespace FDS_FORM_NS

@@ -92,7 +92,13 @@ def test_fp64_tile_shapes_bit_identical(tile):
 
 
 # Row-streaming engine: shape indices after the LDS-tile table (lossless 2D models only; lossy
-# fields asked for a streaming shape run on the default tile shape).
+# fields asked for a streaming shape run on the default tile shape).  Slots index the streaming
+# table: 0 (256 x 16, one step per launch), 1 (256 x 16, two fused steps), 2 (128 x 32, two),
+# 3 (256 x 32, three), 4-6 the same with non-temporal state traffic, 7-9 one-wave 64-wide
+# shapes with one, two and three steps (tests/test_host_api.py pins these traits).
+NSTREAM = 10
+
+
 def _stream_indices():
     return [i for i, e in enumerate(pyfds_amd._native.tile_engines()) if e == 'stream']
 
@@ -101,7 +107,7 @@ LOSSLESS_2D = ['a2d_main', 'a2d_wrap', 'axi_lossless', 'bounds_semantics', 't2d_
                't2d_blocks', 'taxi']
 
 
-@pytest.mark.parametrize('slot', range(8))
+@pytest.mark.parametrize('slot', range(NSTREAM))
 @pytest.mark.parametrize('name', LOSSLESS_2D + ['a2d_lossy', 'axi_lossy'])
 def test_fp64_stream_shapes_match_reference_golden(name, slot):
     idx = _stream_indices()
@@ -124,7 +130,7 @@ def test_fp32_stream_engine_equals_tile_engine(name, form):
     scene = scenes.SCENES[name]()
     print('scene', name, form, flush=True)
     a = run_device(scene, form=form, tile=1)
-    for idx in [_stream_indices()[i] for i in (0, 4, 7)]:
+    for idx in [_stream_indices()[i] for i in (0, 3, 5, 8, 9)]:
         b = run_device(scene, form=form, tile=idx)
         for k in a:
             np.testing.assert_array_equal(a[k], b[k], err_msg='%s %s %d' % (name, k, idx))
@@ -137,8 +143,11 @@ def test_fp32_stream_engine_equals_tile_engine(name, form):
                                               ('Acoustic2D', 130, 101, 4),
                                               ('Acoustic2D', 700, 90, 5),
                                               ('Acoustic3DAxi', 260, 133, 6),
-                                              ('Acoustic2D', 509, 97, 7),
-                                              ('Acoustic3DAxi', 251, 64, 7)])
+                                              ('Acoustic2D', 509, 97, 6),
+                                              ('Acoustic3DAxi', 251, 64, 7),
+                                              ('Acoustic2D', 190, 77, 8),
+                                              ('Acoustic3DAxi', 333, 101, 9),
+                                              ('Acoustic2D', 61, 45, 9)])
 def test_fp64_stream_multi_strip_grids_match_oracle_bitwise(model, nx, ny, slot):
     """Partial column blocks at the right edge and partial strips at the bottom."""
     idx = _stream_indices()[slot]
@@ -150,7 +159,7 @@ def test_fp64_stream_multi_strip_grids_match_oracle_bitwise(model, nx, ny, slot)
         np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
 
 
-@pytest.mark.parametrize('slot', [None, 4, 7])
+@pytest.mark.parametrize('slot', [None, 1, 3, 8])
 @pytest.mark.parametrize('chunk', [0, 1, 7, 50])
 def test_graph_chunks_bit_identical(chunk, slot):
     """HIP-graph replayed step chunks (and plain launches) give identical results, also with
@@ -200,7 +209,8 @@ def test_fp64_multi_tile_grids_match_oracle_bitwise(model, mu, nx, ny):
         np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
 
 
-@pytest.mark.parametrize('tile', [None, 'stream0', 'stream3', 'stream4', 'stream7'])
+@pytest.mark.parametrize('tile', [None, 'stream0', 'stream2', 'stream3', 'stream6', 'stream8',
+                                  'stream9'])
 @pytest.mark.parametrize('model', ['Thermal2D', 'Thermal3DAxi'])
 def test_fp64_thermal_multi_tile_matches_oracle_bitwise(model, tile):
     nx, ny, dx = 190, 150, 1e-3

@@ -40,6 +40,15 @@ def test_tile_table():
     assert all(ty > 1 for _, ty in shapes[first:])
     lib = _native.load()
     assert lib.fds_tile_engine(len(shapes)) < 0 and lib.fds_tile_engine(-1) < 0
+    # the streaming table the GPU tests pick slots from: (tx, ty, steps per launch, nt)
+    stream = [(d['tx'], d['ty'], d['steps'], d['nontemporal'])
+              for d in _native.tile_table() if d['engine'] == 'stream']
+    assert stream == [(256, 16, 1, False), (256, 16, 2, False), (128, 32, 2, False),
+                      (256, 32, 3, False), (256, 16, 1, True), (256, 16, 2, True),
+                      (256, 32, 3, True), (64, 16, 1, False), (64, 16, 2, False),
+                      (64, 32, 3, False)]
+    assert all(d['steps'] == 1 and not d['nontemporal']
+               for d in _native.tile_table() if d['engine'] == 'tile')
 
 
 def test_simulation_without_gpu_fails_loudly():
