@@ -13,8 +13,11 @@ of pyfds_amd and timed through ``Field.simulate`` (device run + output download)
 Prints one JSON line.  ``value`` is the FP64 drop-in form (bit-identical to the reference's
 float64 arithmetic); the FP32 and FP32-Kahan forms are reported under ``forms`` with their
 relative L2 error against the FP64 result.  ``roofline`` uses the algorithmic bytes per cell
-update (FP64: p, vx, vy read + write and the two coefficients K, B = 8 words = 64 B) and the
-kernel time from HIP events recorded on the library's stream around the timed launches.
+update (FP64: p, vx, vy read + write and the two coefficients K, B = 8 words = 64 B) times the
+cell updates of one launch (a launch of the streaming engine advances 1-3 fused time steps), over
+the launch time from HIP events recorded on the library's stream around the timed launches.
+With fused steps the physical HBM traffic per launch (``traffic``, measured with rocprofv3) is
+below the algorithmic bytes, so ``frac`` can pass what one pass per step could reach.
 ``cpu_baseline`` times the numpy oracle (the plain method, single core) on a bounded
 number of steps of the same workload.
 
@@ -80,19 +83,42 @@ def build_field(ns, wl, form='fp64', t_samples=None):
     return fld
 
 
-def tuned(fld):
-    """Pick the shape / strip height / graph chunk by timing (state restored afterwards)."""
-    from pyfds_amd import autotune
-    tile, chunk, table = autotune.tune_field(fld, steps=40, use_cache=False)
+def tuned(fld, fixed=None):
+    """Pick the shape / strip height / graph chunk by timing (state restored afterwards), or
+    use ``fixed`` = (tile, strip)."""
+    from pyfds_amd import autotune, _native as nat
+    if fixed is not None:
+        fld.tile, fld.strip_height, fld.graph_chunk = fixed[0], fixed[1], 50
+        tile, chunk, table = fixed[0], 50, None
+    else:
+        tile, chunk, table = autotune.tune_field(fld, steps=40, use_cache=False)
+        table = {k: round(v, 2) for k, v in table.items()}
+    info = nat.tile_table()[tile]
     return dict(tile=tile, strip=getattr(fld, 'strip_height', 0) or 0, chunk=chunk,
-                table={k: round(v, 2) for k, v in table.items()})
+                shape='%s %dx%d, %d step(s) per launch%s' % (
+                    info['engine'], info['tx'], getattr(fld, 'strip_height', 0) or info['ty'],
+                    info['steps'], ', non-temporal' if info['nontemporal'] else ''),
+                steps_per_launch=info['steps'], table=table)
 
 
-def time_form(ns, wl, form, steps, warmup):
+# HBM bytes per launch of the step kernel measured with rocprofv3 (FETCH_SIZE scaled x2 and
+# WRITE_SIZE x1 after calibration against a plain copy kernel) for the configurations profiled
+# in profiles/: (form, tile index, strip height) -> (FETCH_SIZE KiB, WRITE_SIZE KiB).
+MEASURED_TRAFFIC_KIB = {
+    ('fp64', 14, 24): (109680.25, 99588.06),
+}
+
+
+def measured_traffic(form, tune):
+    kib = MEASURED_TRAFFIC_KIB.get((form, tune['tile'], tune['strip']))
+    return None if kib is None else int(round((2.0 * kib[0] + kib[1]) * 1024))
+
+
+def time_form(ns, wl, form, steps, warmup, fixed=None):
     import torch
     fld = build_field(ns, wl, form, t_samples=steps + warmup + 50)
     fld.assemble_matrices()
-    tune = tuned(fld)
+    tune = tuned(fld, fixed)
     if warmup:
         fld.simulate(warmup)
     runner = fld._runner()
@@ -350,6 +376,10 @@ def main():
     ap.add_argument('--batch-forms', default='fp32,fp32_kahan')
     ap.add_argument('--thermal-steps', type=int, default=2000)
     ap.add_argument('--thermal-forms', default='fp64,fp32_kahan')
+    ap.add_argument('--tile', type=int, default=None,
+                    help='Acoustic2D section: use this shape index instead of autotuning')
+    ap.add_argument('--strip', type=int, default=0,
+                    help='with --tile: strip height of a streaming shape (0: its own)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build has no multi-GPU path '
@@ -359,15 +389,21 @@ def main():
     import pyfds_amd as ns
     wl = workload(steps=args.steps + args.warmup)
     forms = [f for f in args.forms.split(',') if f]
-    res = {f: time_form(ns, wl, f, args.steps, args.warmup) for f in forms}
+    fixed = None if args.tile is None else (args.tile, args.strip)
+    res = {f: time_form(ns, wl, f, args.steps, args.warmup, fixed) for f in forms}
     main_form = 'fp64' if 'fp64' in res else forms[0]
     ref = res[main_form]
     out_forms = {}
     for f, r in res.items():
+        spl = r['tune']['steps_per_launch']
         entry = dict(value=round(r['gcells'], 3), ms_per_step=round(r['ms_per_step'], 4),
                      kernel_us=round(r['kernel_us'], 2),
+                     launch_us=round(r['kernel_us'] * spl, 2),
                      roofline_frac=round(r['achieved_gbs'] / HBM_PEAK_GBS, 4),
                      bytes_per_cell=BYTES_PER_CELL[f], autotune=r['tune'])
+        tr = measured_traffic(f, r['tune'])
+        if tr is not None:
+            entry['hbm_bytes_per_step'] = tr // spl
         if f != main_form and main_form == 'fp64':
             den = np.linalg.norm(ref['p'])
             entry['rel_l2_vs_fp64'] = float(np.linalg.norm(r['p'] - ref['p']) / den)
@@ -397,7 +433,14 @@ def main():
         'roofline': {'bound': 'hbm', 'achieved': round(ref['achieved_gbs'], 1),
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(ref['achieved_gbs'] / HBM_PEAK_GBS, 4),
-                     'traffic': None},
+                     'traffic': measured_traffic(main_form, ref['tune'])},
+        'roofline_detail': {
+            'kernel': ref['tune']['shape'],
+            'launch_us': round(ref['kernel_us'] * ref['tune']['steps_per_launch'], 2),
+            'algorithmic_bytes_per_launch': BYTES_PER_CELL[main_form] * NX * NY *
+            ref['tune']['steps_per_launch'],
+            'traffic': 'HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/), '
+                       'null unless this shape was profiled'},
         'cpu_baseline': cpu,
         'forms': out_forms,
     }

@@ -11,19 +11,20 @@ constexpr int kNumTiles = 6;
 constexpr int kTileShapes[kNumTiles][2] = {
     {64, 16}, {64, 8}, {128, 8}, {256, 4}, {128, 4}, {256, 1}};
 constexpr int kDefaultTile2D = 1;       // default of the tile engine (lossy models)
-constexpr int kDefaultLossless2D = kNumTiles + 1;   // lossless 2D: streaming 256 x 16, 2 steps
+constexpr int kDefaultLossless2D = kNumTiles + 8;   // lossless 2D: streaming 64 x 16, 2 steps
 constexpr int kTileRow = 5;
 
 // Row-streaming engine (lossless 2D models): TX threads = columns, strips of TY rows, S time
 // steps fused per launch, NT = non-temporal state reads and writes; TX = 64 runs one wave per
 // workgroup and exchanges neighbours by lane shuffles.  Shape indices continue the tile table:
 // index kNumTiles + i is kStreamShapes[i] = {TX, TY, S, NT}.
-constexpr int kNumStream = 10;
+constexpr int kNumStream = 12;
 constexpr int kStreamShapes[kNumStream][4] = {
     {256, 16, 1, 0}, {256, 16, 2, 0}, {128, 32, 2, 0}, {256, 32, 3, 0}, {256, 16, 1, 1},
-    {256, 16, 2, 1}, {256, 32, 3, 1}, {64, 16, 1, 0},  {64, 16, 2, 0},  {64, 32, 3, 0}};
+    {256, 16, 2, 1}, {256, 32, 3, 1}, {64, 16, 1, 0},  {64, 16, 2, 0},  {64, 32, 3, 0},
+    {64, 16, 1, 1},  {64, 16, 2, 1}};
 // X-macro over the streaming shape indices (each shape is its own translation unit).
-#define FDS_STREAM_SHAPES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9)
+#define FDS_STREAM_SHAPES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11)
 
 using LaunchFn = void (*)(const void *args, int nblocks, size_t lds_bytes, hipStream_t s);
 

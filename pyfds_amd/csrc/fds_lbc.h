@@ -22,7 +22,7 @@ constexpr int kLbcCin = 2 * 3 * kLbcBand;    // 48: {t, t-1} x {p, vx, vy} x ban
 constexpr int kLbcHidden = 32;
 constexpr int kLbcCout = 3 * kLbcDepth;      // 48: {p, vx, vy} x layer rows
 constexpr int kLbcTaps = 5;                  // conv kernel width along the boundary
-constexpr int kLbcSeg = 64;                  // layer positions per workgroup
+constexpr int kLbcSeg = 16;                  // layer positions per workgroup (4 x 66 groups at 1056)
 
 // Map (side, depth, along) to a flat index of the G x G grid.
 __host__ __device__ inline int lbc_cell(int side, int d, int a, int g) {

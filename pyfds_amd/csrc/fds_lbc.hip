@@ -265,8 +265,8 @@ __global__ __launch_bounds__(256) void k_band_rms(const float *__restrict__ fram
 // ---------------------------------------------------------------- fused inference, one step
 __global__ __launch_bounds__(256) void k_lbc_step(LbcStep A) {
   constexpr int R = kLbcTaps / 2;
-  constexpr int NX = kLbcSeg + 4 * R;   // 72 input positions
-  constexpr int NH = 80;                // hidden rows (5 MFMA row tiles, 68 used)
+  constexpr int NX = kLbcSeg + 4 * R;                       // input positions
+  constexpr int NH = (kLbcSeg + 2 * R + 15) / 16 * 16;      // hidden rows (whole MFMA tiles)
   constexpr int NW1 = kLbcHidden * kLbcTaps * kLbcCin;
   constexpr int NW2 = kLbcCout * kLbcTaps * kLbcHidden;
   __shared__ __attribute__((aligned(16))) uint16_t xin[NX * kLbcCin];

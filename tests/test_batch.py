@@ -70,7 +70,7 @@ def _stream_index(slot):
     return [i for i, e in enumerate(pyfds_amd._native.tile_engines()) if e == 'stream'][slot]
 
 
-@pytest.mark.parametrize('slot', [0, 2, 3, 6, 8, 9])
+@pytest.mark.parametrize('slot', [0, 2, 3, 6, 8, 9, 11])
 def test_fp64_stream_batch_matches_oracle(slot):
     """A batch of lossless members on the row-streaming engine."""
     members = []

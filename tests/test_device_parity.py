@@ -95,8 +95,9 @@ def test_fp64_tile_shapes_bit_identical(tile):
 # fields asked for a streaming shape run on the default tile shape).  Slots index the streaming
 # table: 0 (256 x 16, one step per launch), 1 (256 x 16, two fused steps), 2 (128 x 32, two),
 # 3 (256 x 32, three), 4-6 the same with non-temporal state traffic, 7-9 one-wave 64-wide
-# shapes with one, two and three steps (tests/test_host_api.py pins these traits).
-NSTREAM = 10
+# shapes with one, two and three steps, 10-11 64-wide non-temporal with one and two steps
+# (tests/test_host_api.py pins these traits).
+NSTREAM = 12
 
 
 def _stream_indices():
@@ -147,7 +148,9 @@ def test_fp32_stream_engine_equals_tile_engine(name, form):
                                               ('Acoustic3DAxi', 251, 64, 7),
                                               ('Acoustic2D', 190, 77, 8),
                                               ('Acoustic3DAxi', 333, 101, 9),
-                                              ('Acoustic2D', 61, 45, 9)])
+                                              ('Acoustic2D', 61, 45, 9),
+                                              ('Acoustic3DAxi', 140, 50, 10),
+                                              ('Acoustic2D', 200, 120, 11)])
 def test_fp64_stream_multi_strip_grids_match_oracle_bitwise(model, nx, ny, slot):
     """Partial column blocks at the right edge and partial strips at the bottom."""
     idx = _stream_indices()[slot]
@@ -210,7 +213,7 @@ def test_fp64_multi_tile_grids_match_oracle_bitwise(model, mu, nx, ny):
 
 
 @pytest.mark.parametrize('tile', [None, 'stream0', 'stream2', 'stream3', 'stream6', 'stream8',
-                                  'stream9'])
+                                  'stream9', 'stream11'])
 @pytest.mark.parametrize('model', ['Thermal2D', 'Thermal3DAxi'])
 def test_fp64_thermal_multi_tile_matches_oracle_bitwise(model, tile):
     nx, ny, dx = 190, 150, 1e-3

@@ -46,7 +46,7 @@ def test_tile_table():
     assert stream == [(256, 16, 1, False), (256, 16, 2, False), (128, 32, 2, False),
                       (256, 32, 3, False), (256, 16, 1, True), (256, 16, 2, True),
                       (256, 32, 3, True), (64, 16, 1, False), (64, 16, 2, False),
-                      (64, 32, 3, False)]
+                      (64, 32, 3, False), (64, 16, 1, True), (64, 16, 2, True)]
     assert all(d['steps'] == 1 and not d['nontemporal']
                for d in _native.tile_table() if d['engine'] == 'tile')
 
