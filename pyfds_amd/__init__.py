@@ -6,7 +6,10 @@ models, materials, regions, boundaries and outputs keep pyfds' names and signatu
 ``libpyfds_amd.so`` C-ABI (``include/pyfds_amd.h``).
 """
 from . import fields, regions  # noqa: F401
+from .acoustic_flow import *  # noqa: F401,F403
 from .acoustics import *  # noqa: F401,F403
+from .coupled_fields import *  # noqa: F401,F403
+from .coupling import *  # noqa: F401,F403
 from .nonlinear_acoustics import *  # noqa: F401,F403
 from .regions import *  # noqa: F401,F403
 from .thermal import *  # noqa: F401,F403

@@ -1,5 +1,5 @@
 """Write golden fixtures by running the reference pyfds on the scenes in
-tests/scenes.py.
+tests/scenes.py and tests/coupled_scenes.py.
 
 Run in the development container only (it reads the reference checkout):
 
@@ -40,6 +40,17 @@ def main(reference_root):
         res = scenes.collect(fld, scene)
         res['fingerprint'] = np.array(scenes.fingerprint(scene))
         path = os.path.join(HERE, name + '.npz')
+        np.savez_compressed(path, **res)
+        print('wrote', os.path.relpath(path), 'step', int(res['step']))
+
+    # coupled fields and flow (tests/coupled_scenes.py)
+    import coupled_scenes as cs
+    for name, make in cs.SCENES.items():
+        fld, plan, comps, params = make(pyfds)
+        cs.run(fld, plan)
+        res = cs.collect(fld, comps)
+        res['fingerprint'] = np.array(cs.fingerprint(params))
+        path = os.path.join(HERE, 'coupled_' + name + '.npz')
         np.savez_compressed(path, **res)
         print('wrote', os.path.relpath(path), 'step', int(res['step']))
 

@@ -1,0 +1,143 @@
+"""Coupled fields and acoustic flow (pyfds/coupling.py, coupled_fields.py, acoustic_flow.py).
+
+CPU: the reference's own coupling tests (test/test_coupling.py) restated on pyfds_amd, plus the
+flow profile rules.  GPU: coupled runs and the flow model against fixtures written by running
+the reference (tests/golden/coupled_*.npz), bit for bit in FP64.
+"""
+import os
+import warnings
+
+import numpy as np
+import pytest
+
+import pyfds_amd as fds
+import coupled_scenes as cs
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+
+
+def _thermal1d(**kw):
+    return fds.Thermal1D(t_delta=1, t_samples=1, x_delta=1, x_samples=3,
+                         material=fds.ThermalMaterial(1, 1, 1), **kw)
+
+
+# ------------------------------------------------------------------ CPU
+def test_synchronized_fields_exposes_components():
+    acs = fds.Acoustic1D(t_delta=1, t_samples=1, x_delta=1, x_samples=3,
+                         material=fds.AcousticMaterial(400, 1))
+    ths = _thermal1d()
+    with pytest.raises(RuntimeError):
+        fds.SynchronizedFields([acs, acs], [])
+    cpl = fds.SynchronizedFields([acs, ths], [])
+    for comp in (acs.velocity, acs.pressure, ths.temperature, ths.heat_flux):
+        assert comp in vars(cpl).values()
+    assert len(cpl.material_regions) == 2
+    cpl.assemble_matrices()
+    assert acs.matrices_assembled and ths.matrices_assembled and cpl.matrices_assembled
+    cpl.step = 4
+    assert acs.step == 4 and ths.step == 4 and cpl.num_points == 3
+
+
+def test_boundary_coupling_add_assign_accumulate():
+    comp1 = fds.fields.FieldComponent(num_points=12)
+    comp1.values = 2 * np.ones(12)
+    comp2 = fds.fields.FieldComponent(num_points=12)
+    comp2.values = np.ones(12)
+    fds.BoundaryCoupling(comp1, comp2, lambda x: x ** 2).apply(0)
+    assert np.allclose(comp2.values, 5)
+    # assign every 3rd step what accumulated since: steps 1, 2, 3 -> 3 * 2 at step 3, then 2
+    accu = fds.BoundaryCoupling(comp1, comp2, lambda x: x, False, True, 3)
+    for step in (1, 2, 3, 4):
+        accu.apply(step)
+    assert np.allclose(comp2.values, 6)
+    assert np.allclose(accu.accumulated_transfer, 2)
+
+
+def test_material_coupling_reassembles_with_stepping_and_threshold():
+    base = np.array([[-1, 1, 0], [0, -1, 1], [0, 0, -1]])
+    ths1 = _thermal1d()
+    ths1.assemble_matrices()
+    assert np.allclose(ths1.a_t_q.toarray(), base)
+    comp1 = fds.fields.FieldComponent(num_points=3)
+    comp1.values = np.ones(3)
+    c = fds.MaterialCoupling(comp1, ths1, 'density', lambda x: 1 + x ** 2, stepping=2)
+    assert np.allclose(ths1.material_vector('density'), 2)
+    assert np.allclose(ths1.material_vector('thermal_conductivity_x'), 1)
+    c.apply(0)
+    assert np.allclose(ths1.a_t_q.toarray(), base / 2)
+    comp1.values = 2 * np.ones(3)
+    c.apply(1)                       # not a coupling step
+    assert np.allclose(ths1.a_t_q.toarray(), base / 2)
+    c.apply(2)
+    assert np.allclose(ths1.a_t_q.toarray(), base / 5)
+
+    comp1.values = np.zeros(3)
+    ths2 = _thermal1d()
+    t = fds.MaterialCoupling(comp1, ths2, 'density', lambda x: 1 + x ** 2,
+                             rel_change_threshold=0.4)
+    t.apply(0)
+    assert np.allclose(ths2.a_t_q.toarray(), base)
+    comp1.values = np.ones(3)
+    t.apply(1)                       # change 0.5 > 0.4
+    assert np.allclose(ths2.a_t_q.toarray(), base / 2)
+    comp1.values = 1.5 * np.ones(3)
+    t.apply(2)                       # change (3.25 - 2) / 3.25 = 0.38 < 0.4
+    assert np.allclose(ths2.a_t_q.toarray(), base / 2)
+
+
+def test_exponential_and_power_law_factors():
+    src = fds.fields.FieldComponent(num_points=4)
+    src.values = np.array([0.0, 1.0, -2.0, 3.0])
+    e = fds.MaterialCouplingExponential(src, _thermal1d(), 'density', a=0.25, b=0.5)
+    assert np.allclose(e.transfer_function(src.values), 0.25 + 0.75 * np.exp(0.5 * src.values))
+    assert e.transfer_function(np.zeros(1))[0] == 1.0          # p for q = 0
+    p = fds.MaterialCouplingPowerLaw(src, _thermal1d(), 'density', power=3, factor=0.1)
+    assert np.allclose(p.transfer_function(src.values), 1 + 0.1 * src.values ** 3)
+
+
+def test_flow_profile_rules():
+    kw = dict(x_samples=8, x_delta=1e-3, y_samples=4, y_delta=1e-3, t_samples=4,
+              t_delta=1e-6, material=fds.AcousticMaterial(1500, 1000))
+    with pytest.raises(ValueError):
+        fds.AcousticFlow2D([1.0, 2.0], **kw)
+    f = fds.AcousticFlow2D(100.0, **kw)
+    assert np.array_equal(f.flow, np.full(4, 100.0))
+    assert np.array_equal(f.flow_t_deltas, np.full(4, 10))       # 1e-3 / 100 / 1e-6
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter('always')
+        fds.AcousticFlow2D([100.0, 200.0, 1000.0, 2000.0], **kw)  # deltas 10, 5, 1, 0
+    assert any('Flow velocity may be to high' in str(x.message) for x in w)
+
+
+def test_apply_flow_shifts_rows_on_the_host():
+    nx, ny = 6, 3
+    f = fds.AcousticFlow2D([250.0, 500.0, 1000.0], x_samples=nx, x_delta=1e-3, y_samples=ny,
+                           y_delta=1e-3, t_samples=4, t_delta=1e-6,
+                           material=fds.AcousticMaterial(1500, 1000))
+    assert list(f.flow_t_deltas) == [4, 2, 1]
+    vals = np.arange(nx * ny, dtype=float) + 1
+    for c in (f.pressure, f.velocity_x, f.velocity_y):
+        c.values = vals.copy()
+    f.step = 2                       # rows with period 2 and 1 shift, row 0 (period 4) not
+    f.apply_flow()
+    want = vals.reshape(ny, nx).copy()
+    for r in (1, 2):
+        want[r, 1:] = vals.reshape(ny, nx)[r, :-1]
+        want[r, 0] = 0
+    for c in (f.pressure, f.velocity_x, f.velocity_y):
+        assert np.array_equal(c.values, want.ravel())
+
+
+# ------------------------------------------------------------------ GPU
+@pytest.mark.gpu
+@pytest.mark.parametrize('name', sorted(cs.SCENES))
+def test_coupled_runs_match_reference_golden(name):
+    gold = np.load(os.path.join(GOLDEN, 'coupled_' + name + '.npz'))
+    fld, plan, comps, params = cs.SCENES[name](fds)
+    assert str(gold['fingerprint']) == cs.fingerprint(params), 'stale fixture'
+    print('scene', name, flush=True)
+    res = cs.collect(cs.run(fld, plan), comps)
+    keys = [k for k in gold.files if k != 'fingerprint']
+    assert sorted(keys) == sorted(res)
+    for k in keys:
+        np.testing.assert_array_equal(res[k], gold[k], err_msg='%s %s' % (name, k))
