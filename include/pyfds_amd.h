@@ -164,6 +164,12 @@ int fds_batch_run(fds_batch *b, int64_t start_step, int64_t nsteps, int chunk);
 int fds_batch_last_run_ms(fds_batch *b, double *ms);
 int fds_batch_destroy(fds_batch *b);
 
+/* Flow along x (AcousticFlow2D, pyfds/acoustic_flow.py:49-57 apply_flow): after every step n,
+ * each row r with period[r] == 0 or n % |period[r]| == 0 moves one cell to +x (a zero enters at
+ * the row start), in p, vx, vy (and their compensation).  period = NULL clears it.  The
+ * reference applies the same shift on the host after each sim_step. */
+int fds_set_flow(fds_ctx *ctx, const int64_t *period, int64_t nrows);
+
 /* Learned absorbing boundary (no reference counterpart; pyfds has only the plain method).
  * After every step of an FP32 Acoustic2D context on a square G x G grid, a BF16-MFMA network
  * predicts the outer `layer` cells (16) of p, vx, vy on all four sides from the 8 adjacent

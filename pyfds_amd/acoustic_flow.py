@@ -16,7 +16,8 @@ logger = lo.getLogger('pyfds')
 
 class AcousticFlow2D(acs.Acoustic2D):
     """Acoustic2D whose rows are shifted one cell in +x every ``flow_t_deltas[row]`` steps
-    (the reference's approximation of a flow along x)."""
+    (the reference's approximation of a flow along x).  ``simulate`` and ``sim_step`` shift on
+    the device after every step; ``apply_flow`` is the same shift on the host values."""
 
     def __init__(self, flow, *args, **kwargs):
         super().__init__(*args, **kwargs)
@@ -33,9 +34,15 @@ class AcousticFlow2D(acs.Acoustic2D):
             wn.warn('Flow velocity may be to high. Consider reducing t_delta.', stacklevel=2)
             logger.warning('Flow velocity may be to high. Consider reducing t_delta.')
 
-    def sim_step(self):
-        super().sim_step()
-        self.apply_flow()
+    def _device_sim_step(self):
+        """One step followed by the flow shift, both on the device (the runner hands the
+        shift periods to the library, which shifts after every step it runs)."""
+        acs.Acoustic2D._device_sim_step(self)
+
+    sim_step = _device_sim_step
+
+    def _device_flow_periods(self):
+        return self.flow_t_deltas
 
     def apply_flow(self):
         """Shift every row whose period divides the current step one cell to +x, entering a

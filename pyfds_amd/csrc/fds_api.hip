@@ -134,7 +134,8 @@ struct fds_ctx {
   int tile = -1;          // requested shape index (-1: default)
   int shape = 0;          // shape index in use (tile table, then the streaming table)
   bool streaming = false; // row-streaming engine in use
-  int strip_rows = 0;     // streaming strip height override (0: the shape's own)
+  int strip_rows = 0;     // streaming strip height override (0: from the shape and the grid)
+  int num_cu = 256;       // compute units of the device
   int fuse = 1;           // time steps per launch of the streaming shape
   int ghost = 0;          // ghost columns / rows per side of a streaming tile (fuse - 1)
   int sx = 0;             // x stride of the tiles (tx - 2*ghost)
@@ -150,6 +151,9 @@ struct fds_ctx {
   std::vector<hipGraph_t> graph_objs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;   // bracket the launches of the last run
   double last_run_ms = 0.0;
+  // flow along x (AcousticFlow2D): per-row shift period, applied after every step
+  DevBuf flow_period;
+  bool flow = false;
   // learned absorbing boundary (caller-owned bf16 weights), applied after every step
   const uint16_t *lbc_w1 = nullptr, *lbc_w2 = nullptr;
   int lbc_layer = 0;
@@ -302,9 +306,20 @@ int choose_tiles(fds_ctx *c) {
     c->streaming = t >= fds::kNumTiles;
     shape_of(t, &c->tx, &c->ty);
     if (c->streaming) {
-      if (c->strip_rows > 0) c->ty = c->strip_rows;
       c->fuse = fds::kStreamShapes[t - fds::kNumTiles][2];
       c->ghost = c->fuse - 1;
+      if (c->strip_rows > 0) {
+        c->ty = c->strip_rows;
+      } else {
+        // a strip is walked row by row by its waves: small grids get shorter strips so that
+        // there are enough waves to cover the memory latency (~4 per SIMD)
+        const int64_t sx = c->tx - 2 * c->ghost;
+        const int64_t waves = ((c->nx + sx - 1) / sx) * (c->tx / 64);
+        const int64_t want = 4 * 4 * (int64_t)c->num_cu;
+        const int64_t h = (c->ny * waves + want - 1) / want;
+        const int hmin = 2 * c->fuse + 2;
+        if (h < c->ty) c->ty = (int)std::max<int64_t>(hmin, h);
+      }
     }
   }
   c->sx = c->tx - 2 * c->ghost;
@@ -425,6 +440,35 @@ fds::LaunchFn pick_launch(const fds_ctx *c, bool fused) {
   }
 }
 
+// Flow along x (pyfds/acoustic_flow.py:49-57): after the step at `step`, every row r whose
+// period divides the step (period 0 always; numpy's int % 0 is 0) moves one cell to +x and
+// takes a zero at its start, in every state array (and compensation array) of the step's
+// output.  One workgroup per row shifts it in place, chunk by chunk from the right end, so
+// each chunk reads its left neighbour before that neighbour is overwritten.
+template <typename T>
+__global__ __launch_bounds__(256) void k_flow_shift(T *a0, T *a1, T *a2, T *c0, T *c1, T *c2,
+                                                    int nx, const int64_t *__restrict__ period,
+                                                    const int64_t *__restrict__ dyn, int k) {
+  const int r = blockIdx.x;
+  const int64_t step = dyn[0] + k;
+  const int64_t p = period[r];
+  const int64_t ap = p < 0 ? -p : p;
+  if (!(ap == 0 || step % ap == 0)) return;
+  T *rows[6] = {a0, a1, a2, c0, c1, c2};
+  for (int q = 0; q < 6; ++q) {
+    T *row = rows[q];
+    if (!row) continue;
+    row += (int64_t)r * nx;
+    for (int s = ((nx - 1) / 256) * 256; s >= 0; s -= 256) {
+      const int i = s + (int)threadIdx.x;
+      const T v = (i >= 1 && i < nx) ? row[i - 1] : T(0);
+      __syncthreads();
+      if (i < nx) row[i] = v;
+      __syncthreads();
+    }
+  }
+}
+
 __global__ void k_advance(int64_t *dyn, int64_t by) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     dyn[0] += by;
@@ -454,7 +498,9 @@ void launch_lbc<float>(fds_ctx *c, const fds::StepArgs<float> &a) {
 
 // Time steps per launch: the streaming shapes' fused stage count (one step per launch when a
 // per-step learned boundary follows the step kernel).
-int step_group(const fds_ctx *c) { return (c->streaming && !c->lbc_w1) ? c->fuse : 1; }
+int step_group(const fds_ctx *c) {
+  return (c->streaming && !c->lbc_w1 && !c->flow) ? c->fuse : 1;
+}
 
 // Launch one step (or, fused, step_group(c) steps) starting at step k of the launch sequence:
 // reads state buffer `parity`, writes parity^1.
@@ -467,6 +513,12 @@ int launch_step(fds_ctx *c, int parity, int k, bool store, bool fused) {
   const int nb = fds::xcd_grid(c->ntiles);
   pick_launch(c, fused)(&a, nb, lds_bytes(c), c->stream);
   HIP_TRY(hipGetLastError());
+  if (c->flow) {
+    hipLaunchKernelGGL(k_flow_shift<T>, dim3((unsigned)c->ny), dim3(256), 0, c->stream,
+                       a.out[0], a.out[1], a.out[2], a.cout[0], a.cout[1], a.cout[2],
+                       (int)c->nx, c->flow_period.as<const int64_t>(), a.dyn, k);
+    HIP_TRY(hipGetLastError());
+  }
   if (c->lbc_w1) {
     launch_lbc<T>(c, a);
     HIP_TRY(hipGetLastError());
@@ -612,6 +664,10 @@ int fds_create(fds_ctx **out, int model, int form, int64_t nx, int64_t ny, int d
     delete c;
     return fail(FDS_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
   }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+      cus > 0)
+    c->num_cu = cus;
   // components: all three exist for every model (1D linear models carry an inert third one)
   for (int k = 0; k < 3; ++k) {
     Comp &cp = c->comp[k];
@@ -906,6 +962,25 @@ int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_run_ms = ms;
+  return 0;
+}
+
+int fds_set_flow(fds_ctx *c, const int64_t *period, int64_t nrows) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!period || nrows == 0) {
+    c->flow = false;
+    touch(c);
+    return 0;
+  }
+  if (c->family != kAcoustic || c->one_d)
+    return fail(FDS_E_ARG, "flow is implemented for the 2D acoustic models");
+  if (nrows != c->ny) return fail(FDS_E_ARG, "flow needs %lld row periods, got %lld",
+                                  (long long)c->ny, (long long)nrows);
+  std::vector<int64_t> p(period, period + nrows);
+  HIP_TRY(upload(c->flow_period, p, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->flow = true;
+  touch(c);
   return 0;
 }
 

@@ -279,26 +279,57 @@ __global__ __launch_bounds__(256) void k_lbc_step(LbcStep A) {
   const int s = blockIdx.y, a0 = blockIdx.x * kLbcSeg;
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, m = l & 15, g = l >> 4;
 
-  // this side's weights into LDS first: one round trip instead of one per MFMA
+  // this side's weights and the band inputs into LDS: every global load is issued before the
+  // first LDS store, so the workgroup waits for one memory round trip, not one per element
+  constexpr int N1 = NW1 / 4, N2 = NW2 / 4, NI = NX * kLbcCin;
+  constexpr int I1 = (N1 + 255) / 256, I2 = (N2 + 255) / 256, II = (NI + 255) / 256;
+  s4 r1[I1], r2[I2];
+  float rx[II];
   {
     const s4 *g1 = reinterpret_cast<const s4 *>(st.w1 + (size_t)s * NW1);
     const s4 *g2 = reinterpret_cast<const s4 *>(st.w2 + (size_t)s * NW2);
+#pragma unroll
+    for (int i = 0; i < I1; ++i) {
+      const int e = tid + 256 * i;
+      if (e < N1) r1[i] = g1[e];
+    }
+#pragma unroll
+    for (int i = 0; i < I2; ++i) {
+      const int e = tid + 256 * i;
+      if (e < N2) r2[i] = g2[e];
+    }
+#pragma unroll
+    for (int i = 0; i < II; ++i) {
+      const int e = tid + 256 * i;
+      float v = 0.f;
+      if (e < NI) {
+        const int row = e / kLbcCin, ci = e % kLbcCin;
+        const int a = a0 - 2 * R + row;
+        if (a >= L && a < G - L) {
+          const int h = ci / 24, c = (ci % 24) / kLbcBand, j = ci % kLbcBand;
+          const int cell = lbc_cell(s, L + j, a, G);
+          v = (h == 0 ? st.cur[c][cell] : st.prev[c][cell]) * A.cs[c];
+        }
+      }
+      rx[i] = v;
+    }
     s4 *l1 = reinterpret_cast<s4 *>(sw1);
     s4 *l2 = reinterpret_cast<s4 *>(sw2);
-    for (int e = tid; e < NW1 / 4; e += 256) l1[e] = g1[e];
-    for (int e = tid; e < NW2 / 4; e += 256) l2[e] = g2[e];
-  }
-
-  for (int e = tid; e < NX * kLbcCin; e += 256) {
-    const int row = e / kLbcCin, ci = e % kLbcCin;
-    const int a = a0 - 2 * R + row;
-    float v = 0.f;
-    if (a >= L && a < G - L) {
-      const int h = ci / 24, c = (ci % 24) / kLbcBand, j = ci % kLbcBand;
-      const int cell = lbc_cell(s, L + j, a, G);
-      v = (h == 0 ? st.cur[c][cell] : st.prev[c][cell]) * A.cs[c];
+#pragma unroll
+    for (int i = 0; i < I1; ++i) {
+      const int e = tid + 256 * i;
+      if (e < N1) l1[e] = r1[i];
     }
-    xin[e] = f2bf(v);
+#pragma unroll
+    for (int i = 0; i < I2; ++i) {
+      const int e = tid + 256 * i;
+      if (e < N2) l2[e] = r2[i];
+    }
+#pragma unroll
+    for (int i = 0; i < II; ++i) {
+      const int e = tid + 256 * i;
+      if (e < NI) xin[e] = f2bf(rx[i]);
+    }
   }
   __syncthreads();
 

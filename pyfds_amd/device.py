@@ -129,6 +129,7 @@ class DeviceRunner:
         self.out_cells = [None, None, None]
         self.out_layout = [[], [], []]
         self.owners = [None, None, None]
+        self.flow = None
         self.chunk = getattr(field, 'graph_chunk', 50)
         self.tile = getattr(field, 'tile', None)
         self.strip_height = getattr(field, 'strip_height', None)
@@ -184,6 +185,12 @@ class DeviceRunner:
             nat.check(lib.fds_set_scalars(ctx, nat.dptr(s), s.size))
             nat.check(lib.fds_commit(ctx))
             self.coef_version = field._coef_version
+        periods = getattr(field, '_device_flow_periods', None)
+        if periods is not None:
+            p = nat.i64(periods())
+            if self.flow is None or not np.array_equal(p, self.flow):
+                nat.check(lib.fds_set_flow(ctx, nat.lptr(p), p.size))
+                self.flow = p.copy()
         comps = self._components(field)
         for k, comp in enumerate(comps):
             if comp is None:

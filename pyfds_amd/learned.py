@@ -30,6 +30,7 @@ import time
 import numpy as np
 
 from . import _native as nat
+from . import autotune
 from .acoustics import Acoustic2D, AcousticMaterial
 from . import regions
 
@@ -327,14 +328,16 @@ class LearnedBoundary:
         return res
 
     def time_methods(self, scene, steps):
-        """Wall-clock of the plain enlarged run and of the learned-boundary run (setup
-        excluded, both FP32 with HIP-graph chunks)."""
+        """Wall-clock of the plain enlarged run and of the learned-boundary run (setup and
+        shape tuning excluded, both FP32 with HIP-graph chunks)."""
         torch = self.torch
         out = {}
         for key, make in (('enlarged', lambda: self.field(scene, self.big, self.off, steps)),
                           ('learned', lambda: self.attach(self.field(scene, self.G, 0, steps)))):
             f = make()
             f.assemble_matrices()
+            # each method on its own best kernel shape (tuning restores the field's state)
+            autotune.tune_field(f, steps=20, use_cache=False)
             f.simulate(min(20, steps))           # warm-up (graph capture, first launches)
             torch.cuda.synchronize()
             t0 = time.perf_counter()

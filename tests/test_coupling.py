@@ -141,3 +141,28 @@ def test_coupled_runs_match_reference_golden(name):
     assert sorted(keys) == sorted(res)
     for k in keys:
         np.testing.assert_array_equal(res[k], gold[k], err_msg='%s %s' % (name, k))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('chunk', [0, 1, 7])
+def test_flow_shift_on_device_for_any_graph_chunk(chunk):
+    """The device shift runs inside replayed graph chunks and after plain launches alike."""
+    gold = np.load(os.path.join(GOLDEN, 'coupled_flow2d.npz'))
+    fld, plan, comps, params = cs.SCENES['flow2d'](fds)
+    fld.graph_chunk = chunk
+    res = cs.collect(cs.run(fld, plan), comps)
+    for k in res:
+        np.testing.assert_array_equal(res[k], gold[k], err_msg='%d %s' % (chunk, k))
+
+
+@pytest.mark.gpu
+def test_flow_sim_step_equals_simulate():
+    gold = np.load(os.path.join(GOLDEN, 'coupled_flow2d.npz'))
+    fld, plan, comps, params = cs.SCENES['flow2d'](fds)
+    fld.assemble_matrices()
+    for _ in range(sum(n for _, n in plan)):
+        fld.sim_step()
+        fld.step += 1
+    res = cs.collect(fld, comps)
+    for k in res:
+        np.testing.assert_array_equal(res[k], gold[k], err_msg=k)
