@@ -172,16 +172,19 @@ def batched_fields(ns, form, n_inst=256, seed=2, steps=1000):
     return fields
 
 
-def time_batched(ns, form, steps, warmup, chunk=50, tune=True):
+def time_batched(ns, form, steps, warmup, chunk=50, fixed=None):
     import torch
     from pyfds_amd.batch import BatchRunner
     from pyfds_amd import autotune
     fields = batched_fields(ns, form, steps=steps + warmup + 64)
     table = None
-    tile = None
-    if tune:
+    if fixed is None:
         tile, chunk, table = autotune.tune_batch(fields, steps=10, chunks=(chunk,),
                                                  use_cache=False)
+    else:
+        tile = fixed[0]
+        for f in fields:
+            f.strip_height = fixed[1]
     batch = BatchRunner(fields, tile=tile, chunk=chunk)
     if warmup:
         batch.simulate(warmup)
@@ -220,11 +223,11 @@ def thermal_field(ns, form, steps, nx=4096, seed=3):
     return fld
 
 
-def time_thermal(ns, form, steps, warmup):
+def time_thermal(ns, form, steps, warmup, fixed=None):
     import torch
     fld = thermal_field(ns, form, steps + warmup + 50)
     fld.assemble_matrices()
-    tune = tuned(fld)
+    tune = tuned(fld, fixed)
     if warmup:
         fld.simulate(warmup)
     runner = fld._runner()
@@ -380,6 +383,10 @@ def main():
                     help='Acoustic2D section: use this shape index instead of autotuning')
     ap.add_argument('--strip', type=int, default=0,
                     help='with --tile: strip height of a streaming shape (0: its own)')
+    ap.add_argument('--thermal-tile', type=int, default=None)
+    ap.add_argument('--thermal-strip', type=int, default=0)
+    ap.add_argument('--batch-tile', type=int, default=None)
+    ap.add_argument('--batch-strip', type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build has no multi-GPU path '
@@ -448,7 +455,9 @@ def main():
     if 'batched' in sections:
         bat = {}
         for f in [x for x in args.batch_forms.split(',') if x]:
-            r, _ = time_batched(ns, f, args.batch_steps, 10)
+            r, _ = time_batched(ns, f, args.batch_steps, 10,
+                                fixed=None if args.batch_tile is None
+                                else (args.batch_tile, args.batch_strip))
             bat[f] = dict(value=round(r['gcells'], 3), ms_per_step=round(r['ms_per_step'], 4),
                           kernel_us=round(r['kernel_us'], 2),
                           achieved_gbs=round(r['achieved_gbs'], 1),
@@ -467,7 +476,9 @@ def main():
         th = {}
         tvals = {}
         for f in [x for x in args.thermal_forms.split(',') if x]:
-            r = time_thermal(ns, f, args.thermal_steps, 10)
+            r = time_thermal(ns, f, args.thermal_steps, 10,
+                             None if args.thermal_tile is None
+                             else (args.thermal_tile, args.thermal_strip))
             tvals[f] = r['t']
             th[f] = dict(value=round(r['gcells'], 3), ms_per_step=round(r['ms_per_step'], 4),
                          kernel_us=round(r['kernel_us'], 2),
