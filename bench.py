@@ -179,7 +179,7 @@ def time_batched(ns, form, steps, warmup, chunk=50, fixed=None):
     fields = batched_fields(ns, form, steps=steps + warmup + 64)
     table = None
     if fixed is None:
-        tile, chunk, table = autotune.tune_batch(fields, steps=10, chunks=(chunk,),
+        tile, chunk, table = autotune.tune_batch(fields, steps=20, chunks=(chunk,),
                                                  use_cache=False)
     else:
         tile = fixed[0]
