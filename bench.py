@@ -31,6 +31,11 @@ Further sections (reported under ``batched`` and ``thermal2d``):
   61 440 parameters) trained on plain 3072x3072 FP32 runs with random Ricker sources and
   inclusions (seeds 100..), judged on a held-out scene against the plain 3072^2 run:
   reflected energy, the same small grid without the net (ablation), wall-clock of both methods.
+* acoustic1d: Acoustic1D, 2000 cells, water, dx 1e-4, dt at CFL 0.5, Gaussian pulse, rigid
+  ends (velocity Dirichlet 0 at both ends), 3 recorded pressure points, 5000 steps; FP64 on
+  the resident 1D engine (one workgroup, every cells-per-thread variant that fits) and on the
+  LDS-tile engine (one launch per step in HIP-graph chunks of 50).  Latency-bound: reported
+  as time per step.
 * thermal2d: Thermal2D 4096x4096, k in {0.6, 50} W/mK in seeded 256-cell blocks (seed 3),
   (16x16), rho c_p identical, Fourier number 0.2 on k_max, Gaussian hot spot (sigma 64 cells); FP64
   and FP32-Kahan over a bounded number of the 100 000 steps (per-step cost is constant), with
@@ -247,6 +252,92 @@ def time_thermal(ns, form, steps, warmup, fixed=None):
                 t=np.array(fld.temperature.values), tune=tune)
 
 
+A1D_N, A1D_STEPS = 2000, 5000
+
+
+def acoustic1d_field(ns, form='fp64', n=A1D_N, steps=A1D_STEPS):
+    dx = 1e-4
+    fld = ns.Acoustic1D(x_samples=n, x_delta=dx, t_samples=steps, t_delta=0.5 * dx / 1500.0,
+                        material=ns.AcousticMaterial(1500.0, 1000.0))
+    fld.form = form
+    i = np.arange(n)
+    fld.pressure.values = np.exp(-0.5 * ((i - n // 2) / 20.0) ** 2)
+    fld.velocity.add_boundary(ns.regions.PointRegion([0], None, 'left'), value=0,
+                              additive=False)
+    fld.velocity.add_boundary(ns.regions.PointRegion([n - 1], None, 'right'), value=0,
+                              additive=False)
+    fld.pressure.add_output(ns.regions.Region([n // 4, n // 2, 3 * n // 4], 'probes'))
+    return fld
+
+
+def time_acoustic1d(ns, form, steps, tile=None, cpt=None):
+    import ctypes as ct
+    from pyfds_amd import _native as nat
+    fld = acoustic1d_field(ns, form)
+    if tile is not None:
+        fld.tile = tile
+    if cpt:
+        fld.strip_height = cpt
+    fld.simulate(10)
+    runner = fld._runner()
+    t0 = time.perf_counter()
+    fld.simulate(steps)
+    wall = time.perf_counter() - t0
+    ms = ct.c_double()
+    nat.check(runner.lib.fds_last_run_ms(runner.ctx, ct.byref(ms)))
+    return dict(us_per_step=round(wall / steps * 1e6, 3),
+                gpu_us_per_step=round(ms.value * 1e3 / steps, 3),
+                gcells=round(fld.num_points * steps / wall / 1e9, 4),
+                engine=nat.engine_of(runner.ctx), p=np.array(fld.pressure.values))
+
+
+def acoustic1d_section(ns, steps, no_cpu):
+    out = {}
+    best = None
+    for cpt in (2, 4, 8):
+        r = time_acoustic1d(ns, 'fp64', steps, cpt=cpt)
+        out['line_cpt%d' % cpt] = {k: v for k, v in r.items() if k != 'p'}
+        if best is None or r['us_per_step'] < best[1]['us_per_step']:
+            best = (cpt, r)
+    tile = time_acoustic1d(ns, 'fp64', steps, tile=5)
+    out['tile_engine'] = {k: v for k, v in tile.items() if k != 'p'}
+    out['line_equals_tile_bitwise'] = bool(np.array_equal(best[1]['p'], tile['p']))
+    for form in ('fp32', 'fp32_kahan'):
+        r = time_acoustic1d(ns, form, steps, cpt=best[0])
+        out['line_' + form] = {k: v for k, v in r.items() if k != 'p'}
+        out['line_' + form]['rel_l2_vs_fp64'] = float(
+            np.linalg.norm(r['p'] - best[1]['p']) / np.linalg.norm(best[1]['p']))
+    out['fp64'] = dict(value=best[1]['gcells'], us_per_step=best[1]['us_per_step'],
+                       gpu_us_per_step=best[1]['gpu_us_per_step'], cells_per_thread=best[0],
+                       speedup_vs_tile_engine=round(tile['us_per_step']
+                                                    / best[1]['us_per_step'], 2))
+    out['config'] = ('Acoustic1D %d cells, rigid ends, 3 recorded points, %d steps, FP64 '
+                     '(wall time of simulate(), output download included)' % (A1D_N, steps))
+    if not no_cpu:
+        out['cpu_baseline'] = dict(value=round(cpu_baseline_acoustic1d(steps), 5),
+                                   unit='Gcell-updates/s', cores=1, kind='port',
+                                   sample='numpy oracle, the same Acoustic1D run, all %d steps'
+                                          % steps)
+    return out
+
+
+def cpu_baseline_acoustic1d(steps):
+    import oracle
+    n = A1D_N
+    o = oracle.PlainAcoustic1D(n, 1e-4, steps, 0.5 * 1e-4 / 1500.0,
+                               [(np.arange(n), dict(sound_velocity=1500.0, density=1000.0,
+                                                    absorption_coef=0))])
+    i = np.arange(n)
+    o.components['pressure'] = np.exp(-0.5 * ((i - n // 2) / 20.0) ** 2)
+    o.boundaries['velocity'] = [dict(indices=[0], value=0, additive=False),
+                                dict(indices=[n - 1], value=0, additive=False)]
+    o.outputs['pressure'] = [dict(indices=[n // 4, n // 2, 3 * n // 4], signals=[])]
+    o.assemble()
+    t0 = time.perf_counter()
+    o.simulate(steps)
+    return n * steps / (time.perf_counter() - t0) / 1e9
+
+
 def cpu_baseline_batched(steps, n_inst=32):
     """Oracle over the first ``n_inst`` instances of the sweep, ``steps`` steps each."""
     import oracle
@@ -373,7 +464,8 @@ def main():
     ap.add_argument('--forms', default='fp64,fp32,fp32_kahan')
     ap.add_argument('--cpu-steps', type=int, default=20)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--sections', default='acoustic2d,batched,thermal,learned')
+    ap.add_argument('--sections', default='acoustic2d,acoustic1d,batched,thermal,learned')
+    ap.add_argument('--a1d-steps', type=int, default=A1D_STEPS)
     ap.add_argument('--learned-iters', type=int, default=1500)
     ap.add_argument('--batch-steps', type=int, default=1000)
     ap.add_argument('--batch-forms', default='fp32,fp32_kahan')
@@ -452,6 +544,8 @@ def main():
         'forms': out_forms,
     }
     sections = set(args.sections.split(','))
+    if 'acoustic1d' in sections:
+        line['acoustic1d'] = acoustic1d_section(ns, args.a1d_steps, args.no_cpu)
     if 'batched' in sections:
         bat = {}
         for f in [x for x in args.batch_forms.split(',') if x]:

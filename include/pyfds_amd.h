@@ -141,7 +141,10 @@ int fds_read_outputs(fds_ctx *ctx, int comp, int64_t row0, int64_t nrows, double
  * or -1 for the default.  fds_tile_count reports the table size, fds_tile_shape its entries
  * (tx columns x ty rows).  fds_tile_engine returns 0 for an LDS-tile shape and 1 for a
  * row-streaming shape (a workgroup of tx threads marching down a strip of ty rows); streaming
- * shapes serve the lossless 2D models, other fields given one run on the default tile shape. */
+ * shapes serve the lossless 2D models, other fields given one run on the default tile shape.
+ * 1D linear models (Acoustic1D, Thermal1D) of up to 4096 cells default (-1) to the resident
+ * 1D engine: one workgroup keeps the whole line in registers for up to 2048 steps per launch;
+ * any tile index puts them on the LDS-tile kernels (one launch per step). */
 int fds_set_tile(fds_ctx *ctx, int tile_index);
 int fds_tile_count(void);
 int fds_tile_shape(int tile_index, int *tx, int *ty);
@@ -149,8 +152,12 @@ int fds_tile_engine(int tile_index);
 /* Time steps per launch and cache policy (1: non-temporal state traffic) of a shape. */
 int fds_tile_traits(int tile_index, int *steps, int *nontemporal);
 /* Strip height (rows per workgroup) of a streaming shape; 0 restores the shape's own height.
- * Ignored by LDS-tile shapes. */
+ * Ignored by LDS-tile shapes.  For the resident 1D engine the value is the number of cells per
+ * thread (1, 2, 4 or 8; 0 = chosen from the line length). */
 int fds_set_strip_height(fds_ctx *ctx, int rows);
+/* Engine that runs a context's steps: 0 LDS tiles, 1 row streaming, 2 resident 1D,
+ * 3 nonlinear 1D tiles (no reference counterpart; for tests and benchmarks). */
+int fds_engine(fds_ctx *ctx);
 
 /* Batched sweeps (no reference counterpart: pyfds runs one field per Python loop).  A batch
  * groups contexts that are fully set up (coefficients committed, state, boundaries, outputs)
