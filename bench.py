@@ -35,7 +35,10 @@ Further sections (reported under ``batched`` and ``thermal2d``):
   ends (velocity Dirichlet 0 at both ends), 3 recorded pressure points, 5000 steps; FP64 on
   the resident 1D engine (one workgroup, every cells-per-thread variant that fits) and on the
   LDS-tile engine (one launch per step in HIP-graph chunks of 50).  Latency-bound: reported
-  as time per step.
+  as time per step.  Also ThermoAcoustic1D on the same line (acoustic losses heating a
+  Thermal1D field every 4th step, accumulated in between): the device-coupled loop (one
+  workgroup steps both fields and the transfer) against the reference's host protocol over the
+  same device fields.
 * thermal2d: Thermal2D 4096x4096, k in {0.6, 50} W/mK in seeded 256-cell blocks (seed 3),
   (16x16), rho c_p identical, Fourier number 0.2 on k_max, Gaussian hot spot (sigma 64 cells); FP64
   and FP32-Kahan over a bounded number of the 100 000 steps (per-step cost is constant), with
@@ -313,11 +316,48 @@ def acoustic1d_section(ns, steps, no_cpu):
                                                     / best[1]['us_per_step'], 2))
     out['config'] = ('Acoustic1D %d cells, rigid ends, 3 recorded points, %d steps, FP64 '
                      '(wall time of simulate(), output download included)' % (A1D_N, steps))
+    out['thermoacoustic1d'] = thermoacoustic_section(ns, steps, 200)
     if not no_cpu:
         out['cpu_baseline'] = dict(value=round(cpu_baseline_acoustic1d(steps), 5),
                                    unit='Gcell-updates/s', cores=1, kind='port',
                                    sample='numpy oracle, the same Acoustic1D run, all %d steps'
                                           % steps)
+    return out
+
+
+def thermoacoustic_field(ns, steps, n=A1D_N):
+    dx = 1e-4
+    dt = 0.5 * dx / 1500.0
+    fld = ns.ThermoAcoustic1D(x_samples=n, x_delta=dx, t_samples=steps, t_delta=dt,
+                              thermal_material=ns.ThermalMaterial(4000.0, 1000.0, 0.6),
+                              acoustic_material=ns.AcousticMaterial(1500.0, 1000.0,
+                                                                    absorption_coef=3e-3),
+                              stepping=4)
+    i = np.arange(n)
+    fld.pressure.values = 1e5 * np.exp(-0.5 * ((i - n // 2) / 20.0) ** 2)
+    fld.velocity.add_boundary(ns.regions.PointRegion([0], None, 'left'), value=0,
+                              additive=False)
+    fld.velocity.add_boundary(ns.regions.PointRegion([n - 1], None, 'right'), value=0,
+                              additive=False)
+    fld.temperature.add_output(ns.regions.Region([n // 4, n // 2, 3 * n // 4], 'probes'))
+    return fld
+
+
+def thermoacoustic_section(ns, steps, host_steps):
+    out = {}
+    for mode, k in (('device_coupled', steps), ('host_protocol', host_steps)):
+        fld = thermoacoustic_field(ns, steps + 10)
+        if mode == 'host_protocol':
+            fld._device_coupling = lambda: None
+        fld.simulate(5)
+        t0 = time.perf_counter()
+        fld.simulate(k)
+        wall = time.perf_counter() - t0
+        out[mode] = dict(us_per_step=round(wall / k * 1e6, 3), steps=k)
+    out['speedup'] = round(out['host_protocol']['us_per_step']
+                           / out['device_coupled']['us_per_step'], 1)
+    out['config'] = ('ThermoAcoustic1D %d cells, loss coupling every 4th step (accumulated), '
+                     'FP64, wall time per coupled step' % A1D_N)
     return out
 
 

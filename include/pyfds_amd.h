@@ -177,6 +177,22 @@ int fds_batch_destroy(fds_batch *b);
  * reference applies the same shift on the host after each sim_step. */
 int fds_set_flow(fds_ctx *ctx, const int64_t *period, int64_t nrows);
 
+/* ThermoAcoustic1D on the device (pyfds/coupled_fields.py:10-65: SynchronizedFields.sim_step,
+ * pyfds/coupling.py:78-85, with its acoustic-loss BoundaryCoupling, pyfds/coupling.py:107-116).
+ * Each of nsteps steps from start_step advances the Acoustic1D context, then the Thermal1D
+ * context (same cell count, form and device, both on the resident 1D engine), then adds to the
+ * temperature the heat of the acoustic losses,
+ *   transfer = (loss * d**2) * dt,  d = a_v_p . (v * rho)  (the acoustic context's backward
+ *   factor B applied to v * rho; loss = mu / rho_thermal / c_p, evaluated by the caller),
+ * every step with step % stepping == 0; with `accumulate` the transfer is summed every step
+ * and the sum added (and restarted) on those steps.  accum (n values, in/out) carries that sum
+ * across calls.  Up to 2048 cells; one workgroup runs both fields.  Outputs and state read
+ * back from each context as after fds_run.  The reference applies the coupling on the host
+ * after every pair of sim_steps. */
+int fds_couple_loss_run(fds_ctx *acoustic, fds_ctx *thermal, const double *rho,
+                        const double *loss, double dt, int64_t stepping, int accumulate,
+                        double *accum, int64_t start_step, int64_t nsteps);
+
 /* Learned absorbing boundary (no reference counterpart; pyfds has only the plain method).
  * After every step of an FP32 Acoustic2D context on a square G x G grid, a BF16-MFMA network
  * predicts the outer `layer` cells (16) of p, vx, vy on all four sides from the 8 adjacent

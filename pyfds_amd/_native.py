@@ -23,6 +23,7 @@ EXPORTS = ('fds_create', 'fds_destroy', 'fds_set_scalars', 'fds_set_coeff', 'fds
            'fds_set_state', 'fds_get_state', 'fds_set_boundaries', 'fds_set_outputs',
            'fds_run', 'fds_read_outputs', 'fds_set_tile', 'fds_tile_count', 'fds_tile_shape',
            'fds_tile_engine', 'fds_tile_traits', 'fds_set_strip_height', 'fds_engine', 'fds_set_flow',
+           'fds_couple_loss_run',
            'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_batch_create', 'fds_batch_run',
            'fds_batch_last_run_ms', 'fds_batch_destroy', 'fds_set_learned_boundary',
            'fds_lbc_conv_fwd', 'fds_lbc_conv_bwd_data', 'fds_lbc_conv_bwd_weight',
@@ -103,6 +104,8 @@ def load():
         'fds_tile_traits': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         'fds_set_strip_height': (ct.c_int, [_p, ct.c_int]),
         'fds_engine': (ct.c_int, [_p]),
+        'fds_couple_loss_run': (ct.c_int, [_p, _p, _dp, _dp, ct.c_double, ct.c_int64, ct.c_int,
+                                           _dp, ct.c_int64, ct.c_int64]),
         'fds_set_flow': (ct.c_int, [_p, _lp, ct.c_int64]),
         'fds_tile_shape': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         'fds_last_error': (ct.c_char_p, []),

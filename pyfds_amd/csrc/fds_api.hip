@@ -158,6 +158,8 @@ struct fds_ctx {
   // flow along x (AcousticFlow2D): per-row shift period, applied after every step
   DevBuf flow_period;
   bool flow = false;
+  // ThermoAcoustic1D loss coupling (acoustic context): density, loss factor, accumulator
+  DevBuf couple_rho, couple_loss, couple_acc;
   // learned absorbing boundary (caller-owned bf16 weights), applied after every step
   const uint16_t *lbc_w1 = nullptr, *lbc_w2 = nullptr;
   int lbc_layer = 0;
@@ -982,11 +984,13 @@ int fds_set_strip_height(fds_ctx *c, int rows) {
   return 0;
 }
 
-int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
-  if (int rc = check_ctx(c)) return rc;
-  if (!c->committed) return fail(FDS_E_STATE, "coefficients not committed");
-  if (nsteps < 0 || start < 0) return fail(FDS_E_ARG, "negative step count");
-  if (nsteps == 0) return 0;
+}  // extern "C"
+
+namespace {
+
+// Common start of a run of nsteps steps from `start`: signal lengths, event lists, output
+// rows, the device step counter.
+int prepare_run(fds_ctx *c, int64_t start, int64_t nsteps) {
   // every boundary entry must have a value for every step of the run
   for (int k = 0; k < 3; ++k) {
     const Comp &cp = c->comp[k];
@@ -1014,14 +1018,123 @@ int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (!c->ev0) HIP_TRY(hipEventCreate(&c->ev0));
   if (!c->ev1) HIP_TRY(hipEventCreate(&c->ev1));
-  HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  int64_t done = 0;
   if (c->line) {
     // resident 1D engine: the signal table is addressed with 32-bit indices
     for (int k = 0; k < 3; ++k)
       if (c->comp[k].table.size() > (size_t)std::numeric_limits<int32_t>::max())
         return fail(FDS_E_ARG, "boundary table of component %d too large for the resident 1D "
                     "engine (select tile %d)", k, fds::kTileRow);
+  }
+  return 0;
+}
+
+template <typename T>
+int launch_couple(fds_ctx *ac, fds_ctx *th, int k, int steps, int threads,
+                  const fds::LineCouple<T> &cp) {
+  fds::StepArgs<T> a, h;
+  fill_args<T>(ac, a, ac->parity);
+  fill_args<T>(th, h, th->parity);
+  a.k = h.k = k;
+  a.steps = h.steps = steps;
+  ac->ks->couple[ac->line_var][ac->lossy ? 1 : 0](&a, &h, &cp, threads, ac->stream);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int couple_run(fds_ctx *ac, fds_ctx *th, const double *rho, const double *loss, double dt,
+               int64_t stepping, int accumulate, double *accum, int64_t start, int64_t nsteps,
+               int threads) {
+  const int64_t n = ac->n;
+  std::vector<T> r(n), l(n);
+  for (int64_t i = 0; i < n; ++i) {
+    r[i] = (T)rho[i];
+    l[i] = (T)loss[i];
+  }
+  std::vector<double> acc(accumulate ? accum : nullptr, accumulate ? accum + n : nullptr);
+  if (acc.empty()) acc.assign(1, 0.0);
+  HIP_TRY(upload(ac->couple_rho, r, ac->stream));
+  HIP_TRY(upload(ac->couple_loss, l, ac->stream));
+  HIP_TRY(upload(ac->couple_acc, acc, ac->stream));
+  fds::LineCouple<T> cp;
+  cp.rho = ac->couple_rho.as<const T>();
+  cp.loss = ac->couple_loss.as<const T>();
+  cp.acc = ac->couple_acc.as<double>();
+  cp.dt = (T)dt;
+  cp.stepping = stepping;
+  cp.accumulate = accumulate ? 1 : 0;
+  HIP_TRY(hipEventRecord(ac->ev0, ac->stream));
+  int64_t done = 0;
+  while (done < nsteps) {
+    const int st = (int)std::min<int64_t>(nsteps - done, fds::kLineSteps);
+    if (int rc = launch_couple<T>(ac, th, (int)done, st, threads, cp)) return rc;
+    ac->parity ^= 1;
+    th->parity ^= 1;
+    done += st;
+  }
+  HIP_TRY(hipEventRecord(ac->ev1, ac->stream));
+  HIP_TRY(hipStreamSynchronize(ac->stream));
+  if (accumulate)
+    HIP_TRY(hipMemcpy(accum, ac->couple_acc.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, ac->ev0, ac->ev1));
+  ac->last_run_ms = th->last_run_ms = ms;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fds_couple_loss_run(fds_ctx *ac, fds_ctx *th, const double *rho, const double *loss,
+                        double dt, int64_t stepping, int accumulate, double *accum,
+                        int64_t start, int64_t nsteps) {
+  if (int rc = check_ctx(ac)) return rc;
+  if (int rc = check_ctx(th)) return rc;
+  if (ac->model != FDS_ACOUSTIC1D || th->model != FDS_THERMAL1D)
+    return fail(FDS_E_ARG, "the loss coupling joins an Acoustic1D and a Thermal1D context");
+  if (ac->device != th->device || ac->form != th->form || ac->n != th->n)
+    return fail(FDS_E_ARG, "coupled contexts must share device, form and cell count");
+  if (!ac->committed || !th->committed) return fail(FDS_E_STATE, "coefficients not committed");
+  if (ac->tile >= 0 || th->tile >= 0)
+    return fail(FDS_E_ARG, "coupled contexts must use the resident 1D engine (tile -1)");
+  if (ac->n > fds::kCoupleMaxCells)
+    return fail(FDS_E_ARG, "coupled runs hold at most %d cells", fds::kCoupleMaxCells);
+  if (!rho || !loss || (accumulate && !accum)) return fail(FDS_E_ARG, "null coupling input");
+  if (stepping < 1) return fail(FDS_E_ARG, "coupling stepping must be at least 1");
+  if (nsteps < 0 || start < 0) return fail(FDS_E_ARG, "negative step count");
+  if (nsteps == 0) return 0;
+  // the coupled kernel's variant (fewest cells per thread that fit), shared by both contexts
+  int v = -1;
+  for (int i = 0; i < fds::kNumLineCpt && v < 0; ++i)
+    if ((ac->n + fds::kLineCpt[i] - 1) / fds::kLineCpt[i] <= fds::kCoupleMaxThreads[i]) v = i;
+  const int cpt = fds::kLineCpt[v];
+  const int threads = (int)(((ac->n + cpt - 1) / cpt + 63) / 64 * 64);
+  for (fds_ctx *c : {ac, th}) {
+    if (!c->line || c->line_var != v) {
+      c->line = true;
+      c->line_var = v;
+      c->line_threads = c->tiles_x = c->ntiles = threads;
+      c->events_dirty = true;
+      touch(c);
+    }
+    if (int rc = prepare_run(c, start, nsteps)) return rc;
+  }
+  return ac->elem == 8 ? couple_run<double>(ac, th, rho, loss, dt, stepping, accumulate, accum,
+                                            start, nsteps, threads)
+                       : couple_run<float>(ac, th, rho, loss, dt, stepping, accumulate, accum,
+                                           start, nsteps, threads);
+}
+
+int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->committed) return fail(FDS_E_STATE, "coefficients not committed");
+  if (nsteps < 0 || start < 0) return fail(FDS_E_ARG, "negative step count");
+  if (nsteps == 0) return 0;
+  if (int rc = prepare_run(c, start, nsteps)) return rc;
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  int64_t done = 0;
+  if (c->line) {
     while (done < nsteps) {
       const int st = (int)std::min<int64_t>(nsteps - done, fds::kLineSteps);
       const int rc = c->elem == 8 ? launch_line<double>(c, c->parity, (int)done, st)

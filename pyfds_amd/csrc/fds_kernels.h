@@ -27,6 +27,9 @@ constexpr int kStreamShapes[kNumStream][4] = {
 #define FDS_STREAM_SHAPES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11)
 
 using LaunchFn = void (*)(const void *args, int nblocks, size_t lds_bytes, hipStream_t s);
+// resident 1D coupled launch: acoustic and thermal step arguments, coupling arguments
+using CoupleFn = void (*)(const void *ac, const void *th, const void *cp, int threads,
+                          hipStream_t s);
 
 // Batched launch: one workgroup per tile of every instance; descs[i] are the instances'
 // step arguments, tile_inst maps a global tile to its instance, inst_base[i] is the first
@@ -51,6 +54,8 @@ struct KernelSet {
   // resident 1D engine, one workgroup: [cells per thread][acoustic, lossy acoustic, thermal];
   // the launcher's nblocks argument is the workgroup's thread count
   LaunchFn line[4][3];
+  // ThermoAcoustic1D on the resident engine: [cells per thread][lossless, lossy acoustics]
+  CoupleFn couple[4][2];
 };
 
 const KernelSet &kernels_f64();

@@ -33,6 +33,9 @@ constexpr int kLineCpt[kNumLineCpt] = {1, 2, 4, 8};   // cells per thread of the
 // VGPRs a 1024-thread workgroup leaves each lane
 constexpr int kLineMaxThreads[kNumLineCpt] = {1024, 1024, 512, 512};
 constexpr int kLineMaxCells = 4096;
+// the coupled kernel holds two fields: half the workgroup from 2 cells per thread on
+constexpr int kCoupleMaxThreads[kNumLineCpt] = {1024, 512, 512, 256};
+constexpr int kCoupleMaxCells = 2048;
 constexpr int kLineSteps = 2048;                   // time steps per launch at most
 
 enum LineFamily { kLineAcoustic = 0, kLineAcousticLossy = 1, kLineThermal = 2 };
@@ -40,31 +43,35 @@ enum LineFamily { kLineAcoustic = 0, kLineAcousticLossy = 1, kLineThermal = 2 };
 // A thread's boundary cells of one component.  Lists are CSR over threads: entries
 // ev.ptr[t]..ev.ptr[t+1]-1, ev.pos = register slot, ev.id = unique boundary cell.  The first
 // cell, when its program has a single entry (the usual case), is cached in registers with its
-// signal value fetched one step ahead; any further cell takes the general path.
+// signal value fetched one step ahead; any other cell of the thread takes the general path,
+// which re-reads the thread's list.  Kept to seven registers: two fields' worth live in the
+// coupled kernel.
 struct LineBound {
-  int e0, e1;        // this thread's cells
-  int slot;          // cached cell's register slot, -1 if none
+  int info;          // bits 0-7: cached cell's register slot (0xff: none); bit 8: general path
   int32_t idx;       // table index of the value for the next step to apply
   int32_t stride;
   double a, val;     // x = a*x + val
 
+  __device__ __forceinline__ int slot() const { return info & 0xff; }
+
   template <typename T>
   __device__ __forceinline__ void init(const EvView &ev, const ProgView &pg, int t,
                                        int64_t step0) {
-    e0 = ev.ptr[t];
-    e1 = ev.ptr[t + 1];
-    slot = -1;
+    const int e0 = ev.ptr[t], e1 = ev.ptr[t + 1];
+    info = 0xff | (e1 - e0 > 1 ? 0x100 : 0);
     idx = stride = 0;
     a = val = 0.0;
     if (e0 < e1) {
       const int u = ev.id[e0];
       const int q0 = pg.ptr[u];
       if (pg.ptr[u + 1] - q0 == 1) {
-        slot = ev.pos[e0];
+        info = (info & 0x100) | ev.pos[e0];
         a = pg.a[q0];
         stride = (int32_t)pg.stride[q0];
         idx = (int32_t)(pg.off[q0] + step0 * pg.stride[q0]);
         val = pg.table[idx];
+      } else {
+        info = 0x1ff;
       }
     }
   }
@@ -74,12 +81,13 @@ struct LineBound {
   template <typename T, bool KAHAN, int CPT>
   __device__ __forceinline__ void apply(const EvView &ev, const ProgView &pg, int64_t step,
                                         bool more, T (&x)[CPT], T (&c)[CPT]) {
-    if (slot >= 0) {
+    const int sl = slot();
+    if (sl != 0xff) {
       // the slot is tested against each register position (compile-time index) so that the
       // arrays stay in registers
 #pragma unroll
       for (int m = 0; m < CPT; ++m) {
-        if (slot != m) continue;
+        if (sl != m) continue;
         double v = KAHAN ? (double)x[m] - (double)c[m] : (double)x[m];
         const double ax = a * v;
         v = ax + val;
@@ -89,117 +97,136 @@ struct LineBound {
       idx += stride;
       if (more) val = pg.table[idx];
     }
-    for (int e = e0 + (slot >= 0 ? 1 : 0); e < e1; ++e) {
-      const int s = ev.pos[e], u = ev.id[e];
+    if (info & 0x100) {
+      const int t = threadIdx.x;
+      const int e1 = ev.ptr[t + 1];
+      for (int e = ev.ptr[t] + (sl != 0xff ? 1 : 0); e < e1; ++e) {
+        const int s = ev.pos[e], u = ev.id[e];
 #pragma unroll
-      for (int m = 0; m < CPT; ++m)
-        if (s == m) apply_prog<T, KAHAN>(pg, u, step, x[m], c[m]);
+        for (int m = 0; m < CPT; ++m)
+          if (s == m) apply_prog<T, KAHAN>(pg, u, step, x[m], c[m]);
+      }
     }
   }
 };
 
 // A thread's recorded points of one component (CSR over threads like LineBound; ev.id = the
-// output slot).  The first point is cached in registers.
+// output slot).  The first point is cached in registers; further points re-read the list.
 struct LineRecord {
-  int e0, e1, slot, id;
+  int info, id;      // info: bits 0-7 first point's register slot (0xff: none), bit 8: more
 
   __device__ __forceinline__ void init(const OutView &o, int t) {
-    e0 = o.ev.ptr[t];
-    e1 = o.ev.ptr[t + 1];
-    slot = e0 < e1 ? o.ev.pos[e0] : -1;
+    const int e0 = o.ev.ptr[t], e1 = o.ev.ptr[t + 1];
+    info = (e0 < e1 ? o.ev.pos[e0] : 0xff) | (e1 - e0 > 1 ? 0x100 : 0);
     id = e0 < e1 ? o.ev.id[e0] : 0;
   }
 
   template <typename T, bool KAHAN, int CPT>
   __device__ __forceinline__ void write(const OutView &o, int64_t row, const T (&x)[CPT],
                                         const T (&c)[CPT]) const {
-    if (slot < 0) return;
+    const int sl = info & 0xff;
+    if (sl == 0xff) return;
     double *dst = o.buf + row * o.m;
 #pragma unroll
     for (int m = 0; m < CPT; ++m)
-      if (slot == m) dst[id] = out_value<T, KAHAN>(x[m], c[m]);
-    for (int e = e0 + 1; e < e1; ++e) {
-      const int s = o.ev.pos[e], i = o.ev.id[e];
+      if (sl == m) dst[id] = out_value<T, KAHAN>(x[m], c[m]);
+    if (info & 0x100) {
+      const int t = threadIdx.x;
+      const int e1 = o.ev.ptr[t + 1];
+      for (int e = o.ev.ptr[t] + 1; e < e1; ++e) {
+        const int s = o.ev.pos[e], i = o.ev.id[e];
 #pragma unroll
-      for (int m = 0; m < CPT; ++m)
-        if (s == m) dst[i] = out_value<T, KAHAN>(x[m], c[m]);
+        for (int m = 0; m < CPT; ++m)
+          if (s == m) dst[i] = out_value<T, KAHAN>(x[m], c[m]);
+      }
     }
   }
 };
 
-// The line kernel's body for `a.steps` steps starting at step a.dyn[0] + a.k (output row
-// a.dyn[1] + a.k).  Reads state from a.in / a.cin, writes a.out / a.cout once at the end.
-template <typename T, bool KAHAN, int FAM, int CPT>
-__device__ void line1d(const StepArgs<T> &a) {
-  constexpr bool THERMAL = FAM == kLineThermal;
-  constexpr bool LOSSY = FAM == kLineAcousticLossy;
-  constexpr bool KV = KAHAN && !THERMAL;   // the second component is accumulated (velocity)
-  __shared__ T xa[kLineThreads], xb[LOSSY ? kLineThreads : 1], xc[LOSSY ? kLineThreads : 1];
-  __shared__ T xd[kLineThreads];
+// LDS exchange arrays of one field: a = first component's last cell (before the first
+// barrier), b / c = lossy velocity's first / last cell (before the first barrier), d / e = the
+// second component's first / last cell (before the second barrier).
+template <typename T>
+struct LineXchg {
+  T *a, *b, *c, *d, *e;
+};
 
-  const int t = threadIdx.x, nt = blockDim.x;
-  const int n = (int)a.n;
-  const int j0 = t * CPT;
-  // coefficient slots: backward factor (B or Q), forward factor (K or G), viscous factor M
-  const T *Fb = a.coef[THERMAL ? FDS_C_QX : FDS_C_BX];
-  const T *Ff = a.coef[THERMAL ? FDS_C_GX : FDS_C_KX];
-  const T *Fm = a.coef[FDS_C_MX];
+// One field of the resident 1D engine: a thread's block of CPT cells in registers and the
+// three parts of a step, separated by the caller's barriers.
+template <typename T, bool KAHAN, int FAM, int CPT>
+struct LineField {
+  static constexpr bool THERMAL = FAM == kLineThermal;
+  static constexpr bool LOSSY = FAM == kLineAcousticLossy;
+  static constexpr bool KV = KAHAN && !THERMAL;   // second component accumulated (velocity)
 
   T x0[CPT], c0[CPT], x1[CPT], c1[CPT];
   T fb[CPT + 1], ff[CPT + 1], fm[LOSSY ? CPT + 2 : 1];
-  unsigned valid = 0;
-#pragma unroll
-  for (int m = 0; m < CPT; ++m) {
-    const int j = j0 + m;
-    const bool in = j < n;
-    if (in) valid |= 1u << m;
-    x0[m] = in ? a.in[0][j] : T(0);
-    c0[m] = (KAHAN && in) ? a.cin[0][j] : T(0);
-    x1[m] = (!THERMAL && in) ? a.in[1][j] : T(0);
-    c1[m] = (KV && in) ? a.cin[1][j] : T(0);
-  }
-#pragma unroll
-  for (int m = 0; m <= CPT; ++m) {
-    const int jb = j0 + m - 1, jf = j0 + m;
-    fb[m] = (jb >= 0 && jb < n) ? Fb[jb] : T(0);
-    ff[m] = jf < n ? Ff[jf] : T(0);
-  }
-  if constexpr (LOSSY) {
-#pragma unroll
-    for (int m = 0; m < CPT + 2; ++m) {
-      const int j = j0 + m - 1;
-      fm[m] = (j >= 0 && j < n) ? Fm[j] : T(0);
-    }
-  }
-
-  const int64_t step0 = a.dyn[0] + a.k, row0 = a.dyn[1] + a.k;
+  unsigned valid;
+  int n, j0;
   LineBound b0, b1;
-  b0.init<T>(a.bev[0], a.prog[0], t, step0);
-  b1.init<T>(a.bev[1], a.prog[1], t, step0);
   LineRecord r0, r1;
-  r0.init(a.outp[0], t);
-  r1.init(a.outp[1], t);
 
-  for (int s = 0; s < a.steps; ++s) {
-    const int64_t step = step0 + s, row = row0 + s;
-    const bool more = s + 1 < a.steps;
+  __device__ __forceinline__ void load(const StepArgs<T> &a, int t, int64_t step0) {
+    n = (int)a.n;
+    j0 = t * CPT;
+    // coefficient slots: backward factor (B or Q), forward factor (K or G), viscous factor M
+    const T *Fb = a.coef[THERMAL ? FDS_C_QX : FDS_C_BX];
+    const T *Ff = a.coef[THERMAL ? FDS_C_GX : FDS_C_KX];
+    const T *Fm = a.coef[FDS_C_MX];
+    valid = 0;
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      const int j = j0 + m;
+      const bool in = j < n;
+      if (in) valid |= 1u << m;
+      x0[m] = in ? a.in[0][j] : T(0);
+      c0[m] = (KAHAN && in) ? a.cin[0][j] : T(0);
+      x1[m] = (!THERMAL && in) ? a.in[1][j] : T(0);
+      c1[m] = (KV && in) ? a.cin[1][j] : T(0);
+    }
+#pragma unroll
+    for (int m = 0; m <= CPT; ++m) {
+      const int jb = j0 + m - 1, jf = j0 + m;
+      fb[m] = (jb >= 0 && jb < n) ? Fb[jb] : T(0);
+      ff[m] = jf < n ? Ff[jf] : T(0);
+    }
+    if constexpr (LOSSY) {
+#pragma unroll
+      for (int m = 0; m < CPT + 2; ++m) {
+        const int j = j0 + m - 1;
+        fm[m] = (j >= 0 && j < n) ? Fm[j] : T(0);
+      }
+    }
+    b0.init<T>(a.bev[0], a.prog[0], t, step0);
+    b1.init<T>(a.bev[1], a.prog[1], t, step0);
+    r0.init(a.outp[0], t);
+    r1.init(a.outp[1], t);
+  }
 
-    // ---- first component: boundaries, record, publish the block's last cell
+  __device__ __forceinline__ bool ok(int m) const { return (valid >> m) & 1; }
+
+  // first component: boundaries, record, publish the block's last cell
+  __device__ __forceinline__ void part1(const StepArgs<T> &a, int t, int64_t step, int64_t row,
+                                        bool more, const LineXchg<T> &x) {
     b0.apply<T, KAHAN, CPT>(a.bev[0], a.prog[0], step, more, x0, c0);
     r0.write<T, KAHAN, CPT>(a.outp[0], row, x0, c0);
-    xa[t] = x0[CPT - 1];
+    x.a[t] = x0[CPT - 1];
     if constexpr (LOSSY) {
-      xb[t] = x1[0];
-      xc[t] = x1[CPT - 1];
+      x.b[t] = x1[0];
+      x.c[t] = x1[CPT - 1];
     }
-    __syncthreads();
+  }
 
-    // ---- second component from the first (velocity update / heat flux assignment)
-    const T pm = t > 0 ? xa[t - 1] : T(0);
+  // second component from the first (velocity update / heat flux assignment), its
+  // boundaries and record; publish its first cell (and, with LAST, its last cell)
+  template <bool LAST = false>
+  __device__ __forceinline__ void part2(const StepArgs<T> &a, int t, int nt, int64_t step,
+                                        int64_t row, bool more, const LineXchg<T> &x) {
+    const T pm = t > 0 ? x.a[t - 1] : T(0);
     T vo[LOSSY ? CPT + 2 : 1];
     if constexpr (LOSSY) {
-      vo[0] = t > 0 ? xc[t - 1] : T(0);
-      vo[CPT + 1] = t + 1 < nt ? xb[t + 1] : T(0);
+      vo[0] = t > 0 ? x.c[t - 1] : T(0);
+      vo[CPT + 1] = t + 1 < nt ? x.b[t + 1] : T(0);
 #pragma unroll
       for (int m = 0; m < CPT; ++m) vo[m + 1] = x1[m];
     }
@@ -208,7 +235,7 @@ __device__ void line1d(const StepArgs<T> &a) {
       const T xm = m > 0 ? x0[m - 1] : pm;
       T d = dbwd(fb[m], xm, fb[m + 1], x0[m]);
       if constexpr (THERMAL) {
-        x1[m] = ((valid >> m) & 1) ? -d : T(0);
+        x1[m] = ok(m) ? -d : T(0);
       } else {
         if constexpr (LOSSY) {
           // D2(M) v: offsets -1, 0, +1 in that order (dia_matvec), out-of-line terms absent
@@ -219,38 +246,156 @@ __device__ void line1d(const StepArgs<T> &a) {
           if (j + 1 < n) { const T q = fm[m + 2] * vo[m + 2]; y = y + q; }
           d = d - y;
         }
-        if ((valid >> m) & 1) {
+        if (ok(m)) {
           if (KAHAN) kahan_sub(x1[m], c1[m], d); else x1[m] = x1[m] - d;
         }
       }
     }
-
-    // ---- second component: boundaries, record, publish the block's first cell
     b1.apply<T, KV, CPT>(a.bev[1], a.prog[1], step, more, x1, c1);
     r1.write<T, KV, CPT>(a.outp[1], row, x1, c1);
-    xd[t] = x1[0];
-    __syncthreads();
+    x.d[t] = x1[0];
+    if constexpr (LAST) x.e[t] = x1[CPT - 1];
+  }
 
-    // ---- first component from the second
-    const T vn = t + 1 < nt ? xd[t + 1] : T(0);
+  // first component from the second
+  __device__ __forceinline__ void part3(int t, int nt, const LineXchg<T> &x) {
+    const T vn = t + 1 < nt ? x.d[t + 1] : T(0);
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
       const T xp = m + 1 < CPT ? x1[m + 1] : vn;
       const T d = dfwd(ff[m], x1[m], ff[m + 1], xp);
-      if ((valid >> m) & 1) {
+      if (ok(m)) {
         if (KAHAN) kahan_sub(x0[m], c0[m], d); else x0[m] = x0[m] - d;
       }
     }
   }
 
+  __device__ __forceinline__ void store(const StepArgs<T> &a) const {
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      const int j = j0 + m;
+      if (j >= n) continue;
+      a.out[0][j] = x0[m];
+      a.out[1][j] = x1[m];
+      if (KAHAN) a.cout[0][j] = c0[m];
+      if (KV) a.cout[1][j] = c1[m];
+    }
+  }
+};
+
+// One field for `a.steps` steps starting at step a.dyn[0] + a.k (output row a.dyn[1] + a.k).
+// Reads state from a.in / a.cin, writes a.out / a.cout once at the end.
+template <typename T, bool KAHAN, int FAM, int CPT>
+__device__ void line1d(const StepArgs<T> &a) {
+  constexpr bool LOSSY = FAM == kLineAcousticLossy;
+  __shared__ T xa[kLineThreads], xb[LOSSY ? kLineThreads : 1], xc[LOSSY ? kLineThreads : 1];
+  __shared__ T xd[kLineThreads];
+  const LineXchg<T> x{xa, xb, xc, xd, nullptr};
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int64_t step0 = a.dyn[0] + a.k, row0 = a.dyn[1] + a.k;
+  LineField<T, KAHAN, FAM, CPT> f;
+  f.load(a, t, step0);
+  for (int s = 0; s < a.steps; ++s) {
+    const int64_t step = step0 + s, row = row0 + s;
+    const bool more = s + 1 < a.steps;
+    f.part1(a, t, step, row, more, x);
+    __syncthreads();
+    f.part2(a, t, nt, step, row, more, x);
+    __syncthreads();
+    f.part3(t, nt, x);
+  }
+  f.store(a);
+}
+
+// Acoustic-loss coupling of ThermoAcoustic1D (pyfds/coupled_fields.py:46-65 through
+// pyfds/coupling.py:107-116): per-cell inputs and the accumulated transfer (in/out).
+template <typename T>
+struct LineCouple {
+  const T *rho;      // acoustic density
+  const T *loss;     // mu / rho_thermal / c_p
+  double *acc;       // accumulated transfer, n values
+  T dt;
+  int64_t stepping;
+  int accumulate;
+};
+
+// ThermoAcoustic1D for `ac.steps` steps: per step the acoustic field (a step of
+// pyfds/acoustics.py:40-52), then the thermal field (pyfds/thermal.py:40-51), then the
+// coupling (SynchronizedFields.sim_step, pyfds/coupling.py:78-85): with v the velocity just
+// computed, d = a_v_p . (v * rho) = Dbwd(B)(v * rho) and transfer = ((loss * d**2) * dt);
+// accumulated every step when `accumulate`, added to the temperature when step % stepping == 0
+// (then the accumulator restarts from zero).  Four barriers per step.
+template <typename T, bool KAHAN, int AFAM, int CPT>
+__device__ void line1d_coupled(const StepArgs<T> &ac, const StepArgs<T> &th,
+                               const LineCouple<T> &cp) {
+  constexpr bool LOSSY = AFAM == kLineAcousticLossy;
+  __shared__ T xa[kLineThreads], xb[LOSSY ? kLineThreads : 1], xc[LOSSY ? kLineThreads : 1];
+  __shared__ T xd[kLineThreads], xe[kLineThreads], ya[kLineThreads], yd[kLineThreads];
+  const LineXchg<T> xA{xa, xb, xc, xd, xe}, xT{ya, nullptr, nullptr, yd, nullptr};
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int64_t step0 = ac.dyn[0] + ac.k, row0 = ac.dyn[1] + ac.k;
+  LineField<T, KAHAN, AFAM, CPT> A;
+  LineField<T, KAHAN, kLineThermal, CPT> H;
+  A.load(ac, t, step0);
+  H.load(th, t, step0);
+  const int n = A.n, j0 = A.j0;
+  T rho[CPT + 1], loss[CPT];
+  double acc[CPT];
+#pragma unroll
+  for (int m = 0; m <= CPT; ++m) {
+    const int j = j0 + m - 1;
+    rho[m] = (j >= 0 && j < n) ? cp.rho[j] : T(0);
+  }
 #pragma unroll
   for (int m = 0; m < CPT; ++m) {
     const int j = j0 + m;
-    if (j >= n) continue;
-    a.out[0][j] = x0[m];
-    a.out[1][j] = x1[m];
-    if (KAHAN) a.cout[0][j] = c0[m];
-    if (KV) a.cout[1][j] = c1[m];
+    loss[m] = j < n ? cp.loss[j] : T(0);
+    acc[m] = (cp.accumulate && j < n) ? cp.acc[j] : 0.0;
+  }
+  for (int s = 0; s < ac.steps; ++s) {
+    const int64_t step = step0 + s, row = row0 + s;
+    const bool more = s + 1 < ac.steps;
+    A.part1(ac, t, step, row, more, xA);
+    __syncthreads();
+    A.template part2<true>(ac, t, nt, step, row, more, xA);
+    __syncthreads();
+    A.part3(t, nt, xA);
+    // loss transfer from the step's final velocity
+    T tr[CPT];
+    {
+      const T vm = t > 0 ? xA.e[t - 1] : T(0);
+#pragma unroll
+      for (int m = 0; m < CPT; ++m) {
+        const T wm = (m > 0 ? A.x1[m - 1] : vm) * rho[m];
+        const T wc = A.x1[m] * rho[m + 1];
+        const T d = dbwd(A.fb[m], wm, A.fb[m + 1], wc);
+        const T d2 = d * d;
+        const T ld = loss[m] * d2;
+        tr[m] = ld * cp.dt;
+      }
+    }
+    H.part1(th, t, step, row, more, xT);
+    __syncthreads();
+    H.part2(th, t, nt, step, row, more, xT);
+    __syncthreads();
+    H.part3(t, nt, xT);
+    const bool deliver = step % cp.stepping == 0;
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      if (!H.ok(m)) continue;
+      if (cp.accumulate) acc[m] = acc[m] + (double)tr[m];
+      if (!deliver) continue;
+      const T add = cp.accumulate ? (T)acc[m] : tr[m];
+      if (KAHAN) kahan_sub(H.x0[m], H.c0[m], -add); else H.x0[m] = H.x0[m] + add;
+      if (cp.accumulate) acc[m] = 0.0;
+    }
+  }
+  A.store(ac);
+  H.store(th);
+  if (cp.accumulate) {
+#pragma unroll
+    for (int m = 0; m < CPT; ++m)
+      if (j0 + m < n) cp.acc[j0 + m] = acc[m];
   }
 }
 

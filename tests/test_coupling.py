@@ -166,3 +166,95 @@ def test_flow_sim_step_equals_simulate():
     res = cs.collect(fld, comps)
     for k in res:
         np.testing.assert_array_equal(res[k], gold[k], err_msg=k)
+
+
+# ------------------------------------------------------------------ device-coupled loop
+def _host_protocol(fld):
+    """Force the reference's step-by-step host protocol on this object."""
+    fld._device_coupling = lambda: None
+    return fld
+
+
+def _ta1d_big(ns, n=2000, steps=2300, stepping=4):
+    dx = 1e-4
+    dt = 0.5 * dx / 1500.0
+    fld = ns.ThermoAcoustic1D(x_samples=n, x_delta=dx, t_samples=steps, t_delta=dt,
+                              thermal_material=ns.ThermalMaterial(4000.0, 1000.0, 0.6),
+                              acoustic_material=ns.AcousticMaterial(1500.0, 1000.0,
+                                                                    absorption_coef=3e-3),
+                              stepping=stepping)
+    i = np.arange(n)
+    fld.pressure.values = 1e5 * np.exp(-0.5 * ((i - n / 3) / 15.0) ** 2)
+    t = np.arange(steps) * dt
+    fld.pressure.add_boundary(ns.regions.PointRegion([n // 2], None, 'src'),
+                              value=1e4 * np.sin(2 * np.pi * 2e6 * t), additive=True)
+    fld.velocity.add_boundary(ns.regions.PointRegion([0], None, 'wall'), value=0, additive=False)
+    fld.temperature.add_output(ns.regions.Region([n // 3, n // 2], 'T'))
+    fld.velocity.add_output(ns.regions.Region([n // 2 + 1], 'v'))
+    return fld
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name', ['ta1d_s1', 'ta1d_s3'])
+def test_thermoacoustic_runs_device_coupled(name):
+    fld, plan, comps, params = cs.SCENES[name](fds)
+    assert fld._device_coupling() is not None
+    fld.simulate(5)
+    assert fds._native.engine_of(fld.fields[0]._runner().ctx) == 'line'
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('stepping', [1, 4])
+def test_device_coupled_equals_host_protocol_bitwise(stepping):
+    """2000 cells, 2300 steps in segments that cross the 2048-step launch limit, with a
+    sim_step and an accumulation left open between them."""
+    res = []
+    for host in (False, True):
+        fld = _ta1d_big(fds, stepping=stepping)
+        if host:
+            _host_protocol(fld)
+        print('stepping', stepping, 'host' if host else 'device', flush=True)
+        fld.simulate(2050)
+        fld.sim_step()
+        fld.step += 1
+        fld.simulate(2300 - 2051 - 5)
+        fld.simulate(5)
+        r = cs.collect(fld, ['pressure', 'velocity', 'temperature', 'heat_flux'])
+        r['acc'] = np.broadcast_to(np.asarray(fld.interactions[0].accumulated_transfer,
+                                              dtype=float), (2000,)).copy()
+        res.append(r)
+    for k in res[1]:
+        np.testing.assert_array_equal(res[0][k], res[1][k], err_msg=k)
+
+
+@pytest.mark.gpu
+def test_device_coupled_signal_end_raises_like_host():
+    out = []
+    for host in (False, True):
+        fld = _ta1d_big(fds, n=300, steps=100, stepping=3)
+        if host:
+            _host_protocol(fld)
+        with pytest.raises(IndexError):
+            fld.simulate(130)        # the source signal has 100 samples
+        out.append((fld.step, np.array(fld.pressure.values), np.array(fld.temperature.values),
+                    np.array(fld.temperature.outputs[0].signals)))
+    assert out[0][0] == out[1][0] == 100
+    for a, b in zip(out[0][1:], out[1][1:]):
+        np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('form', ['fp32', 'fp32_kahan'])
+def test_device_coupled_fp32_forms(form):
+    gold = np.load(os.path.join(GOLDEN, 'coupled_ta1d_s3.npz'))
+    fld, plan, comps, params = cs.SCENES['ta1d_s3'](fds)
+    for f in fld.fields:
+        f.form = form
+    res = cs.collect(cs.run(fld, plan), comps)
+    for k in res:
+        if k == 'step':
+            continue
+        ref = gold[k]
+        err = np.linalg.norm(res[k] - ref) / max(np.linalg.norm(ref), 1e-300)
+        print('  %-24s rel L2 %.3e' % (k, err))
+        assert err <= 1e-4, (form, k, err)
