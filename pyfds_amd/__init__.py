@@ -15,3 +15,12 @@ from .regions import *  # noqa: F401,F403
 from .thermal import *  # noqa: F401,F403
 
 __version__ = '0.1.0'
+
+
+def __getattr__(name):
+    # the animators (pyfds/gfx.py) pull in matplotlib.pyplot: imported on first use
+    if name in ('Animator1D', 'Animator2D', 'gfx'):
+        import importlib
+        gfx = importlib.import_module(__name__ + '.gfx')
+        return gfx if name == 'gfx' else getattr(gfx, name)
+    raise AttributeError("module 'pyfds_amd' has no attribute {!r}".format(name))

@@ -51,12 +51,16 @@ class Field:
         for mat_reg in self.material_regions:
             for mat in mat_reg.materials:
                 if hasattr(mat, mat_parameter):
-                    vec[mat_reg.region.indices] = getattr(mat, mat_parameter)
+                    self._fill(vec, mat_reg.region.indices, getattr(mat, mat_parameter))
                     found = True
         if not found:
             raise KeyError('Material parameter {} not found in set materials.'
                            .format(mat_parameter))
         return vec
+
+    def _fill(self, vec, indices, value):
+        """``vec[indices] = value`` (the order of ``indices`` does not matter here)."""
+        vec[indices] = value
 
     def assemble_matrices(self):
         raise NotImplementedError
@@ -214,6 +218,30 @@ class Field2D(Field):
     @property
     def num_points(self):
         return self.x.samples * self.y.samples
+
+    def _fill(self, vec, indices, value):
+        """``vec[indices] = value``.  Rectangles from ``get_rect_region`` (the main region among
+        them) list their cells column by column, so a plain scatter strides through memory by a
+        whole row per element (0.47 s for 4096^2 cells); when the indices are exactly such a
+        rectangle, the same cells are written as a 2D slice instead."""
+        idx = indices
+        nx, ny = self.x.samples, self.y.samples
+        if isinstance(idx, np.ndarray) and idx.ndim == 1 and idx.size >= 1 << 16 and \
+                idx.dtype.kind in 'iu' and nx > 1:
+            first = int(idx[0])
+            # rows of the rectangle: the first column steps by nx until it ends
+            col = idx[:min(idx.size, ny) + 1] - first
+            steps = np.nonzero(col != np.arange(col.size) * nx)[0]
+            h = int(steps[0]) if steps.size else col.size
+            w = idx.size // h if h else 0
+            x0, y0 = first % nx, first // nx
+            if 0 < h and w * h == idx.size and x0 + w <= nx and y0 + h <= ny:
+                expect = (first + np.arange(w, dtype=idx.dtype)[:, None]
+                          + np.arange(h, dtype=idx.dtype)[None, :] * nx).ravel()
+                if np.array_equal(idx, expect):
+                    vec.reshape(ny, nx)[y0:y0 + h, x0:x0 + w] = value
+                    return
+        vec[idx] = value
 
     def d_x(self, factors=None, variant='forward'):
         if factors is None:

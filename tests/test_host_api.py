@@ -316,3 +316,29 @@ def test_boundary_errors():
     prog = device.compile_boundaries([reg.Boundary(reg.Region([1]), np.zeros(7)),
                                       reg.Boundary(reg.Region([2]), 3.0)], 5)
     assert prog.valid_steps == 7
+
+
+def test_material_fill_of_rectangles_equals_scatter():
+    """The 2D-slice fill of column-by-column rectangles writes exactly the cells of a plain
+    scatter, for rectangles, reversed and perturbed index lists and sorted lists alike."""
+    rng = np.random.default_rng(0)
+    for trial in range(120):
+        nx, ny = int(rng.integers(260, 600)), int(rng.integers(260, 600))
+        g = fds.Acoustic2D(x_samples=nx, x_delta=1.0, y_samples=ny, y_delta=1.0,
+                           t_samples=2, t_delta=1e-3,
+                           material=fds.AcousticMaterial(1, 1))
+        x0, y0 = int(rng.integers(0, nx // 2)), int(rng.integers(0, ny // 2))
+        w, h = int(rng.integers(nx // 3, nx - x0 + 1)), int(rng.integers(ny // 3, ny - y0 + 1))
+        idx = (x0 + np.arange(w)[:, None] + (y0 + np.arange(h)[None, :]) * nx).ravel()
+        kind = trial % 4
+        if kind == 1:
+            idx = idx[::-1].copy()
+        elif kind == 2:
+            idx = idx.copy()
+            idx[int(rng.integers(0, idx.size))] = int(rng.integers(0, nx * ny))
+        elif kind == 3:
+            idx = np.sort(idx)
+        a, b = np.zeros(nx * ny), np.zeros(nx * ny)
+        g._fill(a, idx, 2.5)
+        b[idx] = 2.5
+        assert np.array_equal(a, b), (trial, kind)
