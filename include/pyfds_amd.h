@@ -142,9 +142,10 @@ int fds_read_outputs(fds_ctx *ctx, int comp, int64_t row0, int64_t nrows, double
  * (tx columns x ty rows).  fds_tile_engine returns 0 for an LDS-tile shape and 1 for a
  * row-streaming shape (a workgroup of tx threads marching down a strip of ty rows); streaming
  * shapes serve the lossless 2D models, other fields given one run on the default tile shape.
- * 1D linear models (Acoustic1D, Thermal1D) of up to 4096 cells default (-1) to the resident
- * 1D engine: one workgroup keeps the whole line in registers for up to 2048 steps per launch;
- * any tile index puts them on the LDS-tile kernels (one launch per step). */
+ * 1D models (Acoustic1D, Thermal1D, IdealGas1D, Acoustic2ndOrder1D) of up to 4096 cells
+ * default (-1) to the resident 1D engine: one workgroup keeps the whole line in registers for
+ * up to 2048 steps per launch; any tile index puts them on the LDS-tile kernels (one launch
+ * per step; nonlinear models on their 256-cell tiles whatever the index). */
 int fds_set_tile(fds_ctx *ctx, int tile_index);
 int fds_tile_count(void);
 int fds_tile_shape(int tile_index, int *tx, int *ty);

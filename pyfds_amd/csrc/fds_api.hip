@@ -298,10 +298,10 @@ bool can_stream(const fds_ctx *c) {
 // cells per thread that fit (the most threads: a step is bound by the one CU's issue rate, so
 // spreading the line over more waves wins; measured on 2000 cells: 0.59 us per step with 2
 // cells per thread, 0.62 with 4, 0.70 with 8).  Returns the variant index or -1.
-int line_variant(int64_t n, int cpt) {
+int line_variant(int64_t n, int cpt, int fam) {
   for (int i = 0; i < fds::kNumLineCpt; ++i) {
     const int64_t thr = (n + fds::kLineCpt[i] - 1) / fds::kLineCpt[i];
-    if (thr > fds::kLineMaxThreads[i]) continue;
+    if (thr > fds::line_max_threads(fam, i)) continue;
     if (cpt <= 0 || fds::kLineCpt[i] == cpt) return i;
   }
   return -1;
@@ -312,8 +312,9 @@ int choose_tiles(fds_ctx *c) {
   c->line = false;
   c->fuse = 1;
   c->ghost = 0;
-  if (c->one_d && c->tile < 0) {
-    const int v = line_variant(c->n, c->strip_rows);
+  if ((c->one_d || c->family == kNonlinear) && c->tile < 0) {
+    const int v = line_variant(c->n, c->strip_rows,
+                               c->family == kNonlinear ? fds::kLineNonlinear : fds::kLineAcoustic);
     if (v < 0 && c->strip_rows > 0)
       return fail(FDS_E_ARG, "the resident 1D engine has no %d-cell-per-thread variant for %lld "
                   "cells", c->strip_rows, (long long)c->n);
@@ -593,8 +594,9 @@ int launch_line(fds_ctx *c, int parity, int k, int steps) {
   fill_args<T>(c, a, parity);
   a.k = k;
   a.steps = steps;
-  const int fam = c->family == kThermal ? fds::kLineThermal
-                                        : (c->lossy ? fds::kLineAcousticLossy : fds::kLineAcoustic);
+  const int fam = c->family == kNonlinear ? fds::kLineNonlinear
+                  : c->family == kThermal ? fds::kLineThermal
+                  : (c->lossy ? fds::kLineAcousticLossy : fds::kLineAcoustic);
   c->ks->line[c->line_var][fam](&a, c->line_threads, 0, c->stream);
   HIP_TRY(hipGetLastError());
   return 0;
@@ -669,8 +671,9 @@ int fds_tile_count(void) { return fds::kNumTiles + fds::kNumStream; }
 
 int fds_engine(fds_ctx *c) {
   if (int rc = check_ctx(c)) return rc;
+  if (c->line) return 2;
   if (c->family == kNonlinear) return 3;
-  return c->line ? 2 : (c->streaming ? 1 : 0);
+  return c->streaming ? 1 : 0;
 }
 
 int fds_tile_engine(int i) {
@@ -957,7 +960,6 @@ int fds_set_tile(fds_ctx *c, int t) {
   if (int rc = check_ctx(c)) return rc;
   if (t < -1 || t >= fds::kNumTiles + fds::kNumStream)
     return fail(FDS_E_ARG, "tile index %d out of range", t);
-  if (c->family == kNonlinear) return 0;
   const int old = c->tile;
   c->tile = t;
   if (int rc = choose_tiles(c)) return rc;

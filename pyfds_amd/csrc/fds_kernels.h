@@ -51,9 +51,9 @@ struct KernelSet {
   LaunchFn stream_acoustic[kNumStream][3][2];   // [shape][plain, axi, isotropic][fused]
   LaunchFn stream_thermal[kNumStream][3][2];
   BatchFn stream_batch[kNumStream][2];          // [shape][fused]
-  // resident 1D engine, one workgroup: [cells per thread][acoustic, lossy acoustic, thermal];
-  // the launcher's nblocks argument is the workgroup's thread count
-  LaunchFn line[4][3];
+  // resident 1D engine, one workgroup: [cells per thread][acoustic, lossy acoustic, thermal,
+  // nonlinear]; the launcher's nblocks argument is the workgroup's thread count
+  LaunchFn line[4][4];
   // ThermoAcoustic1D on the resident engine: [cells per thread][lossless, lossy acoustics]
   CoupleFn couple[4][2];
 };

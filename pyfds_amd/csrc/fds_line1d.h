@@ -22,6 +22,7 @@
 
 #include "fds_common.h"
 #include "fds_step2d.h"
+#include "fds_step_nl.h"
 #include "../../include/pyfds_amd.h"
 
 namespace fds {
@@ -36,9 +37,18 @@ constexpr int kLineMaxCells = 4096;
 // the coupled kernel holds two fields: half the workgroup from 2 cells per thread on
 constexpr int kCoupleMaxThreads[kNumLineCpt] = {1024, 512, 512, 256};
 constexpr int kCoupleMaxCells = 2048;
+// the nonlinear models hold three components and call pow / expm1 / log1p
+constexpr int kNlLineMaxThreads[kNumLineCpt] = {512, 512, 512, 256};
+constexpr int kNlLineMaxCells = 2048;
 constexpr int kLineSteps = 2048;                   // time steps per launch at most
 
-enum LineFamily { kLineAcoustic = 0, kLineAcousticLossy = 1, kLineThermal = 2 };
+enum LineFamily { kLineAcoustic = 0, kLineAcousticLossy = 1, kLineThermal = 2, kLineNonlinear = 3 };
+constexpr int kNumLineFamilies = 4;
+
+// workgroup size limit of variant i for a family
+__host__ __device__ constexpr int line_max_threads(int fam, int i) {
+  return fam == kLineNonlinear ? kNlLineMaxThreads[i] : kLineMaxThreads[i];
+}
 
 // A thread's boundary cells of one component.  Lists are CSR over threads: entries
 // ev.ptr[t]..ev.ptr[t+1]-1, ev.pos = register slot, ev.id = unique boundary cell.  The first
@@ -305,6 +315,166 @@ __device__ void line1d(const StepArgs<T> &a) {
     f.part3(t, nt, x);
   }
   f.store(a);
+}
+
+// IdealGas1D / Acoustic2ndOrder1D (pyfds/nonlinear_acoustics.py:38-89, 133-171) for `a.steps`
+// steps, the LDS-tile kernel's arithmetic (fds_step_nl.h) term for term:
+//   p boundaries, record p
+//   v -= (Dbwd(f) p / (rho0 + rho') [+ Dcen(h) v] - D2(M) v / (rho0 + rho'))
+//   v boundaries, record v
+//   rho' -= Dfwd(f) ((rho' + rho0) v);  rho' boundaries, record rho'
+//   p = equation of state(rho')
+// f = dt/dx and h = dt/dx/2/2 are scalars; out-of-line neighbours are zeros, and the three D2
+// terms are always summed (zero factors outside), like the tile kernel.  Two barriers per
+// step: p's last cell and v's first and last cells before the first, the mass flux's first
+// cell before the second.
+template <typename T, bool KAHAN, int CPT>
+__device__ void line1d_nonlinear(const StepArgs<T> &a) {
+  __shared__ T xp[kLineThreads], xv0[kLineThreads], xv1[kLineThreads], xw[kLineThreads];
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int n = (int)a.n, j0 = t * CPT;
+  const T f = (T)a.s[FDS_S_DTDX], h = (T)a.s[FDS_S_DTDX_4];
+  const int mode = (int)a.s[FDS_S_FLAGS];
+  const T *R0 = a.coef[FDS_C_RHO0], *Mv = a.coef[FDS_C_M];
+  const T *E1 = a.coef[FDS_C_EOS1], *E2 = a.coef[FDS_C_EOS2], *CS = a.coef[FDS_C_CSQ];
+
+  T p[CPT], cp0[CPT], v[CPT], cv[CPT], r[CPT], cr[CPT];
+  T r0[CPT], mm[CPT + 2], e1[CPT], e2[CPT], cs[CPT];
+  unsigned valid = 0;
+#pragma unroll
+  for (int m = 0; m < CPT; ++m) {
+    const int j = j0 + m;
+    const bool in = j < n;
+    if (in) valid |= 1u << m;
+    p[m] = in ? a.in[0][j] : T(0);
+    cp0[m] = T(0);
+    v[m] = in ? a.in[1][j] : T(0);
+    cv[m] = (KAHAN && in) ? a.cin[1][j] : T(0);
+    r[m] = in ? a.in[2][j] : T(0);
+    cr[m] = (KAHAN && in) ? a.cin[2][j] : T(0);
+    r0[m] = in ? R0[j] : T(0);
+    e1[m] = in ? E1[j] : T(0);
+    e2[m] = in ? E2[j] : T(0);
+    cs[m] = in ? CS[j] : T(0);
+  }
+#pragma unroll
+  for (int m = 0; m < CPT + 2; ++m) {
+    const int j = j0 + m - 1;
+    mm[m] = (j >= 0 && j < n) ? Mv[j] : T(0);
+  }
+  const int64_t step0 = a.dyn[0] + a.k, row0 = a.dyn[1] + a.k;
+  LineBound b0, b1, b2;
+  b0.init<T>(a.bev[0], a.prog[0], t, step0);
+  b1.init<T>(a.bev[1], a.prog[1], t, step0);
+  b2.init<T>(a.bev[2], a.prog[2], t, step0);
+  LineRecord o0, o1, o2;
+  o0.init(a.outp[0], t);
+  o1.init(a.outp[1], t);
+  o2.init(a.outp[2], t);
+
+  for (int s = 0; s < a.steps; ++s) {
+    const int64_t step = step0 + s, row = row0 + s;
+    const bool more = s + 1 < a.steps;
+    b0.apply<T, false, CPT>(a.bev[0], a.prog[0], step, more, p, cp0);
+    o0.write<T, false, CPT>(a.outp[0], row, p, cp0);
+    xp[t] = p[CPT - 1];
+    xv0[t] = v[0];
+    xv1[t] = v[CPT - 1];
+    __syncthreads();
+
+    // velocity
+    T vo[CPT + 2];
+    vo[0] = t > 0 ? xv1[t - 1] : T(0);
+    vo[CPT + 1] = t + 1 < nt ? xv0[t + 1] : T(0);
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) vo[m + 1] = v[m];
+    const T pm = t > 0 ? xp[t - 1] : T(0);
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      const T pa = (-f) * (m > 0 ? p[m - 1] : pm);
+      const T pb = f * p[m];
+      const T A = pa + pb;                               // a_v_p . p
+      const T sr = r0[m] + r[m];                         // rho0 + rho'
+      T D = T(0);                                        // a_v_v . v, offsets -1, 0, 1
+      { const T q = mm[m] * vo[m]; D = D + q; }
+      { const T q = (T(-2) * mm[m + 1]) * vo[m + 1]; D = D + q; }
+      { const T q = mm[m + 2] * vo[m + 2]; D = D + q; }
+      T inc;
+      if (mode & kNlConvective) {
+        const T ca = (-h) * vo[m];
+        const T cb = h * vo[m + 2];
+        const T C = ca + cb;                             // a_v_v2 . v (central)
+        inc = ((A / sr) + C) - (D / sr);
+      } else {
+        inc = (A / sr) - (D / sr);
+      }
+      if ((valid >> m) & 1) {
+        if (KAHAN) kahan_sub(v[m], cv[m], inc); else v[m] = v[m] - inc;
+      }
+    }
+    b1.apply<T, KAHAN, CPT>(a.bev[1], a.prog[1], step, more, v, cv);
+    o1.write<T, KAHAN, CPT>(a.outp[1], row, v, cv);
+
+    // mass flux (rho' + rho0) v, then density
+    T w[CPT];
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      const T sr = r[m] + r0[m];
+      w[m] = ((valid >> m) & 1) ? sr * v[m] : T(0);
+    }
+    xw[t] = w[0];
+    __syncthreads();
+    const T wn = t + 1 < nt ? xw[t + 1] : T(0);
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      const T ta = (-f) * w[m];
+      const T tb = f * (m + 1 < CPT ? w[m + 1] : wn);
+      const T d = ta + tb;
+      if ((valid >> m) & 1) {
+        if (KAHAN) kahan_sub(r[m], cr[m], d); else r[m] = r[m] - d;
+      }
+    }
+    b2.apply<T, KAHAN, CPT>(a.bev[2], a.prog[2], step, more, r, cr);
+    o2.write<T, KAHAN, CPT>(a.outp[2], row, r, cr);
+
+    // equation of state
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      const T dr = KAHAN ? (T)((double)r[m] - (double)cr[m]) : r[m];
+      T pn;
+      if (mode & kNlSecondOrder) {
+        const T sq = dr * dr;
+        const T pa = e1[m] * dr;
+        const T pb = e2[m] * sq;
+        pn = pa + pb;
+      } else if (mode & kNlState) {
+        if (sizeof(T) == 8) {
+          const T ratio = (r0[m] + dr) / r0[m];
+          const T pw = (T)pow((double)ratio, (double)e2[m]);
+          pn = e1[m] * (pw - T(1));
+        } else {
+          const double xr = (double)dr / (double)r0[m];
+          pn = (T)((double)e1[m] * expm1((double)e2[m] * log1p(xr)));
+        }
+      } else {
+        pn = cs[m] * dr;
+      }
+      p[m] = ((valid >> m) & 1) ? pn : T(0);
+    }
+  }
+
+#pragma unroll
+  for (int m = 0; m < CPT; ++m) {
+    const int j = j0 + m;
+    if (j >= n) continue;
+    a.out[0][j] = p[m];
+    a.out[1][j] = v[m];
+    a.out[2][j] = r[m];
+    if (KAHAN) {
+      a.cout[1][j] = cv[m];
+      a.cout[2][j] = cr[m];
+    }
+  }
 }
 
 // Acoustic-loss coupling of ThermoAcoustic1D (pyfds/coupled_fields.py:46-65 through

@@ -1,13 +1,16 @@
-"""Resident 1D engine (one workgroup keeps an Acoustic1D / Thermal1D line in registers for up
-to 2048 steps per launch).
+"""Resident 1D engine (one workgroup keeps an Acoustic1D / Thermal1D / IdealGas1D /
+Acoustic2ndOrder1D line in registers for up to 2048 steps per launch).
 
 GPU: FP64 bit-identical to the oracle for lines that exercise every event path (cached
 boundary cell, further cells of one thread, overlapping boundaries on one cell, several
 recorded points per thread, runs that cross the 2048-step launch limit, ``sim_step`` and
 ``reset`` between runs), for every cells-per-thread variant; the FP32 and FP32-Kahan forms
 equal the LDS-tile engine's bits (both evaluate the same expressions in the same order); the
-engine choice follows the line length and the tile setting.  The reference's own 1D scenes
-(golden fixtures) run on this engine by default in ``test_device_parity.py``.
+engine choice follows the line length and the tile setting.  The nonlinear models: FP64
+bit-identical to the oracle (linear and second-order equations of state), and every form equal
+to the nonlinear tile kernel's bits (also the ideal-gas state, whose pow() comes from the device
+libm and is not pinned by the host oracle).  The reference's own 1D scenes (golden fixtures)
+run on this engine by default in ``test_device_parity.py``.
 """
 import numpy as np
 import pytest
@@ -88,6 +91,53 @@ def line_thermal(n, steps, seed):
         plan=[('simulate', 2100), ('sim_step',), ('simulate', steps - 2101)])
 
 
+def line_nonlinear(model, n, steps, seed, convective=True, nl_state=False):
+    rng = np.random.default_rng(seed)
+    dx = 1e-3
+    t = np.arange(steps + 5)
+    burst = np.sin(2 * np.pi * t / 25.0) * (t < 100)
+    if model == 'IdealGas1D':
+        c = 323.0
+        material = ('AcousticMaterial', dict(sound_velocity=c, density=1.66,
+                                             shear_viscosity=2.2e-5, isobaric_heat_cap=520.0,
+                                             isochoric_heat_cap=312.0))
+        amp, extra = 200.0, dict(model_kwargs=dict(convective=convective, nl_state=nl_state))
+    else:
+        c = 1500.0
+        material = ('AcousticMaterial2ndOrder', dict(sound_velocity=c, density=1000.0,
+                                                     shear_viscosity=1e-3, d_rho_p=c ** 2,
+                                                     d_rho2_p=2.0e7))
+        amp, extra = 1e5, {}
+    a = n // 6
+    return dict(
+        name='line_nl', model=model, **extra,
+        kwargs=dict(x_samples=n, x_delta=dx, t_samples=steps, t_delta=0.4 * dx / c),
+        material=material, regions=[], init={},
+        boundaries={
+            'pressure': [dict(indices=[a], value=amp * burst, additive=True),
+                         dict(indices=[a + 3, a + 4, a + 5],
+                              value=[1e-3 * amp * rng.standard_normal(steps + 5)
+                                     for _ in range(3)], additive=True),
+                         dict(indices=[a + 4, n // 2], value=1e-3 * amp, additive=True)],
+            'velocity': [dict(indices=[0], value=0, additive=False),
+                         dict(indices=[n - 1], value=0, additive=False)],
+            'density': [dict(indices=[2 * n // 3], value=1e-9 * rng.standard_normal(steps + 5),
+                             additive=True)]},
+        outputs={'pressure': [[a + 3, a + 4, a + 5, n // 2]], 'velocity': [[n // 3, 1]],
+                 'density': [[2 * n // 3, n - 1]]},
+        plan=[('simulate', 2100), ('sim_step',), ('simulate', steps - 2101)])
+
+
+NL_SCENES = {
+    'ig1d_clin_700': lambda: line_nonlinear('IdealGas1D', 700, 2300, 7, True, False),
+    'ig1d_nlin_2048': lambda: line_nonlinear('IdealGas1D', 2048, 2200, 8, False, False),
+    'a2o1d_1000': lambda: line_nonlinear('Acoustic2ndOrder1D', 1000, 2200, 9),
+    # the nonlinear equation of state calls pow (device libm): engine against engine only
+    'ig1d_cnl_900': lambda: line_nonlinear('IdealGas1D', 900, 2200, 10, True, True),
+}
+NL_MAX_THREADS = {1: 512, 2: 512, 4: 512, 8: 256}
+
+
 SCENES = {
     'a1d_900': lambda: line_acoustic(901, 2500, False, 1),
     'a1d_lossy_900': lambda: line_acoustic(901, 2500, True, 2),
@@ -161,3 +211,43 @@ def test_engine_choice():
         field(3000, strip_height=2)       # 1500 threads: beyond the 2-cell variant's 1024
     with pytest.raises(ValueError):
         field(100, strip_height=3)
+
+
+NL_CASES = [(name, c) for name, make in NL_SCENES.items() if 'cnl' not in name
+            for c in (0,) + CPT if c == 0 or -(-_n(make()) // c) <= NL_MAX_THREADS[c]]
+
+
+@pytest.mark.parametrize('name,cpt', NL_CASES)
+def test_nonlinear_line_fp64_matches_oracle_bitwise(name, cpt):
+    scene = NL_SCENES[name]()
+    print('nonlinear line scene', name, 'cells per thread', cpt, flush=True)
+    ref = scenes.run_oracle(scene)
+    res, eng = run(scene, strip_height=cpt)
+    assert eng == 'line'
+    for k in ref:
+        np.testing.assert_array_equal(res[k], ref[k], err_msg='%s %s' % (name, k))
+
+
+@pytest.mark.parametrize('form', ['fp64', 'fp32', 'fp32_kahan'])
+@pytest.mark.parametrize('name', sorted(NL_SCENES))
+def test_nonlinear_line_equals_tile_engine(name, form):
+    scene = NL_SCENES[name]()
+    print('nonlinear line scene', name, form, flush=True)
+    line, eng = run(scene, form=form)
+    tile, eng_t = run(scene, form=form, tile=0)
+    assert (eng, eng_t) == ('line', 'nonlinear')
+    for k in tile:
+        np.testing.assert_array_equal(line[k], tile[k], err_msg='%s %s %s' % (name, form, k))
+
+
+def test_nonlinear_engine_choice():
+    mat = pyfds_amd.AcousticMaterial(323.0, 1.66)
+
+    def field(n):
+        f = pyfds_amd.IdealGas1D(x_samples=n, x_delta=1e-3, t_samples=4, t_delta=1e-6,
+                                 material=mat)
+        f.simulate(2)
+        return nat.engine_of(f._runner().ctx)
+
+    assert field(2048) == 'line'
+    assert field(2049) == 'nonlinear'
