@@ -33,7 +33,7 @@ Further sections (reported under ``batched`` and ``thermal2d``):
   reflected energy, the same small grid without the net (ablation), wall-clock of both methods.
 * acoustic1d: Acoustic1D, 2000 cells, water, dx 1e-4, dt at CFL 0.5, Gaussian pulse, rigid
   ends (velocity Dirichlet 0 at both ends), 3 recorded pressure points, 5000 steps; FP64 on
-  the resident 1D engine (one workgroup, every cells-per-thread variant that fits) and on the
+  the resident 1D engine (its default layout of segments, and the other layouts) and on the
   LDS-tile engine (one launch per step in HIP-graph chunks of 50).  Latency-bound: reported
   as time per step.  Also ThermoAcoustic1D on the same line (acoustic losses heating a
   Thermal1D field every 4th step, accumulated in between): the device-coupled loop (one
@@ -273,14 +273,14 @@ def acoustic1d_field(ns, form='fp64', n=A1D_N, steps=A1D_STEPS):
     return fld
 
 
-def time_acoustic1d(ns, form, steps, tile=None, cpt=None):
+def time_acoustic1d(ns, form, steps, tile=None, shape=None):
     import ctypes as ct
     from pyfds_amd import _native as nat
     fld = acoustic1d_field(ns, form)
     if tile is not None:
         fld.tile = tile
-    if cpt:
-        fld.strip_height = cpt
+    if shape is not None:
+        fld.line_shape = shape
     fld.simulate(10)
     runner = fld._runner()
     t0 = time.perf_counter()
@@ -294,26 +294,36 @@ def time_acoustic1d(ns, form, steps, tile=None, cpt=None):
                 engine=nat.engine_of(runner.ctx), p=np.array(fld.pressure.values))
 
 
+# resident-engine layouts timed: (cells per thread, threads, ghost); threads 0 = one workgroup
+A1D_SHAPES = {'segments_2x128_g64': (2, 128, 64), 'segments_1x256_g64': (1, 256, 64),
+              'segments_2x256_g128': (2, 256, 128), 'one_wg_cpt2': (2, 0, 0),
+              'one_wg_cpt4': (4, 0, 0), 'one_wg_cpt8': (8, 0, 0)}
+
+
 def acoustic1d_section(ns, steps, no_cpu):
     out = {}
-    best = None
-    for cpt in (2, 4, 8):
-        r = time_acoustic1d(ns, 'fp64', steps, cpt=cpt)
-        out['line_cpt%d' % cpt] = {k: v for k, v in r.items() if k != 'p'}
-        if best is None or r['us_per_step'] < best[1]['us_per_step']:
-            best = (cpt, r)
+    dflt = time_acoustic1d(ns, 'fp64', steps)
+    out['default'] = {k: v for k, v in dflt.items() if k != 'p'}
+    best = ('default', dflt)
+    for name, shape in A1D_SHAPES.items():
+        r = time_acoustic1d(ns, 'fp64', steps, shape=shape)
+        out[name] = {k: v for k, v in r.items() if k != 'p'}
+        out[name]['equal_to_default'] = bool(np.array_equal(r['p'], dflt['p']))
+        if r['us_per_step'] < best[1]['us_per_step']:
+            best = (name, r)
     tile = time_acoustic1d(ns, 'fp64', steps, tile=5)
     out['tile_engine'] = {k: v for k, v in tile.items() if k != 'p'}
-    out['line_equals_tile_bitwise'] = bool(np.array_equal(best[1]['p'], tile['p']))
+    out['line_equals_tile_bitwise'] = bool(np.array_equal(dflt['p'], tile['p']))
     for form in ('fp32', 'fp32_kahan'):
-        r = time_acoustic1d(ns, form, steps, cpt=best[0])
+        r = time_acoustic1d(ns, form, steps)
         out['line_' + form] = {k: v for k, v in r.items() if k != 'p'}
         out['line_' + form]['rel_l2_vs_fp64'] = float(
-            np.linalg.norm(r['p'] - best[1]['p']) / np.linalg.norm(best[1]['p']))
-    out['fp64'] = dict(value=best[1]['gcells'], us_per_step=best[1]['us_per_step'],
-                       gpu_us_per_step=best[1]['gpu_us_per_step'], cells_per_thread=best[0],
+            np.linalg.norm(r['p'] - dflt['p']) / np.linalg.norm(dflt['p']))
+    out['fp64'] = dict(value=dflt['gcells'], us_per_step=dflt['us_per_step'],
+                       gpu_us_per_step=dflt['gpu_us_per_step'], layout='default',
+                       fastest_layout=best[0],
                        speedup_vs_tile_engine=round(tile['us_per_step']
-                                                    / best[1]['us_per_step'], 2))
+                                                    / dflt['us_per_step'], 2))
     out['config'] = ('Acoustic1D %d cells, rigid ends, 3 recorded points, %d steps, FP64 '
                      '(wall time of simulate(), output download included)' % (A1D_N, steps))
     out['thermoacoustic1d'] = thermoacoustic_section(ns, steps, 200)

@@ -142,10 +142,11 @@ int fds_read_outputs(fds_ctx *ctx, int comp, int64_t row0, int64_t nrows, double
  * (tx columns x ty rows).  fds_tile_engine returns 0 for an LDS-tile shape and 1 for a
  * row-streaming shape (a workgroup of tx threads marching down a strip of ty rows); streaming
  * shapes serve the lossless 2D models, other fields given one run on the default tile shape.
- * 1D models (Acoustic1D, Thermal1D, IdealGas1D, Acoustic2ndOrder1D) of up to 4096 cells
- * default (-1) to the resident 1D engine: one workgroup keeps the whole line in registers for
- * up to 2048 steps per launch; any tile index puts them on the LDS-tile kernels (one launch
- * per step; nonlinear models on their 256-cell tiles whatever the index). */
+ * 1D models (Acoustic1D, Thermal1D, IdealGas1D, Acoustic2ndOrder1D) default (-1) to the
+ * resident 1D engine: workgroups keep the line in registers for many steps per launch (one
+ * workgroup up to 512 cells, segments beyond; see fds_set_line_shape); any tile index puts
+ * them on the LDS-tile kernels (one launch per step; nonlinear models on their 256-cell tiles
+ * whatever the index). */
 int fds_set_tile(fds_ctx *ctx, int tile_index);
 int fds_tile_count(void);
 int fds_tile_shape(int tile_index, int *tx, int *ty);
@@ -156,6 +157,13 @@ int fds_tile_traits(int tile_index, int *steps, int *nontemporal);
  * Ignored by LDS-tile shapes.  For the resident 1D engine the value is the number of cells per
  * thread (1, 2, 4 or 8; 0 = chosen from the line length). */
 int fds_set_strip_height(fds_ctx *ctx, int rows);
+/* Resident 1D engine layout (no reference counterpart; autotuner and test hook).  threads = 0:
+ * one workgroup holds the whole line with cells_per_thread (1, 2, 4, 8) cells per thread.
+ * threads > 0: the line is split into segments of threads * cells_per_thread - 2 * ghost owned
+ * cells, one workgroup of `threads` threads each, every workgroup also stepping `ghost` cells
+ * of each neighbour (at least 2); a launch then runs at most ghost - 1 steps (temporal
+ * blocking), and lines of any length fit. */
+int fds_set_line_shape(fds_ctx *ctx, int cells_per_thread, int threads, int ghost);
 /* Engine that runs a context's steps: 0 LDS tiles, 1 row streaming, 2 resident 1D,
  * 3 nonlinear 1D tiles (no reference counterpart; for tests and benchmarks). */
 int fds_engine(fds_ctx *ctx);

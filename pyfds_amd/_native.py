@@ -22,7 +22,7 @@ NUM_COEFF, NUM_SCALARS = 8, 8
 EXPORTS = ('fds_create', 'fds_destroy', 'fds_set_scalars', 'fds_set_coeff', 'fds_commit',
            'fds_set_state', 'fds_get_state', 'fds_set_boundaries', 'fds_set_outputs',
            'fds_run', 'fds_read_outputs', 'fds_set_tile', 'fds_tile_count', 'fds_tile_shape',
-           'fds_tile_engine', 'fds_tile_traits', 'fds_set_strip_height', 'fds_engine', 'fds_set_flow',
+           'fds_tile_engine', 'fds_tile_traits', 'fds_set_strip_height', 'fds_engine', 'fds_set_line_shape', 'fds_set_flow',
            'fds_couple_loss_run',
            'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_batch_create', 'fds_batch_run',
            'fds_batch_last_run_ms', 'fds_batch_destroy', 'fds_set_learned_boundary',
@@ -104,6 +104,7 @@ def load():
         'fds_tile_traits': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         'fds_set_strip_height': (ct.c_int, [_p, ct.c_int]),
         'fds_engine': (ct.c_int, [_p]),
+        'fds_set_line_shape': (ct.c_int, [_p, ct.c_int, ct.c_int, ct.c_int]),
         'fds_couple_loss_run': (ct.c_int, [_p, _p, _dp, _dp, ct.c_double, ct.c_int64, ct.c_int,
                                            _dp, ct.c_int64, ct.c_int64]),
         'fds_set_flow': (ct.c_int, [_p, _lp, ct.c_int64]),

@@ -143,6 +143,10 @@ struct fds_ctx {
   bool line = false;      // resident 1D engine in use (one workgroup, many steps per launch)
   int line_var = 0;       // its variant: index into fds::kLineCpt
   int line_threads = 0;   // its workgroup size
+  int line_wgs = 1;       // workgroups (segments of the line)
+  int line_seg = 0;       // cells owned per segment
+  int line_ghost = 0;     // ghost cells per segment side = most steps per launch (0: one segment)
+  int line_cfg_threads = -1, line_cfg_ghost = 0;  // requested segmentation (-1: automatic)
   int tx = 0, ty = 0, tiles_x = 0, tiles_y = 0, ntiles = 0;
   bool events_dirty = true;
   DevBuf dyn;
@@ -313,22 +317,50 @@ int choose_tiles(fds_ctx *c) {
   c->fuse = 1;
   c->ghost = 0;
   if ((c->one_d || c->family == kNonlinear) && c->tile < 0) {
-    const int v = line_variant(c->n, c->strip_rows,
-                               c->family == kNonlinear ? fds::kLineNonlinear : fds::kLineAcoustic);
+    // segmented lines take any length; one segment must hold the whole line.  Automatic
+    // layout: one segment up to kLineSegmentAbove cells, else segments of the default shape
+    int cfg_threads = c->line_cfg_threads, cfg_ghost = c->line_cfg_ghost, cpt_req = c->strip_rows;
+    if (cfg_threads < 0) {
+      cfg_threads = 0;
+      if (c->strip_rows <= 0 && c->n > fds::kLineSegmentAbove) {
+        cfg_threads = fds::kLineSegThreads;
+        cfg_ghost = fds::kLineSegGhost;
+        cpt_req = fds::kLineSegCpt;
+      }
+    }
+    int v = -1;
+    if (cfg_threads > 0) {
+      for (int i = 0; i < fds::kNumLineCpt; ++i)
+        if (fds::kLineCpt[i] == cpt_req) v = i;
+    } else {
+      v = line_variant(c->n, cpt_req,
+                       c->family == kNonlinear ? fds::kLineNonlinear : fds::kLineAcoustic);
+    }
     if (v < 0 && c->strip_rows > 0)
       return fail(FDS_E_ARG, "the resident 1D engine has no %d-cell-per-thread variant for %lld "
                   "cells", c->strip_rows, (long long)c->n);
     if (v >= 0) {
-      // one workgroup; event lists are indexed by thread, like tiles
+      // event lists are indexed by thread over all segments, like tiles
       c->line = true;
       c->line_var = v;
       const int cpt = fds::kLineCpt[v];
-      c->line_threads = (int)(((c->n + cpt - 1) / cpt + 63) / 64 * 64);
+      if (cfg_threads > 0) {
+        // W workgroups of cfg_threads threads: L owned + 2 G ghost cells each
+        c->line_threads = cfg_threads;
+        c->line_ghost = cfg_ghost;
+        c->line_seg = c->line_threads * cpt - 2 * c->line_ghost;
+        c->line_wgs = (int)((c->n + c->line_seg - 1) / c->line_seg);
+      } else {
+        c->line_threads = (int)(((c->n + cpt - 1) / cpt + 63) / 64 * 64);
+        c->line_ghost = 0;
+        c->line_seg = (int)c->n;
+        c->line_wgs = 1;
+      }
       c->shape = -1;
       c->tx = (int)c->n;
       c->ty = 1;
       c->sx = c->tx;
-      c->tiles_x = c->ntiles = c->line_threads;
+      c->tiles_x = c->ntiles = c->line_threads * c->line_wgs;
       c->tiles_y = 1;
       c->events_dirty = true;
       return 0;
@@ -382,13 +414,26 @@ size_t lds_bytes(const fds_ctx *c) {
   return (size_t)L.total * c->elem;
 }
 
-// Event lists of the resident 1D engine: CSR over threads, pos = register slot of the cell in
-// its thread's block of line_cpt cells; id = unique boundary cell or output slot.
-void build_line_events(const fds_ctx *c, const std::vector<int64_t> &cells, Events &ev) {
+// Event lists of the resident 1D engine: CSR over the threads of all segments, pos = register
+// slot of the cell in its thread's block of cpt cells; id = unique boundary cell or output
+// slot.  Boundary cells go to every segment whose ghost-extended range holds them (ghost
+// cells must evolve like the owner's), recorded points only to their owner.
+void build_line_events(const fds_ctx *c, const std::vector<int64_t> &cells, bool outputs,
+                       Events &ev) {
   const int64_t cpt = fds::kLineCpt[c->line_var];
+  const int64_t L = c->line_seg, G = c->line_ghost, E = (int64_t)c->line_threads * cpt;
+  const int64_t W = c->line_wgs;
   std::vector<std::pair<int32_t, std::pair<int32_t, int32_t>>> all;
-  for (size_t u = 0; u < cells.size(); ++u)
-    all.push_back({(int32_t)(cells[u] / cpt), {(int32_t)(cells[u] % cpt), (int32_t)u}});
+  for (size_t u = 0; u < cells.size(); ++u) {
+    const int64_t j = cells[u];
+    const int64_t w_lo = outputs ? j / L : std::max<int64_t>(0, ceildiv(j + G - E + 1, L));
+    const int64_t w_hi = outputs ? j / L : std::min<int64_t>(W - 1, floordiv(j + G, L));
+    for (int64_t w = w_lo; w <= w_hi; ++w) {
+      const int64_t local = j - (w * L - G);
+      all.push_back({(int32_t)(w * c->line_threads + local / cpt),
+                     {(int32_t)(local % cpt), (int32_t)u}});
+    }
+  }
   std::sort(all.begin(), all.end());
   ev.ptr.assign(c->ntiles + 1, 0);
   ev.pos.clear();
@@ -409,12 +454,13 @@ int rebuild_events(fds_ctx *c) {
     Comp &cp = c->comp[k];
     if (!cp.present) continue;
     Events ev;
-    if (c->line) build_line_events(c, cp.cells, ev); else build_bound_events(c, k, ev);
+    if (c->line) build_line_events(c, cp.cells, false, ev); else build_bound_events(c, k, ev);
     HIP_TRY(upload(cp.ev_ptr, ev.ptr, c->stream));
     HIP_TRY(upload(cp.ev_pos, ev.pos, c->stream));
     HIP_TRY(upload(cp.ev_id, ev.id, c->stream));
     Events oe;
-    if (c->line) build_line_events(c, cp.out_cells, oe); else build_output_events(c, k, oe);
+    if (c->line) build_line_events(c, cp.out_cells, true, oe);
+    else build_output_events(c, k, oe);
     HIP_TRY(upload(cp.oev_ptr, oe.ptr, c->stream));
     HIP_TRY(upload(cp.oev_pos, oe.pos, c->stream));
     HIP_TRY(upload(cp.oev_id, oe.id, c->stream));
@@ -594,10 +640,12 @@ int launch_line(fds_ctx *c, int parity, int k, int steps) {
   fill_args<T>(c, a, parity);
   a.k = k;
   a.steps = steps;
+  a.strip_h = c->line_seg;
+  a.ghost = c->line_ghost;
   const int fam = c->family == kNonlinear ? fds::kLineNonlinear
                   : c->family == kThermal ? fds::kLineThermal
                   : (c->lossy ? fds::kLineAcousticLossy : fds::kLineAcoustic);
-  c->ks->line[c->line_var][fam](&a, c->line_threads, 0, c->stream);
+  c->ks->line[c->line_var][fam](&a, c->line_wgs, c->line_threads, c->stream);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -972,6 +1020,37 @@ int fds_set_tile(fds_ctx *c, int t) {
   return 0;
 }
 
+int fds_set_line_shape(fds_ctx *c, int cells_per_thread, int threads, int ghost) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->one_d && c->family != kNonlinear)
+    return fail(FDS_E_ARG, "the resident engine serves 1D models only");
+  const int fam = c->family == kNonlinear ? fds::kLineNonlinear : fds::kLineAcoustic;
+  int v = -1;
+  for (int i = 0; i < fds::kNumLineCpt; ++i)
+    if (fds::kLineCpt[i] == cells_per_thread) v = i;
+  if (v < 0) return fail(FDS_E_ARG, "cells per thread must be 1, 2, 4 or 8");
+  if (threads != 0) {
+    if (threads < 64 || threads % 64 || threads > fds::line_max_threads(fam, v))
+      return fail(FDS_E_ARG, "segment workgroup of %d threads: a multiple of 64 up to %d",
+                  threads, fds::line_max_threads(fam, v));
+    if (ghost < 2 || threads * cells_per_thread - 2 * ghost < 1)
+      return fail(FDS_E_ARG, "ghost cells must be at least 2 and leave owned cells");
+  }
+  const int o_rows = c->strip_rows, o_t = c->line_cfg_threads, o_g = c->line_cfg_ghost;
+  c->strip_rows = cells_per_thread;
+  c->line_cfg_threads = threads;
+  c->line_cfg_ghost = threads ? ghost : 0;
+  if (int rc = choose_tiles(c)) {
+    c->strip_rows = o_rows;
+    c->line_cfg_threads = o_t;
+    c->line_cfg_ghost = o_g;
+    choose_tiles(c);
+    return rc;
+  }
+  touch(c);
+  return 0;
+}
+
 int fds_set_strip_height(fds_ctx *c, int rows) {
   if (int rc = check_ctx(c)) return rc;
   if (rows < 0 || rows > 65536) return fail(FDS_E_ARG, "strip height %d out of range", rows);
@@ -1038,7 +1117,9 @@ int launch_couple(fds_ctx *ac, fds_ctx *th, int k, int steps, int threads,
   fill_args<T>(th, h, th->parity);
   a.k = h.k = k;
   a.steps = h.steps = steps;
-  ac->ks->couple[ac->line_var][ac->lossy ? 1 : 0](&a, &h, &cp, threads, ac->stream);
+  a.strip_h = h.strip_h = (int)ac->n;
+  a.ghost = h.ghost = 0;
+  ac->ks->couple[ac->line_var][ac->lossy ? 1 : 0](&a, &h, &cp, 1, threads, ac->stream);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1113,10 +1194,13 @@ int fds_couple_loss_run(fds_ctx *ac, fds_ctx *th, const double *rho, const doubl
   const int cpt = fds::kLineCpt[v];
   const int threads = (int)(((ac->n + cpt - 1) / cpt + 63) / 64 * 64);
   for (fds_ctx *c : {ac, th}) {
-    if (!c->line || c->line_var != v) {
+    if (!c->line || c->line_var != v || c->line_wgs != 1 || c->line_threads != threads) {
       c->line = true;
       c->line_var = v;
       c->line_threads = c->tiles_x = c->ntiles = threads;
+      c->line_wgs = 1;
+      c->line_seg = (int)c->n;
+      c->line_ghost = 0;
       c->events_dirty = true;
       touch(c);
     }
@@ -1137,8 +1221,14 @@ int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   int64_t done = 0;
   if (c->line) {
+    // one segment: up to kLineSteps steps per launch.  Several: a wrong value enters each
+    // segment from its open ends, one cell per step from the left and, where the update of
+    // the second component reads its right neighbour's old value (viscous and convective
+    // terms, the nonlinear mass flux), one more cell on the right: ghost - 1 steps per launch.
+    const int per = c->line_wgs > 1 ? std::min(c->line_ghost - 1, fds::kLineSteps)
+                                    : fds::kLineSteps;
     while (done < nsteps) {
-      const int st = (int)std::min<int64_t>(nsteps - done, fds::kLineSteps);
+      const int st = (int)std::min<int64_t>(nsteps - done, per);
       const int rc = c->elem == 8 ? launch_line<double>(c, c->parity, (int)done, st)
                                   : launch_line<float>(c, c->parity, (int)done, st);
       if (rc) return rc;

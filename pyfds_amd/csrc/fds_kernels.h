@@ -27,9 +27,11 @@ constexpr int kStreamShapes[kNumStream][4] = {
 #define FDS_STREAM_SHAPES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11)
 
 using LaunchFn = void (*)(const void *args, int nblocks, size_t lds_bytes, hipStream_t s);
-// resident 1D coupled launch: acoustic and thermal step arguments, coupling arguments
-using CoupleFn = void (*)(const void *ac, const void *th, const void *cp, int threads,
-                          hipStream_t s);
+// resident 1D launches: `grid` workgroups (segments) of `threads` threads; the coupled one
+// takes acoustic and thermal step arguments and the coupling arguments
+using LineFn = void (*)(const void *args, int grid, int threads, hipStream_t s);
+using CoupleFn = void (*)(const void *ac, const void *th, const void *cp, int grid,
+                          int threads, hipStream_t s);
 
 // Batched launch: one workgroup per tile of every instance; descs[i] are the instances'
 // step arguments, tile_inst maps a global tile to its instance, inst_base[i] is the first
@@ -51,9 +53,8 @@ struct KernelSet {
   LaunchFn stream_acoustic[kNumStream][3][2];   // [shape][plain, axi, isotropic][fused]
   LaunchFn stream_thermal[kNumStream][3][2];
   BatchFn stream_batch[kNumStream][2];          // [shape][fused]
-  // resident 1D engine, one workgroup: [cells per thread][acoustic, lossy acoustic, thermal,
-  // nonlinear]; the launcher's nblocks argument is the workgroup's thread count
-  LaunchFn line[4][4];
+  // resident 1D engine: [cells per thread][acoustic, lossy acoustic, thermal, nonlinear]
+  LineFn line[4][4];
   // ThermoAcoustic1D on the resident engine: [cells per thread][lossless, lossy acoustics]
   CoupleFn couple[4][2];
 };

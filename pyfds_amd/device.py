@@ -133,6 +133,7 @@ class DeviceRunner:
         self.chunk = getattr(field, 'graph_chunk', 50)
         self.tile = getattr(field, 'tile', None)
         self.strip_height = getattr(field, 'strip_height', None)
+        self.line_shape = getattr(field, 'line_shape', None)
 
     # -- lifetime -------------------------------------------------------------------------
     def _create(self, field):
@@ -150,6 +151,9 @@ class DeviceRunner:
             nat.check(self.lib.fds_set_tile(self.ctx, int(self.tile)))
         if self.strip_height is not None:
             nat.check(self.lib.fds_set_strip_height(self.ctx, int(self.strip_height)))
+        if self.line_shape is not None:
+            cpt, threads, ghost = (int(v) for v in self.line_shape)
+            nat.check(self.lib.fds_set_line_shape(self.ctx, cpt, threads, ghost))
 
     def close(self):
         if self.ctx is not None and self.lib is not None:

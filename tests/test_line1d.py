@@ -205,7 +205,7 @@ def test_engine_choice():
         return nat.engine_of(f._runner().ctx)
 
     assert field(4096) == 'line'
-    assert field(4097) == 'tile'
+    assert field(100000) == 'line'           # segments: any length
     assert field(100, tile=TILE_ROW) == 'tile'
     with pytest.raises(ValueError):
         field(3000, strip_height=2)       # 1500 threads: beyond the 2-cell variant's 1024
@@ -250,4 +250,40 @@ def test_nonlinear_engine_choice():
         return nat.engine_of(f._runner().ctx)
 
     assert field(2048) == 'line'
-    assert field(2049) == 'nonlinear'
+    assert field(50000) == 'line'
+
+
+# Segmented lines: several workgroups, each stepping its owned cells plus `ghost` cells of each
+# neighbour for at most `ghost` steps per launch.  (cells per thread, threads, ghost cells)
+SEG_CFGS = [(1, 128, 16), (2, 256, 64), (4, 128, 100), (8, 64, 7)]
+SEG_SCENES = {
+    'a1d_2000': SCENES['a1d_2000'],
+    'a1d_lossy_900': SCENES['a1d_lossy_900'],
+    't1d_900': SCENES['t1d_900'],
+    'a1d_lossy_9000': lambda: line_acoustic(9000, 2200, True, 11),
+    'ig1d_clin_700': NL_SCENES['ig1d_clin_700'],
+    'a2o1d_1000': NL_SCENES['a2o1d_1000'],
+}
+
+
+@pytest.mark.parametrize('cfg', SEG_CFGS)
+@pytest.mark.parametrize('name', sorted(SEG_SCENES))
+def test_segmented_line_fp64_matches_oracle_bitwise(name, cfg):
+    scene = SEG_SCENES[name]()
+    print('segmented line scene', name, cfg, flush=True)
+    ref = scenes.run_oracle(scene)
+    res, eng = run(scene, line_shape=cfg)
+    assert eng == 'line'
+    for k in ref:
+        np.testing.assert_array_equal(res[k], ref[k], err_msg='%s %s %s' % (name, cfg, k))
+
+
+@pytest.mark.parametrize('form', ['fp32', 'fp32_kahan'])
+@pytest.mark.parametrize('name', ['a1d_lossy_900', 't1d_900', 'ig1d_cnl_900'])
+def test_segmented_line_fp32_forms_equal_one_segment(name, form):
+    make = SEG_SCENES.get(name) or NL_SCENES[name]
+    one, _ = run(make(), form=form)
+    seg, eng = run(make(), form=form, line_shape=(2, 128, 40))
+    assert eng == 'line'
+    for k in one:
+        np.testing.assert_array_equal(seg[k], one[k], err_msg='%s %s %s' % (name, form, k))
