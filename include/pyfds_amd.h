@@ -195,9 +195,10 @@ int fds_set_flow(fds_ctx *ctx, const int64_t *period, int64_t nrows);
  *   factor B applied to v * rho; loss = mu / rho_thermal / c_p, evaluated by the caller),
  * every step with step % stepping == 0; with `accumulate` the transfer is summed every step
  * and the sum added (and restarted) on those steps.  accum (n values, in/out) carries that sum
- * across calls.  Up to 2048 cells; one workgroup runs both fields.  Outputs and state read
- * back from each context as after fds_run.  The reference applies the coupling on the host
- * after every pair of sim_steps. */
+ * across calls.  Any length: one workgroup runs both fields up to 512 cells, segments beyond
+ * (the call sets both contexts' resident layout).  Outputs and state read back from each
+ * context as after fds_run.  The reference applies the coupling on the host after every pair
+ * of sim_steps. */
 int fds_couple_loss_run(fds_ctx *acoustic, fds_ctx *thermal, const double *rho,
                         const double *loss, double dt, int64_t stepping, int accumulate,
                         double *accum, int64_t start_step, int64_t nsteps);

@@ -5,7 +5,7 @@
 workgroup steps the acoustic field, the thermal field and the loss transfer together) when the
 object is in the configuration the reference's constructor builds: its own acoustic-loss
 coupling as the only interaction, both fields' own ``sim_step`` and ``material_vector``, one
-form, at most 2048 cells.  Any other configuration (a replaced interaction list or transfer
+form, no tile shape forced on a field.  Any other configuration (a replaced interaction list or transfer
 function, a subclass overriding ``sim_step``, a field forced onto a tile shape, ...) runs the
 reference's host protocol over the device-stepped fields, step by step.
 """
@@ -22,8 +22,6 @@ from . import thermal as th
 __all__ = [
     'ThermoAcoustic1D',
 ]
-
-COUPLED_MAX_CELLS = 2048
 
 
 class ThermoAcoustic1D(cp.SynchronizedFields):
@@ -93,7 +91,7 @@ class ThermoAcoustic1D(cp.SynchronizedFields):
                 return None
         if getattr(acoustic, 'form', 'fp64') != getattr(thermal, 'form', 'fp64'):
             return None
-        if acoustic.num_points != thermal.num_points or acoustic.num_points > COUPLED_MAX_CELLS:
+        if acoustic.num_points != thermal.num_points:
             return None
         return coupling
 

@@ -1117,9 +1117,10 @@ int launch_couple(fds_ctx *ac, fds_ctx *th, int k, int steps, int threads,
   fill_args<T>(th, h, th->parity);
   a.k = h.k = k;
   a.steps = h.steps = steps;
-  a.strip_h = h.strip_h = (int)ac->n;
-  a.ghost = h.ghost = 0;
-  ac->ks->couple[ac->line_var][ac->lossy ? 1 : 0](&a, &h, &cp, 1, threads, ac->stream);
+  a.strip_h = h.strip_h = ac->line_seg;
+  a.ghost = h.ghost = ac->line_ghost;
+  ac->ks->couple[ac->line_var][ac->lossy ? 1 : 0](&a, &h, &cp, ac->line_wgs, threads,
+                                                   ac->stream);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1147,9 +1148,13 @@ int couple_run(fds_ctx *ac, fds_ctx *th, const double *rho, const double *loss, 
   cp.stepping = stepping;
   cp.accumulate = accumulate ? 1 : 0;
   HIP_TRY(hipEventRecord(ac->ev0, ac->stream));
+  // segments: the loss transfer reads the velocity's left neighbour, so wrong values reach
+  // one cell further into the temperature than into the fields: ghost - 2 steps per launch
+  const int per = ac->line_wgs > 1 ? std::min(ac->line_ghost - 2, fds::kLineSteps)
+                                   : fds::kLineSteps;
   int64_t done = 0;
   while (done < nsteps) {
-    const int st = (int)std::min<int64_t>(nsteps - done, fds::kLineSteps);
+    const int st = (int)std::min<int64_t>(nsteps - done, per);
     if (int rc = launch_couple<T>(ac, th, (int)done, st, threads, cp)) return rc;
     ac->parity ^= 1;
     th->parity ^= 1;
@@ -1181,31 +1186,31 @@ int fds_couple_loss_run(fds_ctx *ac, fds_ctx *th, const double *rho, const doubl
   if (!ac->committed || !th->committed) return fail(FDS_E_STATE, "coefficients not committed");
   if (ac->tile >= 0 || th->tile >= 0)
     return fail(FDS_E_ARG, "coupled contexts must use the resident 1D engine (tile -1)");
-  if (ac->n > fds::kCoupleMaxCells)
-    return fail(FDS_E_ARG, "coupled runs hold at most %d cells", fds::kCoupleMaxCells);
   if (!rho || !loss || (accumulate && !accum)) return fail(FDS_E_ARG, "null coupling input");
   if (stepping < 1) return fail(FDS_E_ARG, "coupling stepping must be at least 1");
   if (nsteps < 0 || start < 0) return fail(FDS_E_ARG, "negative step count");
   if (nsteps == 0) return 0;
-  // the coupled kernel's variant (fewest cells per thread that fit), shared by both contexts
-  int v = -1;
-  for (int i = 0; i < fds::kNumLineCpt && v < 0; ++i)
-    if ((ac->n + fds::kLineCpt[i] - 1) / fds::kLineCpt[i] <= fds::kCoupleMaxThreads[i]) v = i;
-  const int cpt = fds::kLineCpt[v];
-  const int threads = (int)(((ac->n + cpt - 1) / cpt + 63) / 64 * 64);
+  // one layout for both contexts: one workgroup of one cell per thread for short lines (the
+  // coupled kernel holds two fields, so it has half the single-field threads from 2 cells per
+  // thread on), else the single-field default segments
+  const bool seg = ac->n > fds::kLineSegmentAbove;
+  const int cpt = seg ? fds::kLineSegCpt : 1;
+  const int cfg_t = seg ? fds::kLineSegThreads : 0, cfg_g = seg ? fds::kLineSegGhost : 0;
+  static_assert(fds::kLineSegThreads <= fds::kCoupleMaxThreads[1] && fds::kLineSegCpt == 2 &&
+                    fds::kLineSegmentAbove <= fds::kCoupleMaxThreads[0],
+                "coupled layouts must fit the coupled kernel's launch bounds");
   for (fds_ctx *c : {ac, th}) {
-    if (!c->line || c->line_var != v || c->line_wgs != 1 || c->line_threads != threads) {
-      c->line = true;
-      c->line_var = v;
-      c->line_threads = c->tiles_x = c->ntiles = threads;
-      c->line_wgs = 1;
-      c->line_seg = (int)c->n;
-      c->line_ghost = 0;
-      c->events_dirty = true;
+    if (!c->line || c->strip_rows != cpt || c->line_cfg_threads != cfg_t ||
+        c->line_cfg_ghost != cfg_g) {
+      c->strip_rows = cpt;
+      c->line_cfg_threads = cfg_t;
+      c->line_cfg_ghost = cfg_g;
+      if (int rc = choose_tiles(c)) return rc;
       touch(c);
     }
     if (int rc = prepare_run(c, start, nsteps)) return rc;
   }
+  const int threads = ac->line_threads;
   return ac->elem == 8 ? couple_run<double>(ac, th, rho, loss, dt, stepping, accumulate, accum,
                                             start, nsteps, threads)
                        : couple_run<float>(ac, th, rho, loss, dt, stepping, accumulate, accum,

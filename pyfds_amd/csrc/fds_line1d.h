@@ -41,7 +41,6 @@ constexpr int kLineMaxThreads[kNumLineCpt] = {1024, 1024, 512, 512};
 constexpr int kLineMaxCells = 4096;
 // the coupled kernel holds two fields: half the workgroup from 2 cells per thread on
 constexpr int kCoupleMaxThreads[kNumLineCpt] = {1024, 512, 512, 256};
-constexpr int kCoupleMaxCells = 2048;
 // the nonlinear models hold three components and call pow / expm1 / log1p
 constexpr int kNlLineMaxThreads[kNumLineCpt] = {512, 512, 512, 256};
 constexpr int kNlLineMaxCells = 2048;
@@ -518,7 +517,9 @@ struct LineCouple {
 // coupling (SynchronizedFields.sim_step, pyfds/coupling.py:78-85): with v the velocity just
 // computed, d = a_v_p . (v * rho) = Dbwd(B)(v * rho) and transfer = ((loss * d**2) * dt);
 // accumulated every step when `accumulate`, added to the temperature when step % stepping == 0
-// (then the accumulator restarts from zero).  Four barriers per step.
+// (then the accumulator restarts from zero).  Four barriers per step.  In segments the
+// transfer reads the velocity's left neighbour, so wrong values reach one cell further into
+// the temperature: a launch runs at most ghost - 2 steps.
 template <typename T, bool KAHAN, int AFAM, int CPT>
 __device__ void line1d_coupled(const StepArgs<T> &ac, const StepArgs<T> &th,
                                const LineCouple<T> &cp) {

@@ -204,13 +204,14 @@ def test_thermoacoustic_runs_device_coupled(name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('n', [400, 2000])
 @pytest.mark.parametrize('stepping', [1, 4])
-def test_device_coupled_equals_host_protocol_bitwise(stepping):
-    """2000 cells, 2300 steps in segments that cross the 2048-step launch limit, with a
-    sim_step and an accumulation left open between them."""
+def test_device_coupled_equals_host_protocol_bitwise(stepping, n):
+    """400 cells (one workgroup) and 2000 cells (segments), 2300 steps in runs that cross the
+    2048-step launch limit, with a sim_step and an accumulation left open between them."""
     res = []
     for host in (False, True):
-        fld = _ta1d_big(fds, stepping=stepping)
+        fld = _ta1d_big(fds, n=n, stepping=stepping)
         if host:
             _host_protocol(fld)
         print('stepping', stepping, 'host' if host else 'device', flush=True)
@@ -221,7 +222,7 @@ def test_device_coupled_equals_host_protocol_bitwise(stepping):
         fld.simulate(5)
         r = cs.collect(fld, ['pressure', 'velocity', 'temperature', 'heat_flux'])
         r['acc'] = np.broadcast_to(np.asarray(fld.interactions[0].accumulated_transfer,
-                                              dtype=float), (2000,)).copy()
+                                              dtype=float), (n,)).copy()
         res.append(r)
     for k in res[1]:
         np.testing.assert_array_equal(res[0][k], res[1][k], err_msg=k)
