@@ -295,9 +295,9 @@ def time_acoustic1d(ns, form, steps, tile=None, shape=None):
 
 
 # resident-engine layouts timed: (cells per thread, threads, ghost); threads 0 = one workgroup
-A1D_SHAPES = {'segments_2x128_g64': (2, 128, 64), 'segments_1x256_g64': (1, 256, 64),
-              'segments_2x256_g128': (2, 256, 128), 'one_wg_cpt2': (2, 0, 0),
-              'one_wg_cpt4': (4, 0, 0), 'one_wg_cpt8': (8, 0, 0)}
+A1D_SHAPES = {'wave_segments_2x64_g32': (2, 64, 32), 'wave_segments_4x64_g96': (4, 64, 96),
+              'segments_2x128_g64': (2, 128, 64), 'segments_2x256_g128': (2, 256, 128),
+              'one_wg_cpt2': (2, 0, 0), 'one_wg_cpt4': (4, 0, 0)}
 
 
 def acoustic1d_section(ns, steps, no_cpu):

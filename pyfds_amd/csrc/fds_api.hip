@@ -109,6 +109,7 @@ struct Comp {
   // outputs
   std::vector<int64_t> out_cells;
   DevBuf oev_ptr, oev_pos, oev_id;
+  DevBuf lhead, ohead;   // resident 1D engine: per-thread first boundary cell / output point
   DevBuf obuf;
   int64_t obuf_rows = 0;
 };
@@ -323,9 +324,10 @@ int choose_tiles(fds_ctx *c) {
     if (cfg_threads < 0) {
       cfg_threads = 0;
       if (c->strip_rows <= 0 && c->n > fds::kLineSegmentAbove) {
+        const bool lng = c->n > fds::kLineSegLongAbove;
         cfg_threads = fds::kLineSegThreads;
-        cfg_ghost = fds::kLineSegGhost;
-        cpt_req = fds::kLineSegCpt;
+        cfg_ghost = lng ? fds::kLineSegLongGhost : fds::kLineSegGhost;
+        cpt_req = lng ? fds::kLineSegLongCpt : fds::kLineSegCpt;
       }
     }
     int v = -1;
@@ -446,6 +448,31 @@ void build_line_events(const fds_ctx *c, const std::vector<int64_t> &cells, bool
   for (int t = 0; t < c->ntiles; ++t) ev.ptr[t + 1] += ev.ptr[t];
 }
 
+// Per-thread heads of the resident 1D engine's event lists: the first boundary cell with its
+// program entry (when the program has one entry), the first recorded point.
+void build_line_heads(const Comp &cp, const Events &ev, const Events &oe, int nthreads,
+                      std::vector<fds::LineEvHead> &bh, std::vector<fds::LineOutHead> &oh) {
+  bh.assign(nthreads, fds::LineEvHead{0xff, 0, 0, 0.0});
+  oh.assign(nthreads, fds::LineOutHead{0xff, 0});
+  for (int t = 0; t < nthreads; ++t) {
+    const int e0 = ev.ptr[t], e1 = ev.ptr[t + 1];
+    if (e0 < e1) {
+      const int u = ev.id[e0];
+      const int q0 = cp.ptr[u];
+      if (cp.ptr[u + 1] - q0 == 1) {
+        bh[t].info = ev.pos[e0] | (e1 - e0 > 1 ? 0x100 : 0);
+        bh[t].stride = (int32_t)cp.stride[q0];
+        bh[t].off = cp.off[q0];
+        bh[t].a = cp.a[q0];
+      } else {
+        bh[t].info = 0x1ff;
+      }
+    }
+    const int o0 = oe.ptr[t], o1 = oe.ptr[t + 1];
+    if (o0 < o1) oh[t] = fds::LineOutHead{oe.pos[o0] | (o1 - o0 > 1 ? 0x100 : 0), oe.id[o0]};
+  }
+}
+
 int rebuild_events(fds_ctx *c) {
   if (!c->events_dirty) return 0;
   std::vector<int32_t> zp(c->ntiles + 1, 0);
@@ -464,6 +491,13 @@ int rebuild_events(fds_ctx *c) {
     HIP_TRY(upload(cp.oev_ptr, oe.ptr, c->stream));
     HIP_TRY(upload(cp.oev_pos, oe.pos, c->stream));
     HIP_TRY(upload(cp.oev_id, oe.id, c->stream));
+    if (c->line) {
+      std::vector<fds::LineEvHead> bh;
+      std::vector<fds::LineOutHead> oh;
+      build_line_heads(cp, ev, oe, c->ntiles, bh, oh);
+      HIP_TRY(upload(cp.lhead, bh, c->stream));
+      HIP_TRY(upload(cp.ohead, oh, c->stream));
+    }
   }
   c->events_dirty = false;
   touch(c);
@@ -515,6 +549,8 @@ void fill_args(const fds_ctx *c, fds::StepArgs<T> &a, int parity) {
     a.outp[k].ev.id = cp.oev_id.as<const int32_t>();
     a.outp[k].buf = cp.obuf.as<double>();
     a.outp[k].m = (int32_t)cp.out_cells.size();
+    a.lhead[k] = cp.lhead.p;
+    a.ohead[k] = cp.ohead.p;
   }
   for (int s = 0; s < FDS_NUM_COEFF; ++s)
     a.coef[s] = c->coef_set[s] ? c->coef[s].as<const T>() : c->zeros.as<const T>();
@@ -1197,7 +1233,8 @@ int fds_couple_loss_run(fds_ctx *ac, fds_ctx *th, const double *rho, const doubl
   const int cpt = seg ? fds::kLineSegCpt : 1;
   const int cfg_t = seg ? fds::kLineSegThreads : 0, cfg_g = seg ? fds::kLineSegGhost : 0;
   static_assert(fds::kLineSegThreads <= fds::kCoupleMaxThreads[1] && fds::kLineSegCpt == 2 &&
-                    fds::kLineSegmentAbove <= fds::kCoupleMaxThreads[0],
+                    fds::kLineSegmentAbove <= fds::kCoupleMaxThreads[0] &&
+                    fds::kLineSegGhost > 2,
                 "coupled layouts must fit the coupled kernel's launch bounds");
   for (fds_ctx *c : {ac, th}) {
     if (!c->line || c->strip_rows != cpt || c->line_cfg_threads != cfg_t ||

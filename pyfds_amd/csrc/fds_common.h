@@ -53,6 +53,23 @@ struct StepArgs {
   int32_t steps;        // resident 1D kernels: steps to run in this launch
   int32_t strip_h;      // row-streaming kernels: rows per strip
   int32_t ghost;        // row-streaming kernels: ghost columns / rows per side (fused steps - 1)
+  const void *lhead[3]; // resident 1D kernels: per-thread first boundary cell (LineEvHead)
+  const void *ohead[3]; // resident 1D kernels: per-thread first recorded point (LineOutHead)
+};
+
+// Resident 1D engine: each thread's first boundary cell of a component, precomputed on the
+// host so that a launch starts with one load instead of a chain through the event and
+// program lists.
+struct LineEvHead {
+  int32_t info;         // bits 0-7: register slot (0xff: none cached); bit 8: general path
+  int32_t stride;       // signal table stride per step
+  int64_t off;          // signal table index of step 0
+  double a;             // x = a*x + table[off + step*stride]
+};
+
+struct LineOutHead {
+  int32_t info;         // bits 0-7: register slot of the first point (0xff: none); bit 8: more
+  int32_t id;           // its output slot
 };
 
 // Map workgroup b to a tile so that each XCD (b % 8) gets one contiguous band of tiles:

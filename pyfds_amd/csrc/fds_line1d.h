@@ -45,12 +45,18 @@ constexpr int kCoupleMaxThreads[kNumLineCpt] = {1024, 512, 512, 256};
 constexpr int kNlLineMaxThreads[kNumLineCpt] = {512, 512, 512, 256};
 constexpr int kNlLineMaxCells = 2048;
 constexpr int kLineSteps = 2048;                   // time steps per launch at most
-// default layout: one workgroup up to kLineSegmentAbove cells; longer lines in segments of
-// 128 threads x 2 cells with 64 ghost cells per side (128 owned cells, 63 steps per launch).
-// Measured on Acoustic1D, 2000 cells: 0.41 us per step against 0.61 in one workgroup (one
-// CU's issue rate bounds a single workgroup; segments spread the line over 16 CUs).
+// default layout: one workgroup up to kLineSegmentAbove cells; longer lines in one-wave
+// segments (DPP neighbour exchange, no barriers) of 64 threads x 2 cells with 48 ghost cells
+// per side: 32 owned cells, 47 steps per launch.  Measured on Acoustic1D, 2000 cells: 0.32 us
+// per step, against 0.41 for 128-thread segments with LDS exchange and 0.61 for one
+// workgroup (one CU's issue rate bounds a single workgroup).  A step of a one-wave segment
+// costs ~0.27 us of dependent latency and a launch ~2.8 us, so more ghost cells (more steps
+// per launch) pay until the redundant ghost work matters; lines past kLineSegLongAbove cells,
+// which fill the GPU, use 4 cells per thread and 64 ghost cells (half the redundancy).
 constexpr int kLineSegmentAbove = 512;
-constexpr int kLineSegCpt = 2, kLineSegThreads = 128, kLineSegGhost = 64;
+constexpr int kLineSegCpt = 2, kLineSegThreads = 64, kLineSegGhost = 48;
+constexpr int kLineSegLongAbove = 16384;
+constexpr int kLineSegLongCpt = 4, kLineSegLongGhost = 64;
 
 enum LineFamily { kLineAcoustic = 0, kLineAcousticLossy = 1, kLineThermal = 2, kLineNonlinear = 3 };
 constexpr int kNumLineFamilies = 4;
@@ -58,6 +64,28 @@ constexpr int kNumLineFamilies = 4;
 // workgroup size limit of variant i for a family
 __host__ __device__ constexpr int line_max_threads(int fam, int i) {
   return fam == kLineNonlinear ? kNlLineMaxThreads[i] : kLineMaxThreads[i];
+}
+
+// One-wave workgroups (64 threads) exchange neighbours by DPP wave shifts instead of LDS, with
+// no barrier at all: lane i receives lane i-1's value (lane 0 gets zero) or lane i+1's value
+// (lane 63 gets zero).  Called with all 64 lanes active.
+__device__ __forceinline__ int dpp_from_left(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);   // wave_shr:1
+}
+__device__ __forceinline__ int dpp_from_right(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);   // wave_shl:1
+}
+__device__ __forceinline__ float wave_from_left(float v) {
+  return __int_as_float(dpp_from_left(__float_as_int(v)));
+}
+__device__ __forceinline__ float wave_from_right(float v) {
+  return __int_as_float(dpp_from_right(__float_as_int(v)));
+}
+__device__ __forceinline__ double wave_from_left(double v) {
+  return __hiloint2double(dpp_from_left(__double2hiint(v)), dpp_from_left(__double2loint(v)));
+}
+__device__ __forceinline__ double wave_from_right(double v) {
+  return __hiloint2double(dpp_from_right(__double2hiint(v)), dpp_from_right(__double2loint(v)));
 }
 
 // Index of this thread in the launch (event lists are CSR over all threads of all segments).
@@ -84,25 +112,14 @@ struct LineBound {
   __device__ __forceinline__ int slot() const { return info & 0xff; }
 
   template <typename T>
-  __device__ __forceinline__ void init(const EvView &ev, const ProgView &pg, int t,
+  __device__ __forceinline__ void init(const void *heads, const ProgView &pg, int t,
                                        int64_t step0) {
-    const int e0 = ev.ptr[t], e1 = ev.ptr[t + 1];
-    info = 0xff | (e1 - e0 > 1 ? 0x100 : 0);
-    idx = stride = 0;
-    a = val = 0.0;
-    if (e0 < e1) {
-      const int u = ev.id[e0];
-      const int q0 = pg.ptr[u];
-      if (pg.ptr[u + 1] - q0 == 1) {
-        info = (info & 0x100) | ev.pos[e0];
-        a = pg.a[q0];
-        stride = (int32_t)pg.stride[q0];
-        idx = (int32_t)(pg.off[q0] + step0 * pg.stride[q0]);
-        val = pg.table[idx];
-      } else {
-        info = 0x1ff;
-      }
-    }
+    const LineEvHead h = static_cast<const LineEvHead *>(heads)[t];
+    info = h.info;
+    stride = h.stride;
+    a = h.a;
+    idx = (int32_t)(h.off + step0 * (int64_t)h.stride);
+    val = (info & 0xff) != 0xff ? pg.table[idx] : 0.0;
   }
 
   // Apply this step's programs to the thread's cells (x, c); `more`: a next step follows in
@@ -144,10 +161,10 @@ struct LineBound {
 struct LineRecord {
   int info, id;      // info: bits 0-7 first point's register slot (0xff: none), bit 8: more
 
-  __device__ __forceinline__ void init(const OutView &o, int t) {
-    const int e0 = o.ev.ptr[t], e1 = o.ev.ptr[t + 1];
-    info = (e0 < e1 ? o.ev.pos[e0] : 0xff) | (e1 - e0 > 1 ? 0x100 : 0);
-    id = e0 < e1 ? o.ev.id[e0] : 0;
+  __device__ __forceinline__ void init(const void *heads, int t) {
+    const LineOutHead h = static_cast<const LineOutHead *>(heads)[t];
+    info = h.info;
+    id = h.id;
   }
 
   template <typename T, bool KAHAN, int CPT>
@@ -227,36 +244,40 @@ struct LineField {
       }
     }
     const int g = line_thread();
-    b0.init<T>(a.bev[0], a.prog[0], g, step0);
-    b1.init<T>(a.bev[1], a.prog[1], g, step0);
-    r0.init(a.outp[0], g);
-    r1.init(a.outp[1], g);
+    b0.init<T>(a.lhead[0], a.prog[0], g, step0);
+    b1.init<T>(a.lhead[1], a.prog[1], g, step0);
+    r0.init(a.ohead[0], g);
+    r1.init(a.ohead[1], g);
   }
 
   __device__ __forceinline__ bool ok(int m) const { return (valid >> m) & 1; }
 
-  // first component: boundaries, record, publish the block's last cell
+  // first component: boundaries, record, publish the block's last cell (WAVE: neighbours come
+  // by DPP in the next part, nothing to publish)
+  template <bool WAVE>
   __device__ __forceinline__ void part1(const StepArgs<T> &a, int t, int64_t step, int64_t row,
                                         bool more, const LineXchg<T> &x) {
     b0.apply<T, KAHAN, CPT>(a.bev[0], a.prog[0], step, more, x0, c0);
     r0.write<T, KAHAN, CPT>(a.outp[0], row, x0, c0);
-    x.a[t] = x0[CPT - 1];
-    if constexpr (LOSSY) {
-      x.b[t] = x1[0];
-      x.c[t] = x1[CPT - 1];
+    if constexpr (!WAVE) {
+      x.a[t] = x0[CPT - 1];
+      if constexpr (LOSSY) {
+        x.b[t] = x1[0];
+        x.c[t] = x1[CPT - 1];
+      }
     }
   }
 
   // second component from the first (velocity update / heat flux assignment), its
   // boundaries and record; publish its first cell (and, with LAST, its last cell)
-  template <bool LAST = false>
+  template <bool WAVE, bool LAST = false>
   __device__ __forceinline__ void part2(const StepArgs<T> &a, int t, int nt, int64_t step,
                                         int64_t row, bool more, const LineXchg<T> &x) {
-    const T pm = t > 0 ? x.a[t - 1] : T(0);
+    const T pm = WAVE ? wave_from_left(x0[CPT - 1]) : (t > 0 ? x.a[t - 1] : T(0));
     T vo[LOSSY ? CPT + 2 : 1];
     if constexpr (LOSSY) {
-      vo[0] = t > 0 ? x.c[t - 1] : T(0);
-      vo[CPT + 1] = t + 1 < nt ? x.b[t + 1] : T(0);
+      vo[0] = WAVE ? wave_from_left(x1[CPT - 1]) : (t > 0 ? x.c[t - 1] : T(0));
+      vo[CPT + 1] = WAVE ? wave_from_right(x1[0]) : (t + 1 < nt ? x.b[t + 1] : T(0));
 #pragma unroll
       for (int m = 0; m < CPT; ++m) vo[m + 1] = x1[m];
     }
@@ -283,13 +304,16 @@ struct LineField {
     }
     b1.apply<T, KV, CPT>(a.bev[1], a.prog[1], step, more, x1, c1);
     r1.write<T, KV, CPT>(a.outp[1], row, x1, c1);
-    x.d[t] = x1[0];
-    if constexpr (LAST) x.e[t] = x1[CPT - 1];
+    if constexpr (!WAVE) {
+      x.d[t] = x1[0];
+      if constexpr (LAST) x.e[t] = x1[CPT - 1];
+    }
   }
 
   // first component from the second
+  template <bool WAVE>
   __device__ __forceinline__ void part3(int t, int nt, const LineXchg<T> &x) {
-    const T vn = t + 1 < nt ? x.d[t + 1] : T(0);
+    const T vn = WAVE ? wave_from_right(x1[0]) : (t + 1 < nt ? x.d[t + 1] : T(0));
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
       const T xp = m + 1 < CPT ? x1[m + 1] : vn;
@@ -315,12 +339,14 @@ struct LineField {
 };
 
 // One field for `a.steps` steps starting at step a.dyn[0] + a.k (output row a.dyn[1] + a.k).
-// Reads state from a.in / a.cin, writes a.out / a.cout once at the end.
-template <typename T, bool KAHAN, int FAM, int CPT>
+// Reads state from a.in / a.cin, writes a.out / a.cout once at the end.  WAVE: one-wave
+// workgroups, neighbours by DPP, no barriers.
+template <typename T, bool KAHAN, int FAM, int CPT, bool WAVE>
 __device__ void line1d(const StepArgs<T> &a) {
   constexpr bool LOSSY = FAM == kLineAcousticLossy;
-  __shared__ T xa[kLineThreads], xb[LOSSY ? kLineThreads : 1], xc[LOSSY ? kLineThreads : 1];
-  __shared__ T xd[kLineThreads];
+  constexpr int NX = WAVE ? 1 : kLineThreads;
+  __shared__ T xa[NX], xb[LOSSY ? NX : 1], xc[LOSSY ? NX : 1];
+  __shared__ T xd[NX];
   const LineXchg<T> x{xa, xb, xc, xd, nullptr};
   const int t = threadIdx.x, nt = blockDim.x;
   const int64_t step0 = a.dyn[0] + a.k, row0 = a.dyn[1] + a.k;
@@ -329,11 +355,11 @@ __device__ void line1d(const StepArgs<T> &a) {
   for (int s = 0; s < a.steps; ++s) {
     const int64_t step = step0 + s, row = row0 + s;
     const bool more = s + 1 < a.steps;
-    f.part1(a, t, step, row, more, x);
-    __syncthreads();
-    f.part2(a, t, nt, step, row, more, x);
-    __syncthreads();
-    f.part3(t, nt, x);
+    f.template part1<WAVE>(a, t, step, row, more, x);
+    if (!WAVE) __syncthreads();
+    f.template part2<WAVE>(a, t, nt, step, row, more, x);
+    if (!WAVE) __syncthreads();
+    f.template part3<WAVE>(t, nt, x);
   }
   f.store(a);
 }
@@ -349,9 +375,10 @@ __device__ void line1d(const StepArgs<T> &a) {
 // terms are always summed (zero factors outside), like the tile kernel.  Two barriers per
 // step: p's last cell and v's first and last cells before the first, the mass flux's first
 // cell before the second.
-template <typename T, bool KAHAN, int CPT>
+template <typename T, bool KAHAN, int CPT, bool WAVE>
 __device__ void line1d_nonlinear(const StepArgs<T> &a) {
-  __shared__ T xp[kLineThreads], xv0[kLineThreads], xv1[kLineThreads], xw[kLineThreads];
+  constexpr int NX = WAVE ? 1 : kLineThreads;
+  __shared__ T xp[NX], xv0[NX], xv1[NX], xw[NX];
   const int t = threadIdx.x, nt = blockDim.x;
   const int n = (int)a.n, j0 = line_base(a) + t * CPT;
   const T f = (T)a.s[FDS_S_DTDX], h = (T)a.s[FDS_S_DTDX_4];
@@ -386,31 +413,33 @@ __device__ void line1d_nonlinear(const StepArgs<T> &a) {
   const int64_t step0 = a.dyn[0] + a.k, row0 = a.dyn[1] + a.k;
   LineBound b0, b1, b2;
   const int g = line_thread();
-  b0.init<T>(a.bev[0], a.prog[0], g, step0);
-  b1.init<T>(a.bev[1], a.prog[1], g, step0);
-  b2.init<T>(a.bev[2], a.prog[2], g, step0);
+  b0.init<T>(a.lhead[0], a.prog[0], g, step0);
+  b1.init<T>(a.lhead[1], a.prog[1], g, step0);
+  b2.init<T>(a.lhead[2], a.prog[2], g, step0);
   LineRecord o0, o1, o2;
-  o0.init(a.outp[0], g);
-  o1.init(a.outp[1], g);
-  o2.init(a.outp[2], g);
+  o0.init(a.ohead[0], g);
+  o1.init(a.ohead[1], g);
+  o2.init(a.ohead[2], g);
 
   for (int s = 0; s < a.steps; ++s) {
     const int64_t step = step0 + s, row = row0 + s;
     const bool more = s + 1 < a.steps;
     b0.apply<T, false, CPT>(a.bev[0], a.prog[0], step, more, p, cp0);
     o0.write<T, false, CPT>(a.outp[0], row, p, cp0);
-    xp[t] = p[CPT - 1];
-    xv0[t] = v[0];
-    xv1[t] = v[CPT - 1];
-    __syncthreads();
+    if constexpr (!WAVE) {
+      xp[t] = p[CPT - 1];
+      xv0[t] = v[0];
+      xv1[t] = v[CPT - 1];
+      __syncthreads();
+    }
 
     // velocity
     T vo[CPT + 2];
-    vo[0] = t > 0 ? xv1[t - 1] : T(0);
-    vo[CPT + 1] = t + 1 < nt ? xv0[t + 1] : T(0);
+    vo[0] = WAVE ? wave_from_left(v[CPT - 1]) : (t > 0 ? xv1[t - 1] : T(0));
+    vo[CPT + 1] = WAVE ? wave_from_right(v[0]) : (t + 1 < nt ? xv0[t + 1] : T(0));
 #pragma unroll
     for (int m = 0; m < CPT; ++m) vo[m + 1] = v[m];
-    const T pm = t > 0 ? xp[t - 1] : T(0);
+    const T pm = WAVE ? wave_from_left(p[CPT - 1]) : (t > 0 ? xp[t - 1] : T(0));
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
       const T pa = (-f) * (m > 0 ? p[m - 1] : pm);
@@ -444,9 +473,14 @@ __device__ void line1d_nonlinear(const StepArgs<T> &a) {
       const T sr = r[m] + r0[m];
       w[m] = ((valid >> m) & 1) ? sr * v[m] : T(0);
     }
-    xw[t] = w[0];
-    __syncthreads();
-    const T wn = t + 1 < nt ? xw[t + 1] : T(0);
+    T wn;
+    if constexpr (WAVE) {
+      wn = wave_from_right(w[0]);
+    } else {
+      xw[t] = w[0];
+      __syncthreads();
+      wn = t + 1 < nt ? xw[t + 1] : T(0);
+    }
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
       const T ta = (-f) * w[m];
@@ -551,11 +585,11 @@ __device__ void line1d_coupled(const StepArgs<T> &ac, const StepArgs<T> &th,
   for (int s = 0; s < ac.steps; ++s) {
     const int64_t step = step0 + s, row = row0 + s;
     const bool more = s + 1 < ac.steps;
-    A.part1(ac, t, step, row, more, xA);
+    A.template part1<false>(ac, t, step, row, more, xA);
     __syncthreads();
-    A.template part2<true>(ac, t, nt, step, row, more, xA);
+    A.template part2<false, true>(ac, t, nt, step, row, more, xA);
     __syncthreads();
-    A.part3(t, nt, xA);
+    A.template part3<false>(t, nt, xA);
     // loss transfer from the step's final velocity
     T tr[CPT];
     {
@@ -570,11 +604,11 @@ __device__ void line1d_coupled(const StepArgs<T> &ac, const StepArgs<T> &th,
         tr[m] = ld * cp.dt;
       }
     }
-    H.part1(th, t, step, row, more, xT);
+    H.template part1<false>(th, t, step, row, more, xT);
     __syncthreads();
-    H.part2(th, t, nt, step, row, more, xT);
+    H.template part2<false>(th, t, nt, step, row, more, xT);
     __syncthreads();
-    H.part3(t, nt, xT);
+    H.template part3<false>(t, nt, xT);
     const bool deliver = step % cp.stepping == 0;
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {

@@ -255,7 +255,7 @@ def test_nonlinear_engine_choice():
 
 # Segmented lines: several workgroups, each stepping its owned cells plus `ghost` cells of each
 # neighbour for at most `ghost` steps per launch.  (cells per thread, threads, ghost cells)
-SEG_CFGS = [(1, 128, 16), (2, 256, 64), (4, 128, 100), (8, 64, 7)]
+SEG_CFGS = [(1, 128, 16), (2, 256, 64), (4, 128, 100), (8, 64, 7), (2, 64, 48), (4, 64, 64)]
 SEG_SCENES = {
     'a1d_2000': SCENES['a1d_2000'],
     'a1d_lossy_900': SCENES['a1d_lossy_900'],
@@ -283,7 +283,9 @@ def test_segmented_line_fp64_matches_oracle_bitwise(name, cfg):
 def test_segmented_line_fp32_forms_equal_one_segment(name, form):
     make = SEG_SCENES.get(name) or NL_SCENES[name]
     one, _ = run(make(), form=form)
-    seg, eng = run(make(), form=form, line_shape=(2, 128, 40))
-    assert eng == 'line'
-    for k in one:
-        np.testing.assert_array_equal(seg[k], one[k], err_msg='%s %s %s' % (name, form, k))
+    for shape in ((2, 128, 40), (4, 64, 40)):          # LDS exchange, one-wave DPP exchange
+        seg, eng = run(make(), form=form, line_shape=shape)
+        assert eng == 'line'
+        for k in one:
+            np.testing.assert_array_equal(seg[k], one[k],
+                                          err_msg='%s %s %s %s' % (name, form, shape, k))
