@@ -99,34 +99,37 @@ __device__ __forceinline__ int line_base(const StepArgs<T> &a) {
 
 // A thread's boundary cells of one component.  Lists are CSR over threads: entries
 // ev.ptr[t]..ev.ptr[t+1]-1, ev.pos = register slot, ev.id = unique boundary cell.  The first
-// cell, when its program has a single entry (the usual case), is cached in registers with its
-// signal value fetched one step ahead; any other cell of the thread takes the general path,
-// which re-reads the thread's list.  Kept to seven registers: two fields' worth live in the
-// coupled kernel.
+// cell, when its program has a single entry (the usual case), is cached in registers; a
+// constant value (stride 0) is loaded once, a signal's values two steps ahead (a step of a
+// segment is shorter than a cache round trip).  Any other cell of the thread takes the
+// general path, which re-reads the thread's list.
 struct LineBound {
   int info;          // bits 0-7: cached cell's register slot (0xff: none); bit 8: general path
-  int32_t idx;       // table index of the value for the next step to apply
+  int32_t idx;       // table index of the value of the step after the next to apply
   int32_t stride;
-  double a, val;     // x = a*x + val
+  double a, val, val2;   // x = a*x + val this step; val2 the next step's value
 
   __device__ __forceinline__ int slot() const { return info & 0xff; }
 
+  // steps: steps of this launch (the signal table covers them all, and no more)
   template <typename T>
   __device__ __forceinline__ void init(const void *heads, const ProgView &pg, int t,
-                                       int64_t step0) {
+                                       int64_t step0, int steps) {
     const LineEvHead h = static_cast<const LineEvHead *>(heads)[t];
     info = h.info;
     stride = h.stride;
     a = h.a;
     idx = (int32_t)(h.off + step0 * (int64_t)h.stride);
-    val = (info & 0xff) != 0xff ? pg.table[idx] : 0.0;
+    const bool on = (info & 0xff) != 0xff;
+    val = on ? pg.table[idx] : 0.0;
+    val2 = (on && stride != 0 && steps > 1) ? pg.table[idx + stride] : val;
   }
 
-  // Apply this step's programs to the thread's cells (x, c); `more`: a next step follows in
-  // this launch, so its value may be fetched.
+  // Apply this step's programs to the thread's cells (x, c); `rem`: steps of this launch
+  // after this one (values up to that step may be fetched).
   template <typename T, bool KAHAN, int CPT>
   __device__ __forceinline__ void apply(const EvView &ev, const ProgView &pg, int64_t step,
-                                        bool more, T (&x)[CPT], T (&c)[CPT]) {
+                                        int rem, T (&x)[CPT], T (&c)[CPT]) {
     const int sl = slot();
     if (sl != 0xff) {
       // the slot is tested against each register position (compile-time index) so that the
@@ -140,8 +143,11 @@ struct LineBound {
         x[m] = (T)v;
         if (KAHAN) c[m] = (T)((double)x[m] - v);
       }
-      idx += stride;
-      if (more) val = pg.table[idx];
+      if (stride != 0) {
+        idx += stride;                     // now the next step's index
+        val = val2;
+        if (rem >= 2) val2 = pg.table[idx + stride];
+      }
     }
     if (info & 0x100) {
       const int t = line_thread();
@@ -244,8 +250,8 @@ struct LineField {
       }
     }
     const int g = line_thread();
-    b0.init<T>(a.lhead[0], a.prog[0], g, step0);
-    b1.init<T>(a.lhead[1], a.prog[1], g, step0);
+    b0.init<T>(a.lhead[0], a.prog[0], g, step0, a.steps);
+    b1.init<T>(a.lhead[1], a.prog[1], g, step0, a.steps);
     r0.init(a.ohead[0], g);
     r1.init(a.ohead[1], g);
   }
@@ -256,8 +262,8 @@ struct LineField {
   // by DPP in the next part, nothing to publish)
   template <bool WAVE>
   __device__ __forceinline__ void part1(const StepArgs<T> &a, int t, int64_t step, int64_t row,
-                                        bool more, const LineXchg<T> &x) {
-    b0.apply<T, KAHAN, CPT>(a.bev[0], a.prog[0], step, more, x0, c0);
+                                        int rem, const LineXchg<T> &x) {
+    b0.apply<T, KAHAN, CPT>(a.bev[0], a.prog[0], step, rem, x0, c0);
     r0.write<T, KAHAN, CPT>(a.outp[0], row, x0, c0);
     if constexpr (!WAVE) {
       x.a[t] = x0[CPT - 1];
@@ -272,7 +278,7 @@ struct LineField {
   // boundaries and record; publish its first cell (and, with LAST, its last cell)
   template <bool WAVE, bool LAST = false>
   __device__ __forceinline__ void part2(const StepArgs<T> &a, int t, int nt, int64_t step,
-                                        int64_t row, bool more, const LineXchg<T> &x) {
+                                        int64_t row, int rem, const LineXchg<T> &x) {
     const T pm = WAVE ? wave_from_left(x0[CPT - 1]) : (t > 0 ? x.a[t - 1] : T(0));
     T vo[LOSSY ? CPT + 2 : 1];
     if constexpr (LOSSY) {
@@ -302,7 +308,7 @@ struct LineField {
         }
       }
     }
-    b1.apply<T, KV, CPT>(a.bev[1], a.prog[1], step, more, x1, c1);
+    b1.apply<T, KV, CPT>(a.bev[1], a.prog[1], step, rem, x1, c1);
     r1.write<T, KV, CPT>(a.outp[1], row, x1, c1);
     if constexpr (!WAVE) {
       x.d[t] = x1[0];
@@ -354,10 +360,10 @@ __device__ void line1d(const StepArgs<T> &a) {
   f.load(a, t, step0);
   for (int s = 0; s < a.steps; ++s) {
     const int64_t step = step0 + s, row = row0 + s;
-    const bool more = s + 1 < a.steps;
-    f.template part1<WAVE>(a, t, step, row, more, x);
+    const int rem = a.steps - s - 1;
+    f.template part1<WAVE>(a, t, step, row, rem, x);
     if (!WAVE) __syncthreads();
-    f.template part2<WAVE>(a, t, nt, step, row, more, x);
+    f.template part2<WAVE>(a, t, nt, step, row, rem, x);
     if (!WAVE) __syncthreads();
     f.template part3<WAVE>(t, nt, x);
   }
@@ -413,9 +419,9 @@ __device__ void line1d_nonlinear(const StepArgs<T> &a) {
   const int64_t step0 = a.dyn[0] + a.k, row0 = a.dyn[1] + a.k;
   LineBound b0, b1, b2;
   const int g = line_thread();
-  b0.init<T>(a.lhead[0], a.prog[0], g, step0);
-  b1.init<T>(a.lhead[1], a.prog[1], g, step0);
-  b2.init<T>(a.lhead[2], a.prog[2], g, step0);
+  b0.init<T>(a.lhead[0], a.prog[0], g, step0, a.steps);
+  b1.init<T>(a.lhead[1], a.prog[1], g, step0, a.steps);
+  b2.init<T>(a.lhead[2], a.prog[2], g, step0, a.steps);
   LineRecord o0, o1, o2;
   o0.init(a.ohead[0], g);
   o1.init(a.ohead[1], g);
@@ -423,8 +429,8 @@ __device__ void line1d_nonlinear(const StepArgs<T> &a) {
 
   for (int s = 0; s < a.steps; ++s) {
     const int64_t step = step0 + s, row = row0 + s;
-    const bool more = s + 1 < a.steps;
-    b0.apply<T, false, CPT>(a.bev[0], a.prog[0], step, more, p, cp0);
+    const int rem = a.steps - s - 1;
+    b0.apply<T, false, CPT>(a.bev[0], a.prog[0], step, rem, p, cp0);
     o0.write<T, false, CPT>(a.outp[0], row, p, cp0);
     if constexpr (!WAVE) {
       xp[t] = p[CPT - 1];
@@ -463,7 +469,7 @@ __device__ void line1d_nonlinear(const StepArgs<T> &a) {
         if (KAHAN) kahan_sub(v[m], cv[m], inc); else v[m] = v[m] - inc;
       }
     }
-    b1.apply<T, KAHAN, CPT>(a.bev[1], a.prog[1], step, more, v, cv);
+    b1.apply<T, KAHAN, CPT>(a.bev[1], a.prog[1], step, rem, v, cv);
     o1.write<T, KAHAN, CPT>(a.outp[1], row, v, cv);
 
     // mass flux (rho' + rho0) v, then density
@@ -490,7 +496,7 @@ __device__ void line1d_nonlinear(const StepArgs<T> &a) {
         if (KAHAN) kahan_sub(r[m], cr[m], d); else r[m] = r[m] - d;
       }
     }
-    b2.apply<T, KAHAN, CPT>(a.bev[2], a.prog[2], step, more, r, cr);
+    b2.apply<T, KAHAN, CPT>(a.bev[2], a.prog[2], step, rem, r, cr);
     o2.write<T, KAHAN, CPT>(a.outp[2], row, r, cr);
 
     // equation of state
@@ -554,12 +560,13 @@ struct LineCouple {
 // (then the accumulator restarts from zero).  Four barriers per step.  In segments the
 // transfer reads the velocity's left neighbour, so wrong values reach one cell further into
 // the temperature: a launch runs at most ghost - 2 steps.
-template <typename T, bool KAHAN, int AFAM, int CPT>
+template <typename T, bool KAHAN, int AFAM, int CPT, bool WAVE>
 __device__ void line1d_coupled(const StepArgs<T> &ac, const StepArgs<T> &th,
                                const LineCouple<T> &cp) {
   constexpr bool LOSSY = AFAM == kLineAcousticLossy;
-  __shared__ T xa[kLineThreads], xb[LOSSY ? kLineThreads : 1], xc[LOSSY ? kLineThreads : 1];
-  __shared__ T xd[kLineThreads], xe[kLineThreads], ya[kLineThreads], yd[kLineThreads];
+  constexpr int NX = WAVE ? 1 : kLineThreads;
+  __shared__ T xa[NX], xb[LOSSY ? NX : 1], xc[LOSSY ? NX : 1];
+  __shared__ T xd[NX], xe[NX], ya[NX], yd[NX];
   const LineXchg<T> xA{xa, xb, xc, xd, xe}, xT{ya, nullptr, nullptr, yd, nullptr};
   const int t = threadIdx.x, nt = blockDim.x;
   const int64_t step0 = ac.dyn[0] + ac.k, row0 = ac.dyn[1] + ac.k;
@@ -584,16 +591,16 @@ __device__ void line1d_coupled(const StepArgs<T> &ac, const StepArgs<T> &th,
   }
   for (int s = 0; s < ac.steps; ++s) {
     const int64_t step = step0 + s, row = row0 + s;
-    const bool more = s + 1 < ac.steps;
-    A.template part1<false>(ac, t, step, row, more, xA);
-    __syncthreads();
-    A.template part2<false, true>(ac, t, nt, step, row, more, xA);
-    __syncthreads();
-    A.template part3<false>(t, nt, xA);
+    const int rem = ac.steps - s - 1;
+    A.template part1<WAVE>(ac, t, step, row, rem, xA);
+    if (!WAVE) __syncthreads();
+    A.template part2<WAVE, true>(ac, t, nt, step, row, rem, xA);
+    if (!WAVE) __syncthreads();
+    A.template part3<WAVE>(t, nt, xA);
     // loss transfer from the step's final velocity
     T tr[CPT];
     {
-      const T vm = t > 0 ? xA.e[t - 1] : T(0);
+      const T vm = WAVE ? wave_from_left(A.x1[CPT - 1]) : (t > 0 ? xA.e[t - 1] : T(0));
 #pragma unroll
       for (int m = 0; m < CPT; ++m) {
         const T wm = (m > 0 ? A.x1[m - 1] : vm) * rho[m];
@@ -604,11 +611,11 @@ __device__ void line1d_coupled(const StepArgs<T> &ac, const StepArgs<T> &th,
         tr[m] = ld * cp.dt;
       }
     }
-    H.template part1<false>(th, t, step, row, more, xT);
-    __syncthreads();
-    H.template part2<false>(th, t, nt, step, row, more, xT);
-    __syncthreads();
-    H.template part3<false>(t, nt, xT);
+    H.template part1<WAVE>(th, t, step, row, rem, xT);
+    if (!WAVE) __syncthreads();
+    H.template part2<WAVE>(th, t, nt, step, row, rem, xT);
+    if (!WAVE) __syncthreads();
+    H.template part3<WAVE>(t, nt, xT);
     const bool deliver = step % cp.stepping == 0;
 #pragma unroll
     for (int m = 0; m < CPT; ++m) {
