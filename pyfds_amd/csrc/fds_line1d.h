@@ -1,16 +1,18 @@
-// Resident 1D engine: one workgroup holds a whole 1D line (Acoustic1D, Thermal1D) in registers
-// and advances it up to kLineSteps time steps per launch.
+// Resident 1D engine: workgroups hold a 1D line (Acoustic1D, Thermal1D, IdealGas1D,
+// Acoustic2ndOrder1D, and the ThermoAcoustic1D pair) in registers and advance it many time
+// steps per launch.
 //
 // Thread t owns the CPT consecutive cells [t*CPT, t*CPT + CPT) of its workgroup's segment and
 // keeps their state, Kahan compensation and coefficients in registers for the whole launch.
-// One workgroup may hold the whole line (segment = line), or W workgroups split it: workgroup
-// w owns cells [w*L, w*L + L) (L = a.strip_h) and also steps G = a.ghost ghost cells on each
-// side, whose values go wrong from the segment's outer edges inwards by one cell per step (two
-// in the first step on the right where a component reads its right neighbour's old value), so
-// a launch runs at most G - 1 steps and stores only the owned cells (temporal blocking).  A step needs one
-// neighbour value from each side of a thread's block; those pass through small LDS arrays
-// behind one barrier each, so a step costs two barriers and touches global memory only for
-// boundary signal values (fetched one step ahead) and recorded outputs.
+// One workgroup may hold the whole line (segment = line, up to kLineSteps steps per launch),
+// or W workgroups split it: workgroup w owns cells [w*L, w*L + L) (L = a.strip_h) and also
+// steps G = a.ghost ghost cells on each side, whose values go wrong from the segment's outer
+// edges inwards by one cell per step (two in the first step on the right where a component
+// reads its right neighbour's old value), so a launch runs at most G - 1 steps and stores only
+// the owned cells (temporal blocking).  A step needs one neighbour value from each side of a
+// thread's block: one-wave workgroups move them by DPP lane shifts with no barrier, larger
+// ones through small LDS arrays behind one barrier each.  Global memory is touched only for
+// boundary signal values (fetched ahead) and recorded outputs.
 //
 // Acoustic1D step (pyfds/acoustics.py:40-52), in the reference's FP64 order:
 //   p boundaries, record p
@@ -38,17 +40,15 @@ constexpr int kLineCpt[kNumLineCpt] = {1, 2, 4, 8};   // cells per thread of the
 // workgroup size limit of each variant: the 4- and 8-cell variants need more than the 128
 // VGPRs a 1024-thread workgroup leaves each lane
 constexpr int kLineMaxThreads[kNumLineCpt] = {1024, 1024, 512, 512};
-constexpr int kLineMaxCells = 4096;
 // the coupled kernel holds two fields: half the workgroup from 2 cells per thread on
 constexpr int kCoupleMaxThreads[kNumLineCpt] = {1024, 512, 512, 256};
 // the nonlinear models hold three components and call pow / expm1 / log1p
 constexpr int kNlLineMaxThreads[kNumLineCpt] = {512, 512, 512, 256};
-constexpr int kNlLineMaxCells = 2048;
 constexpr int kLineSteps = 2048;                   // time steps per launch at most
 // default layout: one workgroup up to kLineSegmentAbove cells; longer lines in one-wave
 // segments (DPP neighbour exchange, no barriers) of 64 threads x 2 cells with 48 ghost cells
-// per side: 32 owned cells, 47 steps per launch.  Measured on Acoustic1D, 2000 cells: 0.32 us
-// per step, against 0.41 for 128-thread segments with LDS exchange and 0.61 for one
+// per side: 32 owned cells, 47 steps per launch.  Measured on Acoustic1D, 2000 cells: 0.30 us
+// per step, against 0.42 for 128-thread segments with LDS exchange and 0.64 for one
 // workgroup (one CU's issue rate bounds a single workgroup).  A step of a one-wave segment
 // costs ~0.27 us of dependent latency and a launch ~2.8 us, so more ghost cells (more steps
 // per launch) pay until the redundant ghost work matters; lines past kLineSegLongAbove cells,
@@ -59,7 +59,6 @@ constexpr int kLineSegLongAbove = 16384;
 constexpr int kLineSegLongCpt = 4, kLineSegLongGhost = 64;
 
 enum LineFamily { kLineAcoustic = 0, kLineAcousticLossy = 1, kLineThermal = 2, kLineNonlinear = 3 };
-constexpr int kNumLineFamilies = 4;
 
 // workgroup size limit of variant i for a family
 __host__ __device__ constexpr int line_max_threads(int fam, int i) {
