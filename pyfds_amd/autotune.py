@@ -50,7 +50,7 @@ def _key(fields):
     lossy = any(getattr(x, '_coefs', {}).get(4) is not None for x in fields)
     cells = sum(x.num_points for x in fields)
     return '|'.join(str(v) for v in (_device_name(), type(f).__name__, f.form, len(fields),
-                                     cells, f._device_grid(), lossy, 'v2'))
+                                     cells, f._device_grid(), lossy, 'v3'))
 
 
 def _snapshot(fields):
@@ -109,7 +109,9 @@ def _tune(fields, steps, chunks, use_cache, batch):
         if (ty == 1) != one_row:
             continue
         if engines[i] == 'stream':
-            if lossy or one_row:
+            # 1D models run on the resident engine; batches stream lossless members only
+            # (a lossy field runs any streaming shape one step per launch)
+            if one_row or (lossy and batch):
                 continue
             cand.append((i, 0))
             if ty == 16:

@@ -139,7 +139,8 @@ struct fds_ctx {
   int strip_rows = 0;     // streaming strip height override (0: from the shape and the grid)
   int num_cu = 256;       // compute units of the device
   int fuse = 1;           // time steps per launch of the streaming shape
-  int ghost = 0;          // ghost columns / rows per side of a streaming tile (fuse - 1)
+  int ghost = 0;          // ghost columns per side of a streaming tile (fuse - 1; lossy: 1)
+  int ghost_rows = 0;     // ghost rows per side of a streaming strip (fuse - 1; lossy: 0)
   int sx = 0;             // x stride of the tiles (tx - 2*ghost)
   bool line = false;      // resident 1D engine in use (one workgroup, many steps per launch)
   int line_var = 0;       // its variant: index into fds::kLineCpt
@@ -214,7 +215,7 @@ Region region_of(const fds_ctx *c, int k) {
     return Region{0, 0, 0, tx - 1, tx};
   }
   // rows of all fused stages: G ghost rows above and below the strip
-  const int G = c->ghost;
+  const int G = c->ghost_rows;
   if (k == 0) return Region{-G - 1, ty + G, -1, tx, tx + 2};
   if (k == 1) return Region{-G, ty - 1 + G, 0, tx, tx + 1};
   return Region{-G, ty + G, 0, tx - 1, tx};
@@ -294,10 +295,13 @@ void shape_of(int t, int *tx, int *ty) {
   }
 }
 
-// The streaming engine covers the lossless 2D models; everything else runs on tiles.
+// The streaming engine covers the 2D models (lossy acoustics one step per launch); 1D models
+// run on the resident engine or tiles.  Batches stream lossless members only.
 bool can_stream(const fds_ctx *c) {
-  return c->family != kNonlinear && c->ny > 1 && !c->lossy;
+  return c->family != kNonlinear && c->ny > 1;
 }
+
+bool can_stream_batch(const fds_ctx *c) { return can_stream(c) && !c->lossy; }
 
 // Resident 1D engine variant for n cells: `cpt` cells per thread if given, else the fewest
 // cells per thread that fit (the most threads: a step is bound by the one CU's issue rate, so
@@ -317,6 +321,7 @@ int choose_tiles(fds_ctx *c) {
   c->line = false;
   c->fuse = 1;
   c->ghost = 0;
+  c->ghost_rows = 0;
   if ((c->one_d || c->family == kNonlinear) && c->tile < 0) {
     // segmented lines take any length; one segment must hold the whole line.  Automatic
     // layout: one segment up to kLineSegmentAbove cells, else segments of the default shape
@@ -374,7 +379,7 @@ int choose_tiles(fds_ctx *c) {
     c->shape = 0;
   } else {
     int t = c->tile;
-    if (t < 0) t = (c->ny == 1) ? fds::kTileRow : (c->lossy ? fds::kDefaultTile2D
+    if (t < 0) t = (c->ny == 1) ? fds::kTileRow : (c->lossy ? fds::kDefaultLossy2D
                                                              : fds::kDefaultLossless2D);
     if (t >= fds::kNumTiles && !can_stream(c))
       t = (c->ny == 1) ? fds::kTileRow : fds::kDefaultTile2D;
@@ -382,8 +387,11 @@ int choose_tiles(fds_ctx *c) {
     c->streaming = t >= fds::kNumTiles;
     shape_of(t, &c->tx, &c->ty);
     if (c->streaming) {
-      c->fuse = fds::kStreamShapes[t - fds::kNumTiles][2];
-      c->ghost = c->fuse - 1;
+      // lossy acoustics: one step per launch, one ghost column per side, no ghost rows
+      const bool lossy = c->lossy && c->family == kAcoustic;
+      c->fuse = lossy ? 1 : fds::kStreamShapes[t - fds::kNumTiles][2];
+      c->ghost = lossy ? 1 : c->fuse - 1;
+      c->ghost_rows = lossy ? 0 : c->ghost;
       if (c->strip_rows > 0) {
         c->ty = c->strip_rows;
       } else {
@@ -569,6 +577,11 @@ void fill_args(const fds_ctx *c, fds::StepArgs<T> &a, int parity) {
 
 fds::LaunchFn pick_launch(const fds_ctx *c, bool fused) {
   const int t = c->shape;
+  if (c->streaming && c->lossy) {
+    // variant: axial-symmetric, isotropic (B, K and M y slots alias the x ones), plain
+    const int v = c->axi ? 1 : (c->iso_b && c->iso_k && !c->coef_set[FDS_C_MY]) ? 2 : 0;
+    return c->ks->stream_lossy[t - fds::kNumTiles][v];
+  }
   if (c->streaming) {
     const int s = t - fds::kNumTiles, f = fused ? 1 : 0;
     // variant: axial-symmetric, isotropic (y coefficient slots alias the x ones), plain
@@ -1526,7 +1539,7 @@ int fds_batch_create(fds_batch **out, fds_ctx *const *ctxs, int n, int tile) {
   // one tile shape for every member (re-derives each member's events); the streaming
   // engine only when it covers every member
   bool all_stream = true;
-  for (auto *c : b->ctxs) all_stream = all_stream && can_stream(c);
+  for (auto *c : b->ctxs) all_stream = all_stream && can_stream_batch(c);
   int t = tile < 0 ? (all_stream ? fds::kDefaultLossless2D : fds::kDefaultTile2D) : tile;
   if (t >= fds::kNumTiles + fds::kNumStream || t < 0) {
     fds_batch_destroy(b);

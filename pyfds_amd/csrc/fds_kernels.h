@@ -12,9 +12,10 @@ constexpr int kTileShapes[kNumTiles][2] = {
     {64, 16}, {64, 8}, {128, 8}, {256, 4}, {128, 4}, {256, 1}};
 constexpr int kDefaultTile2D = 1;       // default of the tile engine (lossy models)
 constexpr int kDefaultLossless2D = kNumTiles + 8;   // lossless 2D: streaming 64 x 16, 2 steps
+constexpr int kDefaultLossy2D = kNumTiles + 7;      // lossy 2D: streaming 64 x 16, one step
 constexpr int kTileRow = 5;
 
-// Row-streaming engine (lossless 2D models): TX threads = columns, strips of TY rows, S time
+// Row-streaming engine (2D models; lossy acoustics one step per launch): TX threads = columns, strips of TY rows, S time
 // steps fused per launch, NT = non-temporal state reads and writes; TX = 64 runs one wave per
 // workgroup and exchanges neighbours by lane shuffles.  Shape indices continue the tile table:
 // index kNumTiles + i is kStreamShapes[i] = {TX, TY, S, NT}.
@@ -53,6 +54,7 @@ struct KernelSet {
   LaunchFn stream_acoustic[kNumStream][3][2];   // [shape][plain, axi, isotropic][fused]
   LaunchFn stream_thermal[kNumStream][3][2];
   BatchFn stream_batch[kNumStream][2];          // [shape][fused]
+  LaunchFn stream_lossy[kNumStream][3];         // lossy acoustics, one step: [shape][variant]
   // resident 1D engine: [cells per thread][acoustic, lossy acoustic, thermal, nonlinear]
   LineFn line[4][4];
   // ThermoAcoustic1D on the resident engine: [cells per thread][lossless, lossy acoustics]

@@ -3,6 +3,7 @@
 // per shape index (FDS_SHAPE), so the shapes compile in parallel.
 #include "fds_kernels.h"
 #include "fds_stream2d.h"
+#include "fds_stream_lossy.h"
 
 #if !defined(FDS_FORM) || !defined(FDS_SHAPE)
 #error "compile with -DFDS_FORM=<0|1|2> -DFDS_SHAPE=<index>"
@@ -47,6 +48,14 @@ __global__ __launch_bounds__(TX) void k_stream_thermal(StepArgs<T> a) {
   thermal_stream<T, K, TX, AXI, S, NT, ISO>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
 }
 
+template <int TX, bool AXI, bool NT, bool ISO>
+__global__ __launch_bounds__(TX) void k_stream_lossy(StepArgs<T> a) {
+  __shared__ LossyLds<T, TX, AXI> L;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x, a.ntiles);
+  if (tile < 0) return;
+  lossy_stream<T, K, TX, AXI, NT, ISO>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
+}
+
 template <int TX, int S, bool NT>
 __global__ __launch_bounds__(TX) void k_stream_batch(const StepArgs<T> *__restrict__ descs,
                                                      const int32_t *__restrict__ tile_inst,
@@ -72,6 +81,12 @@ void launch_stream_thermal(const void *args, int nblocks, size_t, hipStream_t s)
                      *static_cast<const StepArgs<T> *>(args));
 }
 
+template <int TX, bool AXI, bool NT, bool ISO>
+void launch_stream_lossy(const void *args, int nblocks, size_t, hipStream_t s) {
+  hipLaunchKernelGGL((k_stream_lossy<TX, AXI, NT, ISO>), dim3(nblocks), dim3(TX), 0, s,
+                     *static_cast<const StepArgs<T> *>(args));
+}
+
 template <int TX, int S, bool NT>
 void launch_stream_batch(const BatchLaunch &b, int nblocks, size_t, hipStream_t s) {
   hipLaunchKernelGGL((k_stream_batch<TX, S, NT>), dim3(nblocks), dim3(TX), 0, s,
@@ -90,6 +105,11 @@ void fill_variants(KernelSet &ks, int I, int f) {
   ks.stream_thermal[I][1][f] = launch_stream_thermal<TX, true, S, NT, false>;
   ks.stream_thermal[I][2][f] = launch_stream_thermal<TX, false, S, NT, true>;
   ks.stream_batch[I][f] = launch_stream_batch<TX, S, NT>;
+  if (f == 0) {   // lossy acoustics: one step per launch on every shape
+    ks.stream_lossy[I][0] = launch_stream_lossy<TX, false, NT, false>;
+    ks.stream_lossy[I][1] = launch_stream_lossy<TX, true, NT, false>;
+    ks.stream_lossy[I][2] = launch_stream_lossy<TX, false, NT, true>;
+  }
 }
 
 void FDS_CAT(fill_stream_, FDS_SHAPE)(KernelSet &ks) {

@@ -162,6 +162,46 @@ def test_fp64_stream_multi_strip_grids_match_oracle_bitwise(model, nx, ny, slot)
         np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
 
 
+# Lossy acoustics on the streaming engine: one step per launch on every shape, one ghost
+# column per side, the viscous term's old velocities of the rows above and below.
+@pytest.mark.parametrize('model,nx,ny,slot', [('Acoustic2D', 256, 200, 0),
+                                              ('Acoustic2D', 300, 75, 1),
+                                              ('Acoustic3DAxi', 300, 130, 2),
+                                              ('Acoustic3DAxi', 513, 171, 3),
+                                              ('Acoustic2D', 130, 101, 4),
+                                              ('Acoustic3DAxi', 260, 133, 6),
+                                              ('Acoustic2D', 509, 97, 7),
+                                              ('Acoustic3DAxi', 251, 64, 7),
+                                              ('Acoustic2D', 190, 77, 8),
+                                              ('Acoustic3DAxi', 333, 101, 9),
+                                              ('Acoustic2D', 61, 45, 10),
+                                              ('Acoustic3DAxi', 200, 120, 11)])
+def test_fp64_lossy_stream_grids_match_oracle_bitwise(model, nx, ny, slot):
+    idx = _stream_indices()[slot]
+    scene = _big_acoustic(nx, ny, 120, model=model, mu=3e-3)
+    ref = scenes.run_oracle(scene)
+    print('lossy grid', model, nx, ny, 'shape', idx, flush=True)
+    fld = scenes.build_field(pyfds_amd, pyfds_amd.regions, scene)
+    fld.tile = idx
+    scenes.run_plan(fld, scene)
+    assert pyfds_amd._native.engine_of(fld._runner().ctx) == 'stream'
+    res = scenes.collect(fld, scene)
+    for k in ref:
+        np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
+
+
+@pytest.mark.parametrize('form', ['fp32', 'fp32_kahan'])
+@pytest.mark.parametrize('name', ['a2d_lossy', 'axi_lossy'])
+def test_fp32_lossy_stream_equals_tile_engine(name, form):
+    scene = scenes.SCENES[name]()
+    print('scene', name, form, flush=True)
+    a = run_device(scene, form=form, tile=1)
+    for idx in [_stream_indices()[i] for i in (0, 2, 7, 10)]:
+        b = run_device(scene, form=form, tile=idx)
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k], err_msg='%s %s %d' % (name, k, idx))
+
+
 @pytest.mark.parametrize('slot', [None, 1, 3, 8])
 @pytest.mark.parametrize('chunk', [0, 1, 7, 50])
 def test_graph_chunks_bit_identical(chunk, slot):
