@@ -21,7 +21,12 @@ below the algorithmic bytes, so ``frac`` can pass what one pass per step could r
 ``cpu_baseline`` times the numpy oracle (the plain method, single core) on a bounded
 number of steps of the same workload.
 
-Further sections (reported under ``batched`` and ``thermal2d``):
+Further sections (reported under ``lossy``, ``batched`` and ``thermal2d``):
+
+* lossy: the same Acoustic2D 2048^2 scene with absorbing materials (shear viscosity 1e-3 Pa s,
+  bulk viscosity 2.4e-3 Pa s everywhere), FP64 and FP32, autotuned (lossy acoustics stream one
+  step per launch); algorithmic bytes 72 / 36 B per cell update (the viscous factor M adds one
+  word).
 
 * batched: 256 Acoustic2D / Acoustic3DAxi instances, sides uniform in 128..1024 (seed 2),
   1-4 seeded inclusions each (c 1000-6000 m/s, rho 800-8000 kg/m^3), dt at CFL 0.5 on each
@@ -56,6 +61,7 @@ NX = NY = 2048
 STEPS = 2000
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {'fp64': 64, 'fp32': 32, 'fp32_kahan': 56}
+LOSSY_BYTES_PER_CELL = {'fp64': 72, 'fp32': 36, 'fp32_kahan': 60}
 THERMAL_BYTES_PER_CELL = {'fp64': 32, 'fp32': 16, 'fp32_kahan': 24}
 METRIC = 'Gcell-updates/s & % HBM roofline; FP32/batched/learned-BC vs plain FP64'
 
@@ -89,6 +95,35 @@ def build_field(ns, wl, form='fp64', t_samples=None):
                               value=wl['signal'], additive=True)
     fld.pressure.add_output(ns.regions.Region(wl['outs'], 'probes'))
     return fld
+
+
+def make_lossy(ns, fld):
+    for mr in fld.material_regions:
+        for m in mr.materials:
+            m.shear_viscosity, m.bulk_viscosity = 1e-3, 2.4e-3
+    return fld
+
+
+def time_lossy(ns, wl, form, steps, warmup):
+    fld = make_lossy(ns, build_field(ns, wl, form, t_samples=steps + warmup + 50))
+    fld.assemble_matrices()
+    tune = tuned(fld)
+    fld.simulate(warmup)
+    runner = fld._runner()
+    t0 = time.perf_counter()
+    fld.simulate(steps)
+    wall = time.perf_counter() - t0
+    import ctypes as ct
+    ms = ct.c_double()
+    runner.lib.fds_last_run_ms(runner.ctx, ct.byref(ms))
+    cells = wl['nx'] * wl['ny']
+    kernel_s = ms.value / 1e3 / steps
+    achieved = LOSSY_BYTES_PER_CELL[form] * cells / kernel_s / 1e9
+    return dict(value=round(cells * steps / wall / 1e9, 3), ms_per_step=round(wall / steps * 1e3, 4),
+                kernel_us=round(kernel_s * 1e6, 2), achieved_gbs=round(achieved, 1),
+                roofline_frac=round(achieved / HBM_PEAK_GBS, 4),
+                bytes_per_cell=LOSSY_BYTES_PER_CELL[form],
+                autotune={k: v for k, v in tune.items() if k != 'table'})
 
 
 def tuned(fld, fixed=None):
@@ -514,7 +549,8 @@ def main():
     ap.add_argument('--forms', default='fp64,fp32,fp32_kahan')
     ap.add_argument('--cpu-steps', type=int, default=20)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--sections', default='acoustic2d,acoustic1d,batched,thermal,learned')
+    ap.add_argument('--sections', default='acoustic2d,lossy,acoustic1d,batched,thermal,learned')
+    ap.add_argument('--lossy-steps', type=int, default=500)
     ap.add_argument('--a1d-steps', type=int, default=A1D_STEPS)
     ap.add_argument('--learned-iters', type=int, default=1500)
     ap.add_argument('--batch-steps', type=int, default=1000)
@@ -594,6 +630,10 @@ def main():
         'forms': out_forms,
     }
     sections = set(args.sections.split(','))
+    if 'lossy' in sections:
+        line['lossy'] = {f: time_lossy(ns, wl, f, args.lossy_steps, 10) for f in ('fp64', 'fp32')}
+        line['lossy']['config'] = ('Acoustic2D 2048x2048 with absorbing materials, %d steps'
+                                   % args.lossy_steps)
     if 'acoustic1d' in sections:
         line['acoustic1d'] = acoustic1d_section(ns, args.a1d_steps, args.no_cpu)
     if 'batched' in sections:

@@ -141,7 +141,7 @@ int fds_read_outputs(fds_ctx *ctx, int comp, int64_t row0, int64_t nrows, double
  * or -1 for the default.  fds_tile_count reports the table size, fds_tile_shape its entries
  * (tx columns x ty rows).  fds_tile_engine returns 0 for an LDS-tile shape and 1 for a
  * row-streaming shape (a workgroup of tx threads marching down a strip of ty rows); streaming
- * shapes serve the lossless 2D models, other fields given one run on the default tile shape.
+ * shapes serve the 2D models (lossy acoustics one time step per launch on any of them).
  * 1D models (Acoustic1D, Thermal1D, IdealGas1D, Acoustic2ndOrder1D) default (-1) to the
  * resident 1D engine: workgroups keep the line in registers for many steps per launch (one
  * workgroup up to 512 cells, segments beyond; see fds_set_line_shape); any tile index puts
