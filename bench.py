@@ -104,10 +104,10 @@ def make_lossy(ns, fld):
     return fld
 
 
-def time_lossy(ns, wl, form, steps, warmup):
+def time_lossy(ns, wl, form, steps, warmup, fixed=None):
     fld = make_lossy(ns, build_field(ns, wl, form, t_samples=steps + warmup + 50))
     fld.assemble_matrices()
-    tune = tuned(fld)
+    tune = tuned(fld, fixed)
     fld.simulate(warmup)
     runner = fld._runner()
     t0 = time.perf_counter()
@@ -551,6 +551,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--sections', default='acoustic2d,lossy,acoustic1d,batched,thermal,learned')
     ap.add_argument('--lossy-steps', type=int, default=500)
+    ap.add_argument('--lossy-forms', default='fp64,fp32')
+    ap.add_argument('--lossy-tile', type=int, default=None)
+    ap.add_argument('--lossy-strip', type=int, default=0)
     ap.add_argument('--a1d-steps', type=int, default=A1D_STEPS)
     ap.add_argument('--learned-iters', type=int, default=1500)
     ap.add_argument('--batch-steps', type=int, default=1000)
@@ -631,7 +634,9 @@ def main():
     }
     sections = set(args.sections.split(','))
     if 'lossy' in sections:
-        line['lossy'] = {f: time_lossy(ns, wl, f, args.lossy_steps, 10) for f in ('fp64', 'fp32')}
+        lfix = None if args.lossy_tile is None else (args.lossy_tile, args.lossy_strip)
+        line['lossy'] = {f: time_lossy(ns, wl, f, args.lossy_steps, 10, lfix)
+                         for f in args.lossy_forms.split(',') if f}
         line['lossy']['config'] = ('Acoustic2D 2048x2048 with absorbing materials, %d steps'
                                    % args.lossy_steps)
     if 'acoustic1d' in sections:
