@@ -142,10 +142,10 @@ int fds_read_outputs(fds_ctx *ctx, int comp, int64_t row0, int64_t nrows, double
  * (tx columns x ty rows).  fds_tile_engine returns 0 for an LDS-tile shape and 1 for a
  * row-streaming shape (a workgroup of tx threads marching down a strip of ty rows); streaming
  * shapes serve the 2D models (lossy acoustics one time step per launch on any of them).
- * 1D models (Acoustic1D, Thermal1D, IdealGas1D, Acoustic2ndOrder1D) default (-1) to the
- * resident 1D engine: workgroups keep the line in registers for many steps per launch (one
- * workgroup up to 512 cells, segments beyond; see fds_set_line_shape); any tile index puts
- * them on the LDS-tile kernels (one launch per step; nonlinear models on their 256-cell tiles
+ * 1D models (Acoustic1D, Thermal1D, IdealGas1D, Acoustic2ndOrder1D) of up to 2^24 cells
+ * default (-1) to the resident 1D engine: workgroups keep the line in registers for many steps
+ * per launch (one workgroup up to 512 cells, segments beyond; see fds_set_line_shape); any tile
+ * index (and longer lines) puts them on the LDS-tile kernels (one launch per step; nonlinear models on their 256-cell tiles
  * whatever the index). */
 int fds_set_tile(fds_ctx *ctx, int tile_index);
 int fds_tile_count(void);
@@ -162,7 +162,8 @@ int fds_set_strip_height(fds_ctx *ctx, int rows);
  * threads > 0: the line is split into segments of threads * cells_per_thread - 2 * ghost owned
  * cells, one workgroup of `threads` threads each, every workgroup also stepping `ghost` cells
  * of each neighbour (at least 2); a launch then runs at most ghost - 1 steps (temporal
- * blocking), and lines of any length fit. */
+ * blocking).  Lines of up to 2^24 cells; longer ones ignore the layout and step on the
+ * LDS-tile kernels. */
 int fds_set_line_shape(fds_ctx *ctx, int cells_per_thread, int threads, int ghost);
 /* Engine that runs a context's steps: 0 LDS tiles, 1 row streaming, 2 resident 1D,
  * 3 nonlinear 1D tiles (no reference counterpart; for tests and benchmarks). */
@@ -195,7 +196,7 @@ int fds_set_flow(fds_ctx *ctx, const int64_t *period, int64_t nrows);
  *   factor B applied to v * rho; loss = mu / rho_thermal / c_p, evaluated by the caller),
  * every step with step % stepping == 0; with `accumulate` the transfer is summed every step
  * and the sum added (and restarted) on those steps.  accum (n values, in/out) carries that sum
- * across calls.  Any length: one workgroup runs both fields up to 512 cells, segments beyond
+ * across calls.  Up to 2^24 cells: one workgroup runs both fields up to 512 cells, segments beyond
  * (the call sets both contexts' resident layout).  Outputs and state read back from each
  * context as after fds_run.  The reference applies the coupling on the host after every pair
  * of sim_steps. */

@@ -23,6 +23,9 @@ __all__ = [
     'ThermoAcoustic1D',
 ]
 
+# longest line the device-coupled loop takes (the resident 1D engine's limit)
+COUPLED_MAX_CELLS = 1 << 24
+
 
 class ThermoAcoustic1D(cp.SynchronizedFields):
     """An Acoustic1D and a Thermal1D field on the same grid; the acoustic losses heat the
@@ -91,7 +94,7 @@ class ThermoAcoustic1D(cp.SynchronizedFields):
                 return None
         if getattr(acoustic, 'form', 'fp64') != getattr(thermal, 'form', 'fp64'):
             return None
-        if acoustic.num_points != thermal.num_points:
+        if acoustic.num_points != thermal.num_points or acoustic.num_points > COUPLED_MAX_CELLS:
             return None
         return coupling
 

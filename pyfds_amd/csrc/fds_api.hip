@@ -322,8 +322,9 @@ int choose_tiles(fds_ctx *c) {
   c->fuse = 1;
   c->ghost = 0;
   c->ghost_rows = 0;
-  if ((c->one_d || c->family == kNonlinear) && c->tile < 0) {
-    // segmented lines take any length; one segment must hold the whole line.  Automatic
+  if ((c->one_d || c->family == kNonlinear) && c->tile < 0 && c->n <= fds::kLineMaxResident) {
+    // segmented lines take up to kLineMaxResident cells; one segment must hold the whole
+    // line.  Automatic
     // layout: one segment up to kLineSegmentAbove cells, else segments of the default shape
     int cfg_threads = c->line_cfg_threads, cfg_ghost = c->line_cfg_ghost, cpt_req = c->strip_rows;
     if (cfg_threads < 0) {
@@ -1258,6 +1259,9 @@ int fds_couple_loss_run(fds_ctx *ac, fds_ctx *th, const double *rho, const doubl
       if (int rc = choose_tiles(c)) return rc;
       touch(c);
     }
+    if (!c->line)
+      return fail(FDS_E_ARG, "coupled runs hold at most %lld cells",
+                  (long long)fds::kLineMaxResident);
     if (int rc = prepare_run(c, start, nsteps)) return rc;
   }
   const int threads = ac->line_threads;

@@ -57,6 +57,9 @@ constexpr int kLineSegmentAbove = 512;
 constexpr int kLineSegCpt = 2, kLineSegThreads = 64, kLineSegGhost = 48;
 constexpr int kLineSegLongAbove = 16384;
 constexpr int kLineSegLongCpt = 4, kLineSegLongGhost = 64;
+// longest line on the resident engine: its per-thread event tables hold one entry per thread
+// of every segment (half a line's cells); longer lines step on the LDS-tile kernels
+constexpr int64_t kLineMaxResident = int64_t(1) << 24;
 
 enum LineFamily { kLineAcoustic = 0, kLineAcousticLossy = 1, kLineThermal = 2, kLineNonlinear = 3 };
 

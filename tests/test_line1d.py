@@ -205,7 +205,8 @@ def test_engine_choice():
         return nat.engine_of(f._runner().ctx)
 
     assert field(4096) == 'line'
-    assert field(100000) == 'line'           # segments: any length
+    assert field(100000) == 'line'           # segments
+    assert field((1 << 24) + 1) == 'tile'     # past the resident engine's table limit
     assert field(100, tile=TILE_ROW) == 'tile'
     with pytest.raises(ValueError):
         field(3000, strip_height=2)       # 1500 threads: beyond the 2-cell variant's 1024
