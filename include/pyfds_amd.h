@@ -129,13 +129,25 @@ int fds_set_outputs(fds_ctx *ctx, int comp, int64_t m, const int64_t *cells);
 int fds_run(fds_ctx *ctx, int64_t start_step, int64_t nsteps, int chunk);
 
 /* GPU time in milliseconds between HIP events recorded on the context's stream before the
- * first and after the last step launch of the last fds_run (no reference counterpart: the
+ * first and after the last step launch of the last fds_run, or summed over the asynchronous
+ * runs taken since the last one enqueued with none in flight (no reference counterpart: the
  * measurement hook for benchmarks and the autotuner). */
 int fds_last_run_ms(fds_ctx *ctx, double *ms);
 
 /* Read rows [row0, row0+nrows) of a component's recorded outputs from the last run into
  * dst[nrows][m] (row-major, one row per step). */
 int fds_read_outputs(fds_ctx *ctx, int comp, int64_t row0, int64_t nrows, double *dst);
+
+/* Asynchronous runs (no reference counterpart: they let the host turn one run's recorded
+ * outputs into pyfds' per-sample lists while the device computes the next run).
+ * fds_run_async enqueues fds_run's steps on the context's stream, then a copy of the run's
+ * outputs into pinned host memory, and returns at once.  At most two runs may be in flight.
+ * fds_run_take waits for the oldest one and copies its outputs (nsteps rows of m values per
+ * component, as fds_read_outputs) into dst0..dst2 (null entries and components without
+ * outputs are skipped) and adds its time to fds_last_run_ms.  While runs are in flight the
+ * setters, fds_run and fds_read_outputs fail with FDS_E_STATE; fds_get_state waits for them. */
+int fds_run_async(fds_ctx *ctx, int64_t start_step, int64_t nsteps, int chunk);
+int fds_run_take(fds_ctx *ctx, double *dst0, double *dst1, double *dst2);
 
 /* Tile shape of the 2D step kernels (autotuner hook): index into the compiled shape table,
  * or -1 for the default.  fds_tile_count reports the table size, fds_tile_shape its entries
@@ -179,6 +191,10 @@ typedef struct fds_batch fds_batch;
 int fds_batch_create(fds_batch **out, fds_ctx *const *ctxs, int n, int tile);
 int fds_batch_run(fds_batch *b, int64_t start_step, int64_t nsteps, int chunk);
 int fds_batch_last_run_ms(fds_batch *b, double *ms);
+/* Asynchronous batch runs, as fds_run_async / fds_run_take for every member at once: dst holds
+ * three destinations per member (member-major: dst[3 * i + comp]); dst = null discards. */
+int fds_batch_run_async(fds_batch *b, int64_t start_step, int64_t nsteps, int chunk);
+int fds_batch_take(fds_batch *b, double *const *dst);
 int fds_batch_destroy(fds_batch *b);
 
 /* Flow along x (AcousticFlow2D, pyfds/acoustic_flow.py:49-57 apply_flow): after every step n,

@@ -21,11 +21,11 @@ NUM_COEFF, NUM_SCALARS = 8, 8
 
 EXPORTS = ('fds_create', 'fds_destroy', 'fds_set_scalars', 'fds_set_coeff', 'fds_commit',
            'fds_set_state', 'fds_get_state', 'fds_set_boundaries', 'fds_set_outputs',
-           'fds_run', 'fds_read_outputs', 'fds_set_tile', 'fds_tile_count', 'fds_tile_shape',
+           'fds_run', 'fds_run_async', 'fds_run_take', 'fds_read_outputs', 'fds_set_tile', 'fds_tile_count', 'fds_tile_shape',
            'fds_tile_engine', 'fds_tile_traits', 'fds_set_strip_height', 'fds_engine', 'fds_set_line_shape', 'fds_set_flow',
            'fds_couple_loss_run',
            'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_batch_create', 'fds_batch_run',
-           'fds_batch_last_run_ms', 'fds_batch_destroy', 'fds_set_learned_boundary',
+           'fds_batch_run_async', 'fds_batch_take', 'fds_batch_last_run_ms', 'fds_batch_destroy', 'fds_set_learned_boundary',
            'fds_lbc_conv_fwd', 'fds_lbc_conv_bwd_data', 'fds_lbc_conv_bwd_weight',
            'fds_lbc_reduce_weight', 'fds_lbc_mse', 'fds_lbc_adam', 'fds_lbc_pack',
            'fds_lbc_gather', 'fds_lbc_band_rms', 'fds_window_sq_diff', 'fds_output_device_ptr',
@@ -77,12 +77,16 @@ def load():
                                           _lp, _i64, _dp]),
         'fds_set_outputs': (ct.c_int, [_p, ct.c_int, _i64, _lp]),
         'fds_run': (ct.c_int, [_p, _i64, _i64, ct.c_int]),
+        'fds_run_async': (ct.c_int, [_p, _i64, _i64, ct.c_int]),
+        'fds_run_take': (ct.c_int, [_p, _dp, _dp, _dp]),
         'fds_read_outputs': (ct.c_int, [_p, ct.c_int, _i64, _i64, _dp]),
         'fds_set_tile': (ct.c_int, [_p, ct.c_int]),
         'fds_last_run_ms': (ct.c_int, [_p, _dp]),
         'fds_stream_copy_bench': (ct.c_int, [_i64, ct.c_int, _dp, _dp]),
         'fds_batch_create': (ct.c_int, [ct.POINTER(_p), ct.POINTER(_p), ct.c_int, ct.c_int]),
         'fds_batch_run': (ct.c_int, [_p, _i64, _i64, ct.c_int]),
+        'fds_batch_run_async': (ct.c_int, [_p, _i64, _i64, ct.c_int]),
+        'fds_batch_take': (ct.c_int, [_p, ct.POINTER(_p)]),
         'fds_batch_last_run_ms': (ct.c_int, [_p, _dp]),
         'fds_batch_destroy': (ct.c_int, [_p]),
         'fds_set_learned_boundary': (ct.c_int, [_p, _p, _p, ct.c_int, _p]),

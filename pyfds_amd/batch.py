@@ -7,11 +7,13 @@ advances all members with one kernel launch per time step over the tiles of ever
 Acoustic3DAxi, Thermal2D, ...) and grid size; they must share the numeric form and device,
 and they step in lock step from a common step counter.
 """
+import collections
 import ctypes as ct
 
 import numpy as np
 
 from . import _native as nat
+from .device import pipeline_sizes
 
 
 class BatchRunner:
@@ -69,10 +71,48 @@ class BatchRunner:
         valid = [v for v in (r._valid_steps() for r in runners) if v is not None]
         if valid and start + num_steps > min(valid):
             raise IndexError('a boundary signal ends at step {}'.format(min(valid)))
+        sizes = pipeline_sizes(num_steps, num_steps, self.chunk,
+                               sum(r._output_count() for r in runners))
+        if sizes:
+            self._run_pipelined(runners, start, sizes)
+            return
         nat.check(self.lib.fds_batch_run(self.handle, start, num_steps, self.chunk))
         for f, r in zip(self.fields, runners):
             r.collect(f, num_steps)
             f.step += num_steps
+
+    def _run_pipelined(self, runners, start, sizes):
+        """Asynchronous batch runs of ``sizes`` steps, the next one enqueued before the outputs
+        of the current one are appended (``fds_batch_run_async`` / ``fds_batch_take``)."""
+        lib, handle = self.lib, self.handle
+        counts = [[c.size if c is not None else 0 for c in r.out_cells] for r in runners]
+        queue = collections.deque()
+        nxt = [start]
+
+        def enqueue(k):
+            nat.check(lib.fds_batch_run_async(handle, nxt[0], k, self.chunk))
+            queue.append(k)
+            nxt[0] += k
+
+        def take():
+            k = queue.popleft()
+            data = [[np.empty((k, m)) if m else None for m in cs] for cs in counts]
+            ptrs = (ct.c_void_p * (3 * len(data)))(
+                *[None if d is None else d.ctypes.data for ds in data for d in ds])
+            nat.check(lib.fds_batch_take(handle, ptrs))
+            for f, r, ds in zip(self.fields, runners, data):
+                r._append_outputs(f, ds)
+                f.step += k
+
+        try:
+            enqueue(sizes[0])
+            for k in sizes[1:]:
+                enqueue(k)
+                take()
+            take()
+        finally:
+            while queue:
+                take()
 
     def last_run_ms(self):
         ms = ct.c_double()

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <deque>
 #include <map>
 #include <string>
 #include <vector>
@@ -66,6 +67,37 @@ struct DevBuf {
   U *as() const { return static_cast<U *>(p); }
 };
 
+// Pinned host memory (hipHostMalloc), grown on demand: the staging area that asynchronous
+// runs copy their recorded outputs into.
+struct HostBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf &) = delete;
+  HostBuf &operator=(const HostBuf &) = delete;
+  ~HostBuf() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t ensure(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    release();
+    hipError_t e = hipHostMalloc(&p, std::max<size_t>(b, 16), hipHostMallocDefault);
+    if (e == hipSuccess) bytes = std::max<size_t>(b, 16);
+    return e;
+  }
+};
+
+// A run enqueued by fds_run_async / fds_batch_run_async whose outputs are not taken yet:
+// its step count and the staging slot (and event set) it uses.
+struct Flight {
+  int64_t rows = 0;
+  int slot = 0;
+};
+constexpr int kMaxFlights = 2;
+
 template <typename U>
 hipError_t upload(DevBuf &d, const std::vector<U> &h, hipStream_t s) {
   hipError_t e = d.ensure(std::max<size_t>(h.size() * sizeof(U), 16));
@@ -112,6 +144,7 @@ struct Comp {
   DevBuf lhead, ohead;   // resident 1D engine: per-thread first boundary cell / output point
   DevBuf obuf;
   int64_t obuf_rows = 0;
+  HostBuf stage[kMaxFlights];   // outputs of asynchronous runs, one slot per run in flight
 };
 
 enum Family { kAcoustic, kThermal, kNonlinear };
@@ -161,6 +194,12 @@ struct fds_ctx {
   std::vector<hipGraph_t> graph_objs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;   // bracket the launches of the last run
   double last_run_ms = 0.0;
+  // asynchronous runs in flight (oldest first), the next staging slot, and per slot the
+  // events after the run's first launch, its last launch and its output copies
+  std::deque<Flight> flights;
+  int next_slot = 0;
+  int batch_flights = 0;          // asynchronous batch runs in flight that include this context
+  hipEvent_t fev[kMaxFlights][3] = {};
   // flow along x (AcousticFlow2D): per-row shift period, applied after every step
   DevBuf flow_period;
   bool flow = false;
@@ -192,6 +231,9 @@ struct fds_batch {
   int flip = 0;                      // members flipped this many times (mod 2) since build
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_run_ms = 0.0;
+  std::deque<Flight> flights;     // asynchronous batch runs in flight (oldest first)
+  int next_slot = 0;
+  hipEvent_t fev[kMaxFlights][3] = {};
 };
 
 namespace {
@@ -626,6 +668,15 @@ __global__ __launch_bounds__(256) void k_flow_shift(T *a0, T *a1, T *a2, T *c0, 
   }
 }
 
+// Step counters at the start of a run, written in stream order (no host round trip, so a run
+// may be enqueued behind one still executing).
+__global__ void k_set_dyn(int64_t *dyn, int64_t start) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    dyn[0] = start;
+    dyn[1] = 0;
+  }
+}
+
 __global__ void k_advance(int64_t *dyn, int64_t by) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     dyn[0] += by;
@@ -716,6 +767,8 @@ int get_graph(fds_ctx *c, int chunk, int parity, hipGraphExec_t *out) {
     *out = it->second;
     return 0;
   }
+  // capture on an idle stream (an asynchronous run may still be executing)
+  HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
   const int grp = step_group(c);   // chunk is a multiple of grp
@@ -748,6 +801,14 @@ int check_ctx(fds_ctx *c) {
   if (!c) return fail(FDS_E_ARG, "null context");
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return fail(FDS_E_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  return 0;
+}
+
+// Setters, synchronous runs and output reads wait until the asynchronous runs are taken.
+int check_idle(fds_ctx *c) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->flights.empty() || c->batch_flights)
+    return fail(FDS_E_STATE, "asynchronous runs in flight: take them first");
   return 0;
 }
 
@@ -898,13 +959,16 @@ int fds_destroy(fds_ctx *c) {
   drop_graphs(c);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (auto &set : c->fev)
+    for (auto e : set)
+      if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
 }
 
 int fds_set_scalars(fds_ctx *c, const double *s, int n) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_idle(c)) return rc;
   if (n < 0 || n > FDS_NUM_SCALARS) return fail(FDS_E_ARG, "bad scalar count %d", n);
   for (int i = 0; i < n; ++i) c->s[i] = s[i];
   touch(c);
@@ -912,7 +976,7 @@ int fds_set_scalars(fds_ctx *c, const double *s, int n) {
 }
 
 int fds_set_coeff(fds_ctx *c, int slot, const double *data, int64_t n) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_idle(c)) return rc;
   if (slot < 0 || slot >= FDS_NUM_COEFF) return fail(FDS_E_ARG, "bad coefficient slot %d", slot);
   if (n == 0) {
     c->coef[slot].release();
@@ -930,7 +994,7 @@ int fds_set_coeff(fds_ctx *c, int slot, const double *data, int64_t n) {
 }
 
 int fds_commit(fds_ctx *c) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_idle(c)) return rc;
   if (c->family == kAcoustic) {
     if (!c->coef_set[FDS_C_KX] || !c->coef_set[FDS_C_BX])
       return fail(FDS_E_STATE, "acoustic coefficients K and B must be set");
@@ -961,7 +1025,7 @@ int fds_commit(fds_ctx *c) {
 }
 
 int fds_set_state(fds_ctx *c, int comp, const double *data, int64_t n) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_idle(c)) return rc;
   if (int rc = check_comp(c, comp)) return rc;
   if (n != c->n) return fail(FDS_E_ARG, "state needs %lld values", (long long)c->n);
   Comp &cp = c->comp[comp];
@@ -1007,7 +1071,7 @@ int fds_set_boundaries(fds_ctx *c, int comp, int64_t ncell, const int64_t *cells
                        const int32_t *ptr, int64_t nent, const double *a, const int64_t *off,
                        const int64_t *stride, const int64_t *len, int64_t ntable,
                        const double *table) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_idle(c)) return rc;
   if (int rc = check_comp(c, comp)) return rc;
   Comp &cp = c->comp[comp];
   if (ncell < 0 || nent < 0 || ntable < 0) return fail(FDS_E_ARG, "negative size");
@@ -1041,7 +1105,7 @@ int fds_set_boundaries(fds_ctx *c, int comp, int64_t ncell, const int64_t *cells
 }
 
 int fds_set_outputs(fds_ctx *c, int comp, int64_t m, const int64_t *cells) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_idle(c)) return rc;
   if (int rc = check_comp(c, comp)) return rc;
   if (m < 0) return fail(FDS_E_ARG, "negative output count");
   for (int64_t k = 0; k < m; ++k)
@@ -1055,7 +1119,7 @@ int fds_set_outputs(fds_ctx *c, int comp, int64_t m, const int64_t *cells) {
 }
 
 int fds_set_tile(fds_ctx *c, int t) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_idle(c)) return rc;
   if (t < -1 || t >= fds::kNumTiles + fds::kNumStream)
     return fail(FDS_E_ARG, "tile index %d out of range", t);
   const int old = c->tile;
@@ -1071,7 +1135,7 @@ int fds_set_tile(fds_ctx *c, int t) {
 }
 
 int fds_set_line_shape(fds_ctx *c, int cells_per_thread, int threads, int ghost) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_idle(c)) return rc;
   if (!c->one_d && c->family != kNonlinear)
     return fail(FDS_E_ARG, "the resident engine serves 1D models only");
   const int fam = c->family == kNonlinear ? fds::kLineNonlinear : fds::kLineAcoustic;
@@ -1102,7 +1166,7 @@ int fds_set_line_shape(fds_ctx *c, int cells_per_thread, int threads, int ghost)
 }
 
 int fds_set_strip_height(fds_ctx *c, int rows) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_idle(c)) return rc;
   if (rows < 0 || rows > 65536) return fail(FDS_E_ARG, "strip height %d out of range", rows);
   const int old = c->strip_rows;
   c->strip_rows = rows;
@@ -1144,9 +1208,8 @@ int prepare_run(fds_ctx *c, int64_t start, int64_t nsteps) {
     }
     cp.obuf_rows = cp.out_cells.empty() ? 0 : nsteps;
   }
-  int64_t dyn0[2] = {start, 0};
-  HIP_TRY(hipMemcpyAsync(c->dyn.p, dyn0, sizeof dyn0, hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  hipLaunchKernelGGL(k_set_dyn, dim3(1), dim3(64), 0, c->stream, c->dyn.as<int64_t>(), start);
+  HIP_TRY(hipGetLastError());
   if (!c->ev0) HIP_TRY(hipEventCreate(&c->ev0));
   if (!c->ev1) HIP_TRY(hipEventCreate(&c->ev1));
   if (c->line) {
@@ -1220,64 +1283,10 @@ int couple_run(fds_ctx *ac, fds_ctx *th, const double *rho, const double *loss, 
   return 0;
 }
 
-}  // namespace
-
-extern "C" {
-
-int fds_couple_loss_run(fds_ctx *ac, fds_ctx *th, const double *rho, const double *loss,
-                        double dt, int64_t stepping, int accumulate, double *accum,
-                        int64_t start, int64_t nsteps) {
-  if (int rc = check_ctx(ac)) return rc;
-  if (int rc = check_ctx(th)) return rc;
-  if (ac->model != FDS_ACOUSTIC1D || th->model != FDS_THERMAL1D)
-    return fail(FDS_E_ARG, "the loss coupling joins an Acoustic1D and a Thermal1D context");
-  if (ac->device != th->device || ac->form != th->form || ac->n != th->n)
-    return fail(FDS_E_ARG, "coupled contexts must share device, form and cell count");
-  if (!ac->committed || !th->committed) return fail(FDS_E_STATE, "coefficients not committed");
-  if (ac->tile >= 0 || th->tile >= 0)
-    return fail(FDS_E_ARG, "coupled contexts must use the resident 1D engine (tile -1)");
-  if (!rho || !loss || (accumulate && !accum)) return fail(FDS_E_ARG, "null coupling input");
-  if (stepping < 1) return fail(FDS_E_ARG, "coupling stepping must be at least 1");
-  if (nsteps < 0 || start < 0) return fail(FDS_E_ARG, "negative step count");
-  if (nsteps == 0) return 0;
-  // one layout for both contexts: one workgroup of one cell per thread for short lines (the
-  // coupled kernel holds two fields, so it has half the single-field threads from 2 cells per
-  // thread on), else the single-field default segments
-  const bool seg = ac->n > fds::kLineSegmentAbove;
-  const int cpt = seg ? fds::kLineSegCpt : 1;
-  const int cfg_t = seg ? fds::kLineSegThreads : 0, cfg_g = seg ? fds::kLineSegGhost : 0;
-  static_assert(fds::kLineSegThreads <= fds::kCoupleMaxThreads[1] && fds::kLineSegCpt == 2 &&
-                    fds::kLineSegmentAbove <= fds::kCoupleMaxThreads[0] &&
-                    fds::kLineSegGhost > 2,
-                "coupled layouts must fit the coupled kernel's launch bounds");
-  for (fds_ctx *c : {ac, th}) {
-    if (!c->line || c->strip_rows != cpt || c->line_cfg_threads != cfg_t ||
-        c->line_cfg_ghost != cfg_g) {
-      c->strip_rows = cpt;
-      c->line_cfg_threads = cfg_t;
-      c->line_cfg_ghost = cfg_g;
-      if (int rc = choose_tiles(c)) return rc;
-      touch(c);
-    }
-    if (!c->line)
-      return fail(FDS_E_ARG, "coupled runs hold at most %lld cells",
-                  (long long)fds::kLineMaxResident);
-    if (int rc = prepare_run(c, start, nsteps)) return rc;
-  }
-  const int threads = ac->line_threads;
-  return ac->elem == 8 ? couple_run<double>(ac, th, rho, loss, dt, stepping, accumulate, accum,
-                                            start, nsteps, threads)
-                       : couple_run<float>(ac, th, rho, loss, dt, stepping, accumulate, accum,
-                                           start, nsteps, threads);
-}
-
-int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
-  if (int rc = check_ctx(c)) return rc;
-  if (!c->committed) return fail(FDS_E_STATE, "coefficients not committed");
-  if (nsteps < 0 || start < 0) return fail(FDS_E_ARG, "negative step count");
-  if (nsteps == 0) return 0;
-  if (int rc = prepare_run(c, start, nsteps)) return rc;
-  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+// The steps of a prepared run (prepare_run) enqueued on the context's stream, between events
+// e0 and e1.
+int enqueue_steps(fds_ctx *c, int64_t nsteps, int chunk, hipEvent_t e0, hipEvent_t e1) {
+  HIP_TRY(hipEventRecord(e0, c->stream));
   int64_t done = 0;
   if (c->line) {
     // one segment: up to kLineSteps steps per launch.  Several: a wrong value enters each
@@ -1320,7 +1329,111 @@ int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
     if (int rc = launch_step_any(c, c->parity, (int)k, last)) return rc;
     c->parity ^= 1;
   }
-  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipEventRecord(e1, c->stream));
+  return 0;
+}
+
+// True when prepare_run(c, ., nsteps) neither rebuilds the event tables nor grows an output
+// buffer (so it frees no device memory a run in flight may use).
+bool run_steady(const fds_ctx *c, int64_t nsteps) {
+  if (c->events_dirty || (!c->graphs.empty() && c->graph_generation != c->generation))
+    return false;
+  for (int k = 0; k < 3; ++k) {
+    const Comp &cp = c->comp[k];
+    if (cp.obuf.bytes < std::max<size_t>(cp.out_cells.size() * nsteps * 8, 16)) return false;
+  }
+  return true;
+}
+
+int ensure_events(hipEvent_t (&ev)[3]) {
+  for (int i = 0; i < 3; ++i)
+    if (!ev[i]) HIP_TRY(hipEventCreate(&ev[i]));
+  return 0;
+}
+
+// Enqueue the copy of a run's recorded outputs (rows x outputs per component) into staging
+// slot `slot` on stream s.
+int stage_outputs(fds_ctx *c, int slot, int64_t rows, hipStream_t s) {
+  for (int k = 0; k < 3; ++k) {
+    Comp &cp = c->comp[k];
+    const size_t bytes = cp.out_cells.size() * (size_t)rows * 8;
+    if (!bytes) continue;
+    HIP_TRY(cp.stage[slot].ensure(bytes));
+    HIP_TRY(hipMemcpyAsync(cp.stage[slot].p, cp.obuf.p, bytes, hipMemcpyDeviceToHost, s));
+  }
+  return 0;
+}
+
+// Copy a finished run's staged outputs to the caller (null destinations are skipped).
+void take_outputs(fds_ctx *c, const Flight &f, double *const *dst) {
+  for (int k = 0; k < 3; ++k) {
+    const Comp &cp = c->comp[k];
+    const size_t bytes = cp.out_cells.size() * (size_t)f.rows * 8;
+    if (bytes && dst[k]) std::memcpy(dst[k], cp.stage[f.slot].p, bytes);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fds_couple_loss_run(fds_ctx *ac, fds_ctx *th, const double *rho, const double *loss,
+                        double dt, int64_t stepping, int accumulate, double *accum,
+                        int64_t start, int64_t nsteps) {
+  if (int rc = check_idle(ac)) return rc;
+  if (int rc = check_idle(th)) return rc;
+  if (ac->model != FDS_ACOUSTIC1D || th->model != FDS_THERMAL1D)
+    return fail(FDS_E_ARG, "the loss coupling joins an Acoustic1D and a Thermal1D context");
+  if (ac->device != th->device || ac->form != th->form || ac->n != th->n)
+    return fail(FDS_E_ARG, "coupled contexts must share device, form and cell count");
+  if (!ac->committed || !th->committed) return fail(FDS_E_STATE, "coefficients not committed");
+  if (ac->tile >= 0 || th->tile >= 0)
+    return fail(FDS_E_ARG, "coupled contexts must use the resident 1D engine (tile -1)");
+  if (!rho || !loss || (accumulate && !accum)) return fail(FDS_E_ARG, "null coupling input");
+  if (stepping < 1) return fail(FDS_E_ARG, "coupling stepping must be at least 1");
+  if (nsteps < 0 || start < 0) return fail(FDS_E_ARG, "negative step count");
+  if (nsteps == 0) return 0;
+  // one layout for both contexts: one workgroup of one cell per thread for short lines (the
+  // coupled kernel holds two fields, so it has half the single-field threads from 2 cells per
+  // thread on), else the single-field default segments
+  const bool seg = ac->n > fds::kLineSegmentAbove;
+  const int cpt = seg ? fds::kLineSegCpt : 1;
+  const int cfg_t = seg ? fds::kLineSegThreads : 0, cfg_g = seg ? fds::kLineSegGhost : 0;
+  static_assert(fds::kLineSegThreads <= fds::kCoupleMaxThreads[1] && fds::kLineSegCpt == 2 &&
+                    fds::kLineSegmentAbove <= fds::kCoupleMaxThreads[0] &&
+                    fds::kLineSegGhost > 2,
+                "coupled layouts must fit the coupled kernel's launch bounds");
+  for (fds_ctx *c : {ac, th}) {
+    if (!c->line || c->strip_rows != cpt || c->line_cfg_threads != cfg_t ||
+        c->line_cfg_ghost != cfg_g) {
+      c->strip_rows = cpt;
+      c->line_cfg_threads = cfg_t;
+      c->line_cfg_ghost = cfg_g;
+      if (int rc = choose_tiles(c)) return rc;
+      touch(c);
+    }
+    if (!c->line)
+      return fail(FDS_E_ARG, "coupled runs hold at most %lld cells",
+                  (long long)fds::kLineMaxResident);
+    if (int rc = prepare_run(c, start, nsteps)) return rc;
+  }
+  // the coupled kernel runs on the acoustic context's stream and reads the thermal context's
+  // step counters, which prepare_run wrote on the thermal stream
+  HIP_TRY(hipStreamSynchronize(th->stream));
+  const int threads = ac->line_threads;
+  return ac->elem == 8 ? couple_run<double>(ac, th, rho, loss, dt, stepping, accumulate, accum,
+                                            start, nsteps, threads)
+                       : couple_run<float>(ac, th, rho, loss, dt, stepping, accumulate, accum,
+                                           start, nsteps, threads);
+}
+
+int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
+  if (int rc = check_idle(c)) return rc;
+  if (!c->committed) return fail(FDS_E_STATE, "coefficients not committed");
+  if (nsteps < 0 || start < 0) return fail(FDS_E_ARG, "negative step count");
+  if (nsteps == 0) return 0;
+  if (int rc = prepare_run(c, start, nsteps)) return rc;
+  if (int rc = enqueue_steps(c, nsteps, chunk, c->ev0, c->ev1)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -1328,8 +1441,44 @@ int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
   return 0;
 }
 
-int fds_set_flow(fds_ctx *c, const int64_t *period, int64_t nrows) {
+int fds_run_async(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
   if (int rc = check_ctx(c)) return rc;
+  if (!c->committed) return fail(FDS_E_STATE, "coefficients not committed");
+  if (nsteps < 1 || start < 0) return fail(FDS_E_ARG, "an asynchronous run needs steps");
+  if (c->batch_flights) return fail(FDS_E_STATE, "a batch run of this context is in flight");
+  if ((int)c->flights.size() >= kMaxFlights)
+    return fail(FDS_E_STATE, "%d runs in flight: take the oldest first", kMaxFlights);
+  // a run that rebuilds tables or grows the output rows frees device memory the runs in
+  // flight may read: let them finish first
+  if (!run_steady(c, nsteps)) HIP_TRY(hipStreamSynchronize(c->stream));
+  if (int rc = prepare_run(c, start, nsteps)) return rc;
+  const int slot = c->next_slot;
+  if (int rc = ensure_events(c->fev[slot])) return rc;
+  if (int rc = enqueue_steps(c, nsteps, chunk, c->fev[slot][0], c->fev[slot][1])) return rc;
+  if (int rc = stage_outputs(c, slot, nsteps, c->stream)) return rc;
+  HIP_TRY(hipEventRecord(c->fev[slot][2], c->stream));
+  if (c->flights.empty()) c->last_run_ms = 0.0;   // a new sequence of asynchronous runs
+  c->flights.push_back(Flight{nsteps, slot});
+  c->next_slot = (slot + 1) % kMaxFlights;
+  return 0;
+}
+
+int fds_run_take(fds_ctx *c, double *dst0, double *dst1, double *dst2) {
+  if (int rc = check_ctx(c)) return rc;
+  if (c->flights.empty()) return fail(FDS_E_STATE, "no asynchronous run in flight");
+  const Flight f = c->flights.front();
+  HIP_TRY(hipEventSynchronize(c->fev[f.slot][2]));
+  c->flights.pop_front();
+  double *dst[3] = {dst0, dst1, dst2};
+  take_outputs(c, f, dst);
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, c->fev[f.slot][0], c->fev[f.slot][1]));
+  c->last_run_ms += ms;
+  return 0;
+}
+
+int fds_set_flow(fds_ctx *c, const int64_t *period, int64_t nrows) {
+  if (int rc = check_idle(c)) return rc;
   if (!period || nrows == 0) {
     c->flow = false;
     touch(c);
@@ -1349,7 +1498,7 @@ int fds_set_flow(fds_ctx *c, const int64_t *period, int64_t nrows) {
 
 int fds_set_learned_boundary(fds_ctx *c, const uint16_t *w1, const uint16_t *w2, int layer,
                              const float *comp_scale) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_idle(c)) return rc;
   if (!w1) {
     c->lbc_w1 = c->lbc_w2 = nullptr;
     touch(c);
@@ -1381,7 +1530,7 @@ int fds_last_run_ms(fds_ctx *c, double *ms) {
 }
 
 int fds_read_outputs(fds_ctx *c, int comp, int64_t row0, int64_t nrows, double *dst) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_idle(c)) return rc;
   if (int rc = check_comp(c, comp)) return rc;
   Comp &cp = c->comp[comp];
   const int64_t m = (int64_t)cp.out_cells.size();
@@ -1484,6 +1633,7 @@ int batch_graph(fds_batch *b, int chunk, int rel_parity, hipGraphExec_t *out) {
     *out = it->second;
     return 0;
   }
+  HIP_TRY(hipStreamSynchronize(b->stream));
   HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
   const int grp = b->fuse;   // chunk is a multiple of grp
@@ -1511,6 +1661,68 @@ int batch_graph(fds_batch *b, int chunk, int rel_parity, hipGraphExec_t *out) {
   return 0;
 }
 
+// Checks a batch run's members, (re)builds the descriptors and output rows for nsteps steps
+// and enqueues the step counters' start.
+int batch_prepare(fds_batch *b, int64_t start, int64_t nsteps) {
+  for (auto *c : b->ctxs) {
+    if (!c->committed) return fail(FDS_E_STATE, "coefficients not committed");
+    if (!c->flights.empty()) return fail(FDS_E_STATE, "a member has asynchronous runs in flight");
+    for (int k = 0; k < 3; ++k)
+      for (size_t e = 0; e < c->comp[k].len.size(); ++e)
+        if (start + nsteps > c->comp[k].len[e])
+          return fail(FDS_E_INDEX, "boundary signal too short for step %lld",
+                      (long long)(start + nsteps - 1));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  int rc = b->elem == 8 ? batch_build<double>(b, nsteps) : batch_build<float>(b, nsteps);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_set_dyn, dim3(1), dim3(64), 0, b->stream, b->dyn.as<int64_t>(), start);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// True when batch_prepare rebuilds nothing (no device memory freed under a run in flight).
+bool batch_steady(const fds_batch *b, int64_t nsteps) {
+  for (const auto *c : b->ctxs)
+    if (!run_steady(c, nsteps)) return false;
+  return !batch_stale(b);
+}
+
+// The steps of a prepared batch run between events e0 and e1; the members' buffer parities
+// follow the launches.
+int batch_enqueue(fds_batch *b, int64_t nsteps, int chunk, hipEvent_t e0, hipEvent_t e1) {
+  HIP_TRY(hipEventRecord(e0, b->stream));
+  int rel = b->flip;
+  int64_t done = 0;
+  const int g = b->fuse;
+  if (chunk > 0) {
+    chunk = std::max(g, chunk - chunk % g);
+    while (nsteps - done > chunk) {
+      hipGraphExec_t ex;
+      if (int rc2 = batch_graph(b, chunk, rel, &ex)) return rc2;
+      HIP_TRY(hipGraphLaunch(ex, b->stream));
+      rel ^= ((chunk / g) & 1);
+      done += chunk;
+    }
+  }
+  int64_t k = 0;
+  for (; g > 1 && nsteps - done - k >= g; k += g) {
+    if (int rc2 = batch_launch(b, rel, (int)k, done + k + g == nsteps, true)) return rc2;
+    rel ^= 1;
+  }
+  for (; done + k < nsteps; ++k) {
+    if (int rc2 = batch_launch(b, rel, (int)k, done + k == nsteps - 1)) return rc2;
+    rel ^= 1;
+  }
+  HIP_TRY(hipEventRecord(e1, b->stream));
+  // members advanced nsteps steps in lock step, flipping buffers once per launch
+  if (rel != b->flip) {
+    for (auto *c : b->ctxs) c->parity ^= 1;
+    b->flip ^= 1;
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1528,6 +1740,10 @@ int fds_batch_create(fds_batch **out, fds_ctx *const *ctxs, int n, int tile) {
   b->tile = tile;
   for (auto *c : b->ctxs) {
     if (!c || !c->committed) { delete b; return fail(FDS_E_STATE, "batch member not committed"); }
+    if (!c->flights.empty() || c->batch_flights) {
+      delete b;
+      return fail(FDS_E_STATE, "a batch member has asynchronous runs in flight");
+    }
     if (c->device != b->device || c->form != b->form) {
       delete b;
       return fail(FDS_E_ARG, "batch members must share device and form");
@@ -1587,6 +1803,12 @@ int fds_batch_destroy(fds_batch *b) {
   batch_drop_graphs(b);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  for (auto &set : b->fev)
+    for (auto e : set)
+      if (e) (void)hipEventDestroy(e);
+  // members no longer wait for this batch's runs
+  for (size_t i = 0; i < b->flights.size(); ++i)
+    for (auto *c : b->ctxs) c->batch_flights--;
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return 0;
@@ -1597,55 +1819,53 @@ int fds_batch_run(fds_batch *b, int64_t start, int64_t nsteps, int chunk) {
   HIP_TRY(hipSetDevice(b->device));
   if (nsteps < 0 || start < 0) return fail(FDS_E_ARG, "negative step count");
   if (nsteps == 0) return 0;
-  for (auto *c : b->ctxs) {
-    if (!c->committed) return fail(FDS_E_STATE, "coefficients not committed");
-    for (int k = 0; k < 3; ++k)
-      for (size_t e = 0; e < c->comp[k].len.size(); ++e)
-        if (start + nsteps > c->comp[k].len[e])
-          return fail(FDS_E_INDEX, "boundary signal too short for step %lld",
-                      (long long)(start + nsteps - 1));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-  }
-  int rc = b->elem == 8 ? batch_build<double>(b, nsteps) : batch_build<float>(b, nsteps);
-  if (rc) return rc;
-  int64_t dyn0[2] = {start, 0};
-  HIP_TRY(hipMemcpyAsync(b->dyn.p, dyn0, sizeof dyn0, hipMemcpyHostToDevice, b->stream));
-  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (!b->flights.empty()) return fail(FDS_E_STATE, "asynchronous runs in flight: take them first");
+  if (int rc = batch_prepare(b, start, nsteps)) return rc;
   if (!b->ev0) HIP_TRY(hipEventCreate(&b->ev0));
   if (!b->ev1) HIP_TRY(hipEventCreate(&b->ev1));
-  HIP_TRY(hipEventRecord(b->ev0, b->stream));
-  int rel = b->flip;
-  int64_t done = 0;
-  const int g = b->fuse;
-  if (chunk > 0) {
-    chunk = std::max(g, chunk - chunk % g);
-    while (nsteps - done > chunk) {
-      hipGraphExec_t ex;
-      if (int rc2 = batch_graph(b, chunk, rel, &ex)) return rc2;
-      HIP_TRY(hipGraphLaunch(ex, b->stream));
-      rel ^= ((chunk / g) & 1);
-      done += chunk;
-    }
-  }
-  int64_t k = 0;
-  for (; g > 1 && nsteps - done - k >= g; k += g) {
-    if (int rc2 = batch_launch(b, rel, (int)k, done + k + g == nsteps, true)) return rc2;
-    rel ^= 1;
-  }
-  for (; done + k < nsteps; ++k) {
-    if (int rc2 = batch_launch(b, rel, (int)k, done + k == nsteps - 1)) return rc2;
-    rel ^= 1;
-  }
-  HIP_TRY(hipEventRecord(b->ev1, b->stream));
+  if (int rc = batch_enqueue(b, nsteps, chunk, b->ev0, b->ev1)) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->last_run_ms = ms;
-  // members advanced nsteps steps in lock step, flipping buffers once per launch
-  if (rel != b->flip) {
-    for (auto *c : b->ctxs) c->parity ^= 1;
-    b->flip ^= 1;
+  return 0;
+}
+
+int fds_batch_run_async(fds_batch *b, int64_t start, int64_t nsteps, int chunk) {
+  if (!b) return fail(FDS_E_ARG, "null batch");
+  HIP_TRY(hipSetDevice(b->device));
+  if (nsteps < 1 || start < 0) return fail(FDS_E_ARG, "an asynchronous run needs steps");
+  if ((int)b->flights.size() >= kMaxFlights)
+    return fail(FDS_E_STATE, "%d runs in flight: take the oldest first", kMaxFlights);
+  if (!batch_steady(b, nsteps)) HIP_TRY(hipStreamSynchronize(b->stream));
+  if (int rc = batch_prepare(b, start, nsteps)) return rc;
+  const int slot = b->next_slot;
+  if (int rc = ensure_events(b->fev[slot])) return rc;
+  if (int rc = batch_enqueue(b, nsteps, chunk, b->fev[slot][0], b->fev[slot][1])) return rc;
+  for (auto *c : b->ctxs)
+    if (int rc = stage_outputs(c, slot, nsteps, b->stream)) return rc;
+  HIP_TRY(hipEventRecord(b->fev[slot][2], b->stream));
+  if (b->flights.empty()) b->last_run_ms = 0.0;
+  b->flights.push_back(Flight{nsteps, slot});
+  for (auto *c : b->ctxs) c->batch_flights++;
+  b->next_slot = (slot + 1) % kMaxFlights;
+  return 0;
+}
+
+int fds_batch_take(fds_batch *b, double *const *dst) {
+  if (!b) return fail(FDS_E_ARG, "null batch");
+  HIP_TRY(hipSetDevice(b->device));
+  if (b->flights.empty()) return fail(FDS_E_STATE, "no asynchronous run in flight");
+  const Flight f = b->flights.front();
+  HIP_TRY(hipEventSynchronize(b->fev[f.slot][2]));
+  b->flights.pop_front();
+  for (size_t i = 0; i < b->ctxs.size(); ++i) {
+    b->ctxs[i]->batch_flights--;
+    if (dst) take_outputs(b->ctxs[i], f, dst + 3 * i);
   }
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, b->fev[f.slot][0], b->fev[f.slot][1]));
+  b->last_run_ms += ms;
   return 0;
 }
 

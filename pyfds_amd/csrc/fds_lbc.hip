@@ -267,9 +267,12 @@ __global__ __launch_bounds__(256) void k_lbc_step(LbcStep A) {
   constexpr int R = kLbcTaps / 2;
   constexpr int NX = kLbcSeg + 4 * R;                       // input positions
   constexpr int NH = (kLbcSeg + 2 * R + 15) / 16 * 16;      // hidden rows (whole MFMA tiles)
+  constexpr int NXP = NH + 2 * R;                           // input rows the tiles read
   constexpr int NW1 = kLbcHidden * kLbcTaps * kLbcCin;
   constexpr int NW2 = kLbcCout * kLbcTaps * kLbcHidden;
-  __shared__ __attribute__((aligned(16))) uint16_t xin[NX * kLbcCin];
+  // input rows past NX feed only hidden rows past kLbcSeg + 2R, which layer 2 never reads;
+  // they are zeroed so that every tile reads its operands without a bounds test
+  __shared__ __attribute__((aligned(16))) uint16_t xin[NXP * kLbcCin];
   __shared__ __attribute__((aligned(16))) uint16_t hid[NH * kLbcHidden];
   __shared__ __attribute__((aligned(16))) uint16_t sw1[NW1];
   __shared__ __attribute__((aligned(16))) uint16_t sw2[NW2];
@@ -278,9 +281,15 @@ __global__ __launch_bounds__(256) void k_lbc_step(LbcStep A) {
   const int G = st.g, L = st.l;
   const int s = blockIdx.y, a0 = blockIdx.x * kLbcSeg;
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, m = l & 15, g = l >> 4;
+  // top and bottom rows run along a (contiguous in memory), left and right columns along the
+  // depth: the gather and the scatter let consecutive lanes walk the contiguous direction
+  const bool along = s < 2;
 
   // this side's weights and the band inputs into LDS: every global load is issued before the
-  // first LDS store, so the workgroup waits for one memory round trip, not one per element
+  // first LDS store, so the workgroup waits for one memory round trip, not one per element.
+  // Components are picked by selects among the argument block's pointers: indexing the block
+  // with a lane-varying component makes the compiler load the pointer itself first, one
+  // dependent round trip per element
   constexpr int N1 = NW1 / 4, N2 = NW2 / 4, NI = NX * kLbcCin;
   constexpr int I1 = (N1 + 255) / 256, I2 = (N2 + 255) / 256, II = (NI + 255) / 256;
   s4 r1[I1], r2[I2];
@@ -288,27 +297,26 @@ __global__ __launch_bounds__(256) void k_lbc_step(LbcStep A) {
   {
     const s4 *g1 = reinterpret_cast<const s4 *>(st.w1 + (size_t)s * NW1);
     const s4 *g2 = reinterpret_cast<const s4 *>(st.w2 + (size_t)s * NW2);
+    // loads past the end re-read the last element (the LDS store below skips them): a load
+    // under a branch would make the compiler wait for it where the branches join
 #pragma unroll
-    for (int i = 0; i < I1; ++i) {
-      const int e = tid + 256 * i;
-      if (e < N1) r1[i] = g1[e];
-    }
+    for (int i = 0; i < I1; ++i) r1[i] = g1[min(tid + 256 * i, N1 - 1)];
 #pragma unroll
-    for (int i = 0; i < I2; ++i) {
-      const int e = tid + 256 * i;
-      if (e < N2) r2[i] = g2[e];
-    }
+    for (int i = 0; i < I2; ++i) r2[i] = g2[min(tid + 256 * i, N2 - 1)];
+    const float *cur0 = st.cur[0], *cur1 = st.cur[1], *cur2 = st.cur[2];
+    const float *prv0 = st.prev[0], *prv1 = st.prev[1], *prv2 = st.prev[2];
 #pragma unroll
     for (int i = 0; i < II; ++i) {
       const int e = tid + 256 * i;
       float v = 0.f;
       if (e < NI) {
-        const int row = e / kLbcCin, ci = e % kLbcCin;
+        const int row = along ? e % NX : e / kLbcCin, ci = along ? e / NX : e % kLbcCin;
         const int a = a0 - 2 * R + row;
         if (a >= L && a < G - L) {
           const int h = ci / 24, c = (ci % 24) / kLbcBand, j = ci % kLbcBand;
-          const int cell = lbc_cell(s, L + j, a, G);
-          v = (h == 0 ? st.cur[c][cell] : st.prev[c][cell]) * A.cs[c];
+          const float *src = h == 0 ? (c == 0 ? cur0 : c == 1 ? cur1 : cur2)
+                                    : (c == 0 ? prv0 : c == 1 ? prv1 : prv2);
+          v = src[lbc_cell(s, L + j, a, G)];
         }
       }
       rx[i] = v;
@@ -325,10 +333,16 @@ __global__ __launch_bounds__(256) void k_lbc_step(LbcStep A) {
       const int e = tid + 256 * i;
       if (e < N2) l2[e] = r2[i];
     }
+    const float cs0 = A.cs[0], cs1 = A.cs[1], cs2 = A.cs[2];
+    for (int e = NI + tid; e < NXP * kLbcCin; e += 256) xin[e] = 0;
 #pragma unroll
     for (int i = 0; i < II; ++i) {
       const int e = tid + 256 * i;
-      if (e < NI) xin[e] = f2bf(rx[i]);
+      if (e < NI) {
+        const int row = along ? e % NX : e / kLbcCin, ci = along ? e / NX : e % kLbcCin;
+        const int c = (ci % 24) / kLbcBand;
+        xin[row * kLbcCin + ci] = f2bf(rx[i] * (c == 0 ? cs0 : c == 1 ? cs1 : cs2));
+      }
     }
   }
   __syncthreads();
@@ -337,15 +351,19 @@ __global__ __launch_bounds__(256) void k_lbc_step(LbcStep A) {
   const uint16_t *w1 = sw1;
   for (int t = wave; t < (NH / 16) * (kLbcHidden / 16); t += 4) {
     const int mt = t / (kLbcHidden / 16), nt = t % (kLbcHidden / 16);
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k = 0; k < kLbcTaps; ++k) {
-      const int row = 16 * mt + m + k;
-      for (int c0 = 0; c0 < kLbcCin; c0 += 16) {
-        const s4 av = row < NX ? ld4(xin + row * kLbcCin + c0 + 4 * g) : s4{0, 0, 0, 0};
-        const s4 bv = ld4(w1 + ((size_t)(16 * nt + m) * kLbcTaps + k) * kLbcCin + c0 + 4 * g);
-        acc = mfma(av, bv, acc);
+    // all operands of the tile first, then the MFMA chain: one LDS round trip per tile
+    constexpr int NC = kLbcCin / 16;
+    s4 av[kLbcTaps * NC], bv[kLbcTaps * NC];
+#pragma unroll
+    for (int k = 0; k < kLbcTaps; ++k)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        av[k * NC + c] = ld4(xin + (16 * mt + m + k) * kLbcCin + 16 * c + 4 * g);
+        bv[k * NC + c] = ld4(w1 + ((16 * nt + m) * kLbcTaps + k) * kLbcCin + 16 * c + 4 * g);
       }
-    }
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < kLbcTaps * NC; ++i) acc = mfma(av[i], bv[i], acc);
     for (int v = 0; v < 4; ++v) {
       const int hp = 16 * mt + 4 * g + v;
       const int a = a0 - R + hp;
@@ -358,28 +376,43 @@ __global__ __launch_bounds__(256) void k_lbc_step(LbcStep A) {
   const uint16_t *w2 = sw2;
   for (int t = wave; t < (kLbcSeg / 16) * (kLbcCout / 16); t += 4) {
     const int mt = t / (kLbcCout / 16), nt = t % (kLbcCout / 16);
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k = 0; k < kLbcTaps; ++k) {
-      const int row = 16 * mt + m + k;
-      for (int c0 = 0; c0 < kLbcHidden; c0 += 16) {
-        const s4 av = ld4(hid + row * kLbcHidden + c0 + 4 * g);
-        const s4 bv = ld4(w2 + ((size_t)(16 * nt + m) * kLbcTaps + k) * kLbcHidden + c0 + 4 * g);
-        acc = mfma(av, bv, acc);
+    constexpr int NC = kLbcHidden / 16;
+    s4 av[kLbcTaps * NC], bv[kLbcTaps * NC];
+#pragma unroll
+    for (int k = 0; k < kLbcTaps; ++k)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        av[k * NC + c] = ld4(hid + (16 * mt + m + k) * kLbcHidden + 16 * c + 4 * g);
+        bv[k * NC + c] = ld4(w2 + ((16 * nt + m) * kLbcTaps + k) * kLbcHidden + 16 * c + 4 * g);
       }
-    }
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < kLbcTaps * NC; ++i) acc = mfma(av[i], bv[i], acc);
     for (int v = 0; v < 4; ++v) outs[(16 * mt + 4 * g + v) * kLbcCout + 16 * nt + m] = acc[v];
   }
   __syncthreads();
 
-  const bool full = s < 2;
-  for (int e = tid; e < kLbcSeg * kLbcCout; e += 256) {
-    const int op = e / kLbcCout, co = e % kLbcCout;
+  static_assert(kLbcSeg * kLbcCout % 256 == 0, "scatter covers the outputs in whole passes");
+  float *const cur[3] = {st.cur[0], st.cur[1], st.cur[2]};
+  float *const comp[3] = {st.cur_comp[0], st.cur_comp[1], st.cur_comp[2]};
+  const float cs[3] = {A.cs[0], A.cs[1], A.cs[2]};
+#pragma unroll
+  for (int i = 0; i < kLbcSeg * kLbcCout / 256; ++i) {
+    const int e = tid + 256 * i;
+    const int op = along ? e % kLbcSeg : e / kLbcCout, co = along ? e / kLbcSeg : e % kLbcCout;
     const int a = a0 + op;
-    if (a >= G || (!full && (a < L || a >= G - L))) continue;
+    if (a >= G || (!along && (a < L || a >= G - L))) continue;
     const int c = co / kLbcDepth, d = co % kLbcDepth;
     const int cell = lbc_cell(s, d, a, G);
-    st.cur[c][cell] = outs[e] / A.cs[c];
-    if (st.cur_comp[c]) st.cur_comp[c][cell] = 0.f;
+    const float v = outs[op * kLbcCout + co];
+    // one branch per component with a compile-time index keeps the pointers in scalar
+    // registers (along the top and bottom rows c is uniform over each wave)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (c != k) continue;
+      cur[k][cell] = v / cs[k];
+      if (comp[k]) comp[k][cell] = 0.f;
+    }
   }
 }
 

@@ -11,6 +11,7 @@ Boundary lists are compiled into per-cell programs with the reference's semantic
 within one boundary all old values are read before any write and, for a repeated index, the
 last occurrence wins; the value is a scalar, ``signal[step]`` or one signal per point.
 """
+import collections
 import ctypes as ct
 
 import numpy as np
@@ -19,6 +20,28 @@ from . import _native as nat
 
 INT64_MAX = np.iinfo(np.int64).max
 OUTPUT_BYTES_PER_SEGMENT = 256 << 20
+# A run with recorded outputs is cut into about PIPELINE_SEGMENTS asynchronous runs (each
+# staging at most PIPELINE_STAGE_BYTES of outputs) so that the host builds the per-sample
+# lists of one while the device computes the next.
+PIPELINE_SEGMENTS = 8
+PIPELINE_STAGE_BYTES = 16 << 20
+
+
+def pipeline_sizes(n, seg, chunk, m):
+    """Step counts of the asynchronous runs of an ``n``-step run with ``m`` recorded values per
+    step (``seg``: the longest run the caller allows), or None for one synchronous run."""
+    unit = max(1, chunk)
+    if m == 0 or n < 4 * unit:
+        return None
+    size = -(-n // PIPELINE_SEGMENTS)
+    size = -(-size // unit) * unit                # whole graph chunks
+    size = min(size, seg, max(1, PIPELINE_STAGE_BYTES // (8 * m)))
+    if size >= n:
+        return None
+    sizes = [size] * (n // size)
+    if n % size:
+        sizes.append(n % size)
+    return sizes
 
 
 class BoundaryProgram:
@@ -243,6 +266,44 @@ class DeviceRunner:
             seg = min(seg, max(self.chunk, -(-num_steps // 20)))
         return max(1, min(seg, num_steps))
 
+    def _output_count(self):
+        return sum(c.size for c in self.out_cells if c is not None)
+
+    def _run_pipelined(self, field, sizes, progress):
+        """Run ``sizes`` steps after steps as asynchronous runs, the next one enqueued before
+        the outputs of the current one are appended (``fds_run_async`` / ``fds_run_take``)."""
+        lib, ctx, chunk = self.lib, self.ctx, int(self.chunk)
+        counts = [c.size if c is not None else 0 for c in self.out_cells]
+        queue = collections.deque()
+        start = [field.step]
+
+        def enqueue(k):
+            nat.check(lib.fds_run_async(ctx, start[0], k, chunk))
+            queue.append(k)
+            start[0] += k
+
+        def take():
+            k = queue.popleft()
+            data = [np.empty((k, m)) if m else None for m in counts]
+            nat.check(lib.fds_run_take(ctx, *[None if d is None else nat.dptr(d)
+                                               for d in data]))
+            self._append_outputs(field, data)
+            if progress is not None:
+                for s in range(field.step, field.step + k):
+                    progress.log(s)
+            field.step += k
+
+        try:
+            enqueue(sizes[0])
+            for k in sizes[1:]:
+                enqueue(k)
+                take()
+            take()
+        finally:
+            # after an error, keep the host in step with what the device ran
+            while queue:
+                take()
+
     def run_steps(self, field, start, nsteps):
         """Run ``nsteps`` steps from pyfds step ``start``; append the recorded outputs."""
         self.sync(field)
@@ -252,16 +313,26 @@ class DeviceRunner:
     def collect(self, field, nsteps):
         """After a device run of ``nsteps`` steps: append the recorded samples to the
         ``Output`` objects and mark the components as held by the device."""
-        comps = self._components(field)
-        for k, comp in enumerate(comps):
+        data = []
+        for k, comp in enumerate(self._components(field)):
+            m = self.out_cells[k].size if comp is not None else 0
+            d = None
+            if m:
+                d = np.empty((nsteps, m))
+                nat.check(self.lib.fds_read_outputs(self.ctx, k, 0, nsteps, nat.dptr(d)))
+            data.append(d)
+        self._append_outputs(field, data)
+
+    def _append_outputs(self, field, data):
+        """Append one run's recorded samples (``data[k]``: steps x outputs of component k, or
+        None) to the ``Output`` objects as ``write_outputs`` would have, step by step
+        (``pyfds/fields.py:602-611``), and mark the components as held by the device."""
+        for k, comp in enumerate(self._components(field)):
             if comp is None:
                 continue
-            m = self.out_cells[k].size
-            if m:
-                data = np.empty((nsteps, m))
-                nat.check(self.lib.fds_read_outputs(self.ctx, k, 0, nsteps, nat.dptr(data)))
+            if data[k] is not None:
                 for out, s0, cnt in self.out_layout[k]:
-                    cols = data[:, s0:s0 + cnt].T
+                    cols = data[k][:, s0:s0 + cnt].T
                     if not out.signals:
                         out.signals = [list(col) for col in cols]
                     else:
@@ -298,7 +369,10 @@ class DeviceRunner:
         valid = self._valid_steps()
         n_ok = num_steps if valid is None else max(0, min(num_steps, valid - start))
         seg = self._segment(num_steps, progress)
-        done = 0
+        sizes = pipeline_sizes(n_ok, seg, int(self.chunk), self._output_count())
+        if sizes:
+            self._run_pipelined(field, sizes, progress)
+        done = n_ok if sizes else 0
         while done < n_ok:
             k = min(seg, n_ok - done)
             self.run_steps(field, field.step, k)
