@@ -95,14 +95,16 @@ class BatchRunner:
             nxt[0] += k
 
         def take():
-            k = queue.popleft()
+            k = queue[0]
             data = [[np.empty((k, m)) if m else None for m in cs] for cs in counts]
             ptrs = (ct.c_void_p * (3 * len(data)))(
                 *[None if d is None else d.ctypes.data for ds in data for d in ds])
             nat.check(lib.fds_batch_take(handle, ptrs))
+            queue.popleft()
+            for f in self.fields:
+                f.step += k
             for f, r, ds in zip(self.fields, runners, data):
                 r._append_outputs(f, ds)
-                f.step += k
 
         try:
             enqueue(sizes[0])

@@ -283,15 +283,18 @@ class DeviceRunner:
             start[0] += k
 
         def take():
-            k = queue.popleft()
+            k = queue[0]
             data = [np.empty((k, m)) if m else None for m in counts]
             nat.check(lib.fds_run_take(ctx, *[None if d is None else nat.dptr(d)
                                                for d in data]))
+            # the device has run these steps: count them before anything else can fail
+            queue.popleft()
+            first = field.step
+            field.step += k
             self._append_outputs(field, data)
             if progress is not None:
-                for s in range(field.step, field.step + k):
+                for s in range(first, first + k):
                     progress.log(s)
-            field.step += k
 
         try:
             enqueue(sizes[0])
