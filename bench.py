@@ -31,7 +31,9 @@ Further sections (reported under ``lossy``, ``batched`` and ``thermal2d``):
 * batched: 256 Acoustic2D / Acoustic3DAxi instances, sides uniform in 128..1024 (seed 2),
   1-4 seeded inclusions each (c 1000-6000 m/s, rho 800-8000 kg/m^3), dt at CFL 0.5 on each
   instance's own c_max, a 1 MHz burst source at each centre and 16 recorded points, FP32 and
-  FP32-Kahan, one launch per step over all instances' tiles in HIP-graph chunks of 50.
+  FP32-Kahan, one launch per step over all instances' tiles in HIP-graph chunks of 50; each
+  form's pressure after the run against the same sweep in FP64 (the oracle's bits):
+  relative L2 per member (largest, median) and over all cells (``rel_l2_vs_fp64``).
 * learned_bc: Acoustic2D 1024x1024 FP32 interior + 16-cell learned layer (BF16-MFMA conv net,
   61 440 parameters) trained on plain 3072x3072 FP32 runs with random Ricker sources and
   inclusions (seeds 100..), judged on a held-out scene against the plain 3072^2 run:
@@ -244,6 +246,30 @@ def time_batched(ns, form, steps, warmup, chunk=50, fixed=None):
                tune_us_per_step=table)
     batch.close()
     return res, fields
+
+
+def batched_reference(ns, steps):
+    """Pressure of every member of the sweep after ``steps`` steps of the FP64 batch (bit for
+    bit the oracle's), against which the FP32 forms' error is measured."""
+    from pyfds_amd.batch import BatchRunner
+    fields = batched_fields(ns, 'fp64', steps=steps + 64)
+    batch = BatchRunner(fields, chunk=50)
+    batch.simulate(steps)
+    out = [np.array(f.pressure.values) for f in fields]
+    batch.close()
+    for f in fields:
+        f._runner().close()
+    return out
+
+
+def batched_error(fields, ref):
+    """Relative L2 error of each member's pressure against the FP64 reference: the largest,
+    the median and the one over all members' cells together."""
+    num = np.array([np.linalg.norm(np.asarray(f.pressure.values) - r) for f, r in zip(fields, ref)])
+    den = np.array([np.linalg.norm(r) for r in ref])
+    per = num / np.where(den > 0, den, 1.0)
+    return dict(max=float(per.max()), median=float(np.median(per)),
+                all_cells=float(np.sqrt((num ** 2).sum() / (den ** 2).sum())))
 
 
 def thermal_field(ns, form, steps, nx=4096, seed=3):
@@ -643,16 +669,23 @@ def main():
         line['acoustic1d'] = acoustic1d_section(ns, args.a1d_steps, args.no_cpu)
     if 'batched' in sections:
         bat = {}
+        # every member after the warm-up and the timed steps, FP64 (the oracle's bits)
+        ref = batched_reference(ns, args.batch_steps + 10)
         for f in [x for x in args.batch_forms.split(',') if x]:
-            r, _ = time_batched(ns, f, args.batch_steps, 10,
-                                fixed=None if args.batch_tile is None
-                                else (args.batch_tile, args.batch_strip))
+            r, flds = time_batched(ns, f, args.batch_steps, 10,
+                                   fixed=None if args.batch_tile is None
+                                   else (args.batch_tile, args.batch_strip))
             bat[f] = dict(value=round(r['gcells'], 3), ms_per_step=round(r['ms_per_step'], 4),
                           kernel_us=round(r['kernel_us'], 2),
                           achieved_gbs=round(r['achieved_gbs'], 1),
                           roofline_frac=round(r['achieved_gbs'] / HBM_PEAK_GBS, 4),
                           bytes_per_cell=BYTES_PER_CELL[f], cells=r['cells'],
-                          tile=r['tile'], autotune_us_per_step=r['tune_us_per_step'])
+                          tile=r['tile'], autotune_us_per_step=r['tune_us_per_step'],
+                          rel_l2_vs_fp64={k: float('%.3g' % v)
+                                          for k, v in batched_error(flds, ref).items()})
+            for fl in flds:
+                fl._runner().close()
+        del ref
         bat['config'] = ('256 Acoustic2D/Acoustic3DAxi instances, sides 128-1024, %d steps, '
                          'HIP-graph chunks of 50' % args.batch_steps)
         if not args.no_cpu:
