@@ -139,11 +139,15 @@ def tuned(fld, fixed=None):
         tile, chunk, table = autotune.tune_field(fld, steps=40, use_cache=False)
         table = {k: round(v, 2) for k, v in table.items()}
     info = nat.tile_table()[tile]
+    # steps per launch as the context runs them (lossy fields: one, whatever the shape)
+    runner = fld._runner()
+    runner.sync(fld)
+    spl = nat.check(runner.lib.fds_steps_per_launch(runner.ctx))
     return dict(tile=tile, strip=getattr(fld, 'strip_height', 0) or 0, chunk=chunk,
                 shape='%s %dx%d, %d step(s) per launch%s' % (
                     info['engine'], info['tx'], getattr(fld, 'strip_height', 0) or info['ty'],
-                    info['steps'], ', non-temporal' if info['nontemporal'] else ''),
-                steps_per_launch=info['steps'], table=table)
+                    spl, ', non-temporal' if info['nontemporal'] else ''),
+                steps_per_launch=spl, table=table)
 
 
 # HBM bytes per launch of the step kernel measured with rocprofv3 (FETCH_SIZE scaled x2 and
@@ -151,6 +155,7 @@ def tuned(fld, fixed=None):
 # in profiles/: (form, tile index, strip height) -> (FETCH_SIZE KiB, WRITE_SIZE KiB).
 MEASURED_TRAFFIC_KIB = {
     ('fp64', 14, 24): (109680.25, 99588.06),
+    ('fp64', 15, 24): (139184.53, 99810.06),
 }
 
 
@@ -176,7 +181,7 @@ def time_form(ns, wl, form, steps, warmup, fixed=None):
     ms = ct.c_double()
     runner.lib.fds_last_run_ms(runner.ctx, ct.byref(ms))
     cells = wl['nx'] * wl['ny']
-    kernel_s = ms.value / 1e3 / steps       # per step, last run segment = whole timed run
+    kernel_s = ms.value / 1e3 / steps       # per step, summed over the run's segments
     achieved = BYTES_PER_CELL[form] * cells / kernel_s / 1e9
     p = np.array(fld.pressure.values)
     return dict(gcells=cells * steps / wall / 1e9, wall_s=wall, ms_per_step=wall / steps * 1e3,

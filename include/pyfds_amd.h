@@ -180,6 +180,10 @@ int fds_set_line_shape(fds_ctx *ctx, int cells_per_thread, int threads, int ghos
 /* Engine that runs a context's steps: 0 LDS tiles, 1 row streaming, 2 resident 1D,
  * 3 nonlinear 1D tiles (no reference counterpart; for tests and benchmarks). */
 int fds_engine(fds_ctx *ctx);
+/* Time steps one launch of the context's 2D step kernel advances (the streaming shape's fused
+ * stage count, else 1; no reference counterpart): runs whose length is a multiple of it use
+ * fused launches only. */
+int fds_steps_per_launch(fds_ctx *ctx);
 
 /* Batched sweeps (no reference counterpart: pyfds runs one field per Python loop).  A batch
  * groups contexts that are fully set up (coefficients committed, state, boundaries, outputs)
@@ -195,6 +199,8 @@ int fds_batch_last_run_ms(fds_batch *b, double *ms);
  * three destinations per member (member-major: dst[3 * i + comp]); dst = null discards. */
 int fds_batch_run_async(fds_batch *b, int64_t start_step, int64_t nsteps, int chunk);
 int fds_batch_take(fds_batch *b, double *const *dst);
+/* Time steps one launch of the batch kernel advances (as fds_steps_per_launch). */
+int fds_batch_steps_per_launch(fds_batch *b);
 int fds_batch_destroy(fds_batch *b);
 
 /* Flow along x (AcousticFlow2D, pyfds/acoustic_flow.py:49-57 apply_flow): after every step n,

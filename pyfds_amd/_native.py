@@ -22,10 +22,11 @@ NUM_COEFF, NUM_SCALARS = 8, 8
 EXPORTS = ('fds_create', 'fds_destroy', 'fds_set_scalars', 'fds_set_coeff', 'fds_commit',
            'fds_set_state', 'fds_get_state', 'fds_set_boundaries', 'fds_set_outputs',
            'fds_run', 'fds_run_async', 'fds_run_take', 'fds_read_outputs', 'fds_set_tile', 'fds_tile_count', 'fds_tile_shape',
-           'fds_tile_engine', 'fds_tile_traits', 'fds_set_strip_height', 'fds_engine', 'fds_set_line_shape', 'fds_set_flow',
+           'fds_tile_engine', 'fds_tile_traits', 'fds_set_strip_height', 'fds_engine', 'fds_steps_per_launch', 'fds_set_line_shape', 'fds_set_flow',
            'fds_couple_loss_run',
            'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_batch_create', 'fds_batch_run',
-           'fds_batch_run_async', 'fds_batch_take', 'fds_batch_last_run_ms', 'fds_batch_destroy', 'fds_set_learned_boundary',
+           'fds_batch_run_async', 'fds_batch_take', 'fds_batch_steps_per_launch',
+           'fds_batch_last_run_ms', 'fds_batch_destroy', 'fds_set_learned_boundary',
            'fds_lbc_conv_fwd', 'fds_lbc_conv_bwd_data', 'fds_lbc_conv_bwd_weight',
            'fds_lbc_reduce_weight', 'fds_lbc_mse', 'fds_lbc_adam', 'fds_lbc_pack',
            'fds_lbc_gather', 'fds_lbc_band_rms', 'fds_window_sq_diff', 'fds_output_device_ptr',
@@ -108,6 +109,8 @@ def load():
         'fds_tile_traits': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         'fds_set_strip_height': (ct.c_int, [_p, ct.c_int]),
         'fds_engine': (ct.c_int, [_p]),
+        'fds_steps_per_launch': (ct.c_int, [_p]),
+        'fds_batch_steps_per_launch': (ct.c_int, [_p]),
         'fds_set_line_shape': (ct.c_int, [_p, ct.c_int, ct.c_int, ct.c_int]),
         'fds_couple_loss_run': (ct.c_int, [_p, _p, _dp, _dp, ct.c_double, ct.c_int64, ct.c_int,
                                            _dp, ct.c_int64, ct.c_int64]),

@@ -50,7 +50,7 @@ def _key(fields):
     lossy = any(getattr(x, '_coefs', {}).get(4) is not None for x in fields)
     cells = sum(x.num_points for x in fields)
     return '|'.join(str(v) for v in (_device_name(), type(f).__name__, f.form, len(fields),
-                                     cells, f._device_grid(), lossy, 'v3'))
+                                     cells, f._device_grid(), lossy, 'v4'))
 
 
 def _snapshot(fields):
@@ -73,8 +73,10 @@ def _restore(fields, snap):
         f.step = step
 
 
-# strip heights also tried for the 16-row streaming shapes (the others: their own height)
-STRIP_HEIGHTS = (6, 12, 24)
+# strip heights also tried for the 16-row and the 32-row streaming shapes (the others: their own
+# height); measured on 2048^2 acoustics, the time per step depends on the height irregularly
+# (power-of-two heights did worst), so the tuner times each
+STRIP_HEIGHTS = {16: (6, 12, 20, 24), 32: (12, 20, 24)}
 
 
 def tune_field(fld, steps=40, chunks=(50,), use_cache=True):
@@ -114,8 +116,7 @@ def _tune(fields, steps, chunks, use_cache, batch):
             if one_row or (lossy and batch):
                 continue
             cand.append((i, 0))
-            if ty == 16:
-                cand += [(i, h) for h in STRIP_HEIGHTS]
+            cand += [(i, h) for h in STRIP_HEIGHTS.get(ty, ())]
         else:
             cand.append((i, 0))
     table = {}

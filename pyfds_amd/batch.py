@@ -72,7 +72,8 @@ class BatchRunner:
         if valid and start + num_steps > min(valid):
             raise IndexError('a boundary signal ends at step {}'.format(min(valid)))
         sizes = pipeline_sizes(num_steps, num_steps, self.chunk,
-                               sum(r._output_count() for r in runners))
+                               sum(r._output_count() for r in runners),
+                               nat.check(self.lib.fds_batch_steps_per_launch(self.handle)))
         if sizes:
             self._run_pipelined(runners, start, sizes)
             return

@@ -835,6 +835,11 @@ int fds_engine(fds_ctx *c) {
   return c->streaming ? 1 : 0;
 }
 
+int fds_steps_per_launch(fds_ctx *c) {
+  if (int rc = check_ctx(c)) return rc;
+  return step_group(c);
+}
+
 int fds_tile_engine(int i) {
   if (i < 0 || i >= fds::kNumTiles + fds::kNumStream)
     return fail(FDS_E_ARG, "tile index %d out of range", i);
@@ -1873,6 +1878,11 @@ int fds_batch_last_run_ms(fds_batch *b, double *ms) {
   if (!b || !ms) return fail(FDS_E_ARG, "null argument");
   *ms = b->last_run_ms;
   return 0;
+}
+
+int fds_batch_steps_per_launch(fds_batch *b) {
+  if (!b) return fail(FDS_E_ARG, "null batch");
+  return b->fuse;
 }
 
 }  // extern "C"

@@ -27,10 +27,13 @@ PIPELINE_SEGMENTS = 8
 PIPELINE_STAGE_BYTES = 16 << 20
 
 
-def pipeline_sizes(n, seg, chunk, m):
+def pipeline_sizes(n, seg, chunk, m, group=1):
     """Step counts of the asynchronous runs of an ``n``-step run with ``m`` recorded values per
-    step (``seg``: the longest run the caller allows), or None for one synchronous run."""
-    unit = max(1, chunk)
+    step (``seg``: the longest run the caller allows), or None for one synchronous run.  Runs
+    are whole graph chunks of ``chunk`` steps rounded to ``group`` (steps per launch), so that
+    only the last run can end in shorter launches."""
+    group = max(1, group)
+    unit = max(group, chunk - chunk % group) if chunk > 0 else group
     if m == 0 or n < 4 * unit:
         return None
     size = -(-n // PIPELINE_SEGMENTS)
@@ -372,7 +375,8 @@ class DeviceRunner:
         valid = self._valid_steps()
         n_ok = num_steps if valid is None else max(0, min(num_steps, valid - start))
         seg = self._segment(num_steps, progress)
-        sizes = pipeline_sizes(n_ok, seg, int(self.chunk), self._output_count())
+        sizes = pipeline_sizes(n_ok, seg, int(self.chunk), self._output_count(),
+                               nat.check(self.lib.fds_steps_per_launch(self.ctx)))
         if sizes:
             self._run_pipelined(field, sizes, progress)
         done = n_ok if sizes else 0

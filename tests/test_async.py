@@ -40,6 +40,10 @@ def test_pipeline_sizes():
     assert device.pipeline_sizes(5000, 300, 50, 8) == [300] * 16 + [200]   # caller's segment
     # the staging cap: 16 MiB of outputs per run
     assert device.pipeline_sizes(100000, 100000, 1, 1 << 16) == [32] * 3125
+    # whole fused launches: chunks of 50 with three steps per launch are 48 steps
+    assert device.pipeline_sizes(2000, 2000, 50, 64, 3) == [288] * 6 + [272]
+    assert device.pipeline_sizes(210, 210, 50, 64, 3) == [48] * 4 + [18]
+    assert device.pipeline_sizes(150, 150, 0, 4, 2) == [20] * 7 + [10]
 
 
 # IdealGas1D's ideal-gas state law calls pow (device libm): its FP64 bits follow the

@@ -190,6 +190,21 @@ def test_fp64_lossy_stream_grids_match_oracle_bitwise(model, nx, ny, slot):
         np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
 
 
+@pytest.mark.parametrize('model,nx,ny,slot,strip', [('Acoustic2D', 190, 77, 7, 12),
+                                                    ('Acoustic3DAxi', 200, 97, 9, 20),
+                                                    ('Acoustic2D', 150, 71, 8, 6),
+                                                    ('Acoustic3DAxi', 170, 60, 1, 24),
+                                                    ('Acoustic2D', 260, 90, 3, 12)])
+def test_fp64_lossy_stream_strip_heights_match_oracle_bitwise(model, nx, ny, slot, strip):
+    idx = _stream_indices()[slot]
+    scene = _big_acoustic(nx, ny, 120, model=model, mu=3e-3)
+    ref = scenes.run_oracle(scene)
+    print('lossy grid', model, nx, ny, 'shape', idx, 'strip', strip, flush=True)
+    res = run_device(scene, tile=idx, strip_height=strip)
+    for k in ref:
+        np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
+
+
 @pytest.mark.parametrize('form', ['fp32', 'fp32_kahan'])
 @pytest.mark.parametrize('name', ['a2d_lossy', 'axi_lossy'])
 def test_fp32_lossy_stream_equals_tile_engine(name, form):
@@ -239,6 +254,29 @@ def _big_acoustic(nx, ny, steps, form='fp64', seed=1, model='Acoustic2D', mu=0.0
         plan=[('simulate', None)])
 
 
+# Strip heights other than the shape's own (the autotuner's candidates): one-, two- and
+# three-step shapes, partial strips at the bottom, ghost rows crossing strip boundaries.
+@pytest.mark.parametrize('model,nx,ny,slot,strip', [('Acoustic2D', 300, 130, 8, 6),
+                                                    ('Acoustic3DAxi', 250, 101, 8, 24),
+                                                    ('Acoustic2D', 190, 77, 9, 12),
+                                                    ('Acoustic2D', 333, 101, 9, 24),
+                                                    ('Acoustic3DAxi', 200, 97, 9, 20),
+                                                    ('Acoustic2D', 260, 90, 3, 12),
+                                                    ('Acoustic3DAxi', 270, 83, 6, 24),
+                                                    ('Acoustic2D', 130, 61, 2, 20),
+                                                    ('Acoustic2D', 150, 71, 7, 20),
+                                                    ('Acoustic3DAxi', 180, 95, 1, 6),
+                                                    ('Acoustic2D', 210, 85, 11, 12)])
+def test_fp64_stream_strip_heights_match_oracle_bitwise(model, nx, ny, slot, strip):
+    idx = _stream_indices()[slot]
+    scene = _big_acoustic(nx, ny, 120, model=model)
+    ref = scenes.run_oracle(scene)
+    print('grid', model, nx, ny, 'shape', idx, 'strip', strip, flush=True)
+    res = run_device(scene, tile=idx, strip_height=strip)
+    for k in ref:
+        np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
+
+
 @pytest.mark.parametrize('model,mu,nx,ny', [('Acoustic2D', 0.0, 256, 200),
                                             ('Acoustic3DAxi', 0.0, 300, 130),
                                             ('Acoustic3DAxi', 3e-3, 201, 77),
@@ -253,7 +291,8 @@ def test_fp64_multi_tile_grids_match_oracle_bitwise(model, mu, nx, ny):
 
 
 @pytest.mark.parametrize('tile', [None, 'stream0', 'stream2', 'stream3', 'stream6', 'stream8',
-                                  'stream9', 'stream11'])
+                                  'stream9', 'stream11', 'stream5/12', 'stream9/24',
+                                  'stream3/20'])
 @pytest.mark.parametrize('model', ['Thermal2D', 'Thermal3DAxi'])
 def test_fp64_thermal_multi_tile_matches_oracle_bitwise(model, tile):
     nx, ny, dx = 190, 150, 1e-3
@@ -274,7 +313,12 @@ def test_fp64_thermal_multi_tile_matches_oracle_bitwise(model, tile):
                  'heat_flux_y': [[nx * 70 + 3]]},
         plan=[('simulate', 77), ('simulate', 73)])
     ref = scenes.run_oracle(scene)
-    attrs = {} if tile is None else dict(tile=_stream_indices()[int(tile[6:])])
+    attrs = {}
+    if tile is not None:
+        slot, _, strip = tile[6:].partition('/')
+        attrs = dict(tile=_stream_indices()[int(slot)])
+        if strip:
+            attrs['strip_height'] = int(strip)
     res = run_device(scene, **attrs)
     for k in ref:
         np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
