@@ -88,6 +88,26 @@ def test_fp64_stream_batch_matches_oracle(slot):
             np.testing.assert_array_equal(got[k], ref[k], err_msg='%s %s' % (scene['name'], k))
 
 
+@pytest.mark.parametrize('slot,strip', [(9, 24), (8, 20), (9, 12), (5, 6)])
+def test_fp64_stream_batch_strip_heights_match_oracle(slot, strip):
+    """Batches on strip heights other than the shape's own (the autotuner's candidates)."""
+    members = []
+    for name in LOSSLESS:
+        scene = dict(scenes.SCENES[name](), plan=PLAN)
+        fld = scenes.build_field(pyfds_amd, pyfds_amd.regions, scene)
+        fld.strip_height = strip
+        members.append((scene, fld))
+    batch = BatchRunner([f for _, f in members], tile=_stream_index(slot), chunk=5)
+    for _, n in PLAN:
+        batch.simulate(n)
+    for scene, fld in members:
+        print('member', scene['name'], 'strip', strip, flush=True)
+        ref = scenes.run_oracle(scene)
+        got = scenes.collect(fld, scene)
+        for k in ref:
+            np.testing.assert_array_equal(got[k], ref[k], err_msg='%s %s' % (scene['name'], k))
+
+
 @pytest.mark.parametrize('slot', [0, 5, 9])
 def test_kahan_stream_batch_equals_single_runs(slot):
     tile = _stream_index(slot)

@@ -123,6 +123,18 @@ def test_fp64_stream_shapes_match_reference_golden(name, slot):
             np.testing.assert_array_equal(res[k], gold[k], err_msg='%s %s' % (name, k))
 
 
+@pytest.mark.parametrize('form,slot,strip', [('fp32', 8, 12), ('fp32', 8, 20), ('fp32', 9, 24),
+                                             ('fp32_kahan', 9, 20), ('fp32_kahan', 9, 12),
+                                             ('fp32_kahan', 8, 24)])
+def test_fp32_stream_strip_heights_equal_tile_engine(form, slot, strip):
+    """FP32 forms on strip heights other than the shape's own: the tile engine's bits."""
+    scene = _big_acoustic(210, 97, 120, model='Acoustic3DAxi' if slot == 9 else 'Acoustic2D')
+    a = run_device(scene, form=form, tile=1)
+    b = run_device(scene, form=form, tile=_stream_indices()[slot], strip_height=strip)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg='%s %d %d %s' % (form, slot, strip, k))
+
+
 @pytest.mark.parametrize('form', ['fp32', 'fp32_kahan'])
 @pytest.mark.parametrize('name', LOSSLESS_2D)
 def test_fp32_stream_engine_equals_tile_engine(name, form):
