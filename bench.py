@@ -601,8 +601,8 @@ def main():
     ap.add_argument('--batch-strip', type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print('bench.py measures one GPU only: this build has no multi-GPU path '
-              '(run with --gpus 1)', file=sys.stderr)
+        print('bench.py measures one GPU only: pyfds_amd runs each field on one GPU and has '
+              'no multi-GPU path (run with --gpus 1)', file=sys.stderr)
         sys.exit(2)
 
     import pyfds_amd as ns
