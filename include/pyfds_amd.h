@@ -145,7 +145,9 @@ int fds_read_outputs(fds_ctx *ctx, int comp, int64_t row0, int64_t nrows, double
  * fds_run_take waits for the oldest one and copies its outputs (nsteps rows of m values per
  * component, as fds_read_outputs) into dst0..dst2 (null entries and components without
  * outputs are skipped) and adds its time to fds_last_run_ms.  While runs are in flight the
- * setters, fds_run and fds_read_outputs fail with FDS_E_STATE; fds_get_state waits for them. */
+ * setters, fds_run, fds_read_outputs, fds_window_sq_diff and fds_output_device_ptr fail with
+ * FDS_E_STATE; fds_get_state waits for the context's own runs and fails while a batch run of
+ * the context is in flight. */
 int fds_run_async(fds_ctx *ctx, int64_t start_step, int64_t nsteps, int chunk);
 int fds_run_take(fds_ctx *ctx, double *dst0, double *dst1, double *dst2);
 

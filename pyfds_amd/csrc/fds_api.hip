@@ -1052,6 +1052,9 @@ int fds_set_state(fds_ctx *c, int comp, const double *data, int64_t n) {
 int fds_get_state(fds_ctx *c, int comp, double *data, int64_t n) {
   if (int rc = check_ctx(c)) return rc;
   if (int rc = check_comp(c, comp)) return rc;
+  // the context's own runs in flight are ordered before the copy on its stream; a batch run
+  // in flight is on the batch's stream
+  if (c->batch_flights) return fail(FDS_E_STATE, "a batch run of this context is in flight");
   if (n != c->n) return fail(FDS_E_ARG, "state holds %lld values", (long long)c->n);
   Comp &cp = c->comp[comp];
   const int b = (cp.buf[1].p && (cp.accumulated || c->family != kThermal || comp == 0)) ? c->parity : 0;
@@ -1932,8 +1935,9 @@ const void *current_state(const fds_ctx *c, int comp) {
 
 extern "C" int fds_window_sq_diff(fds_ctx *a, int comp_a, fds_ctx *b, int comp_b, int wx, int wy,
                                   int ax0, int ay0, int bx0, int by0, double *num, double *den) {
-  if (int rc = check_ctx(a)) return rc;
+  if (int rc = check_idle(a)) return rc;
   if (!b || b->device != a->device) return fail(FDS_E_ARG, "contexts must share a device");
+  if (int rc = check_idle(b)) return rc;
   if (int rc = check_comp(a, comp_a)) return rc;
   if (int rc = check_comp(b, comp_b)) return rc;
   if (wx < 1 || wy < 1 || ax0 < 0 || ay0 < 0 || bx0 < 0 || by0 < 0 || ax0 + wx > a->nx ||
@@ -1973,7 +1977,7 @@ extern "C" int fds_window_sq_diff(fds_ctx *a, int comp_a, fds_ctx *b, int comp_b
 }
 
 extern "C" int fds_output_device_ptr(fds_ctx *c, int comp, double **ptr, int64_t *rows, int64_t *m) {
-  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_idle(c)) return rc;
   if (int rc = check_comp(c, comp)) return rc;
   *ptr = c->comp[comp].obuf.as<double>();
   *rows = c->comp[comp].obuf_rows;

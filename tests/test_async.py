@@ -181,3 +181,32 @@ def test_pipelined_batch_equals_synchronous(form, monkeypatch):
             ref = scenes.run_oracle(dict(_scene(name), plan=[('simulate', 100)]))
             for k in ref:
                 np.testing.assert_array_equal(got[k], ref[k], err_msg='%s %s' % (name, k))
+
+
+def test_batch_async_abi_rules():
+    """While a batch run is in flight its members refuse their own runs, setters and state
+    reads (the batch runs on its own stream); taking it releases them."""
+    fields = []
+    for name in ('a2d_main', 'taxi'):
+        fld = _build(_scene(name), chunk=0)
+        fld.simulate(1)
+        fields.append(fld)
+    batch = BatchRunner(fields, chunk=0)
+    batch.simulate(1)
+    lib, handle = batch.lib, batch.handle
+    ctx0 = fields[0]._runner().ctx
+    assert lib.fds_batch_take(handle, None) == nat.E_STATE
+    assert lib.fds_batch_run_async(handle, 2, 10, 0) == 0
+    buf = np.empty(fields[0].num_points)
+    assert lib.fds_get_state(ctx0, 0, nat.dptr(buf), buf.size) == nat.E_STATE
+    assert lib.fds_run_async(ctx0, 12, 5, 0) == nat.E_STATE
+    assert lib.fds_run(ctx0, 12, 5, 0) == nat.E_STATE
+    assert lib.fds_set_tile(ctx0, 0) == nat.E_STATE
+    assert lib.fds_batch_run(handle, 12, 5, 0) == nat.E_STATE
+    assert lib.fds_batch_take(handle, None) == 0          # outputs discarded
+    assert lib.fds_get_state(ctx0, 0, nat.dptr(buf), buf.size) == 0
+    ms = ct.c_double()
+    assert lib.fds_batch_last_run_ms(handle, ct.byref(ms)) == 0 and ms.value > 0
+    batch.close()
+    for f in fields:
+        f._runner().close()
