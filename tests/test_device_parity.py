@@ -135,6 +135,16 @@ def test_fp32_stream_strip_heights_equal_tile_engine(form, slot, strip):
         np.testing.assert_array_equal(a[k], b[k], err_msg='%s %d %d %s' % (form, slot, strip, k))
 
 
+@pytest.mark.parametrize('name,slot,strip', [('t2d_blocks', 9, 20), ('t2d_aniso', 6, 24),
+                                             ('taxi', 9, 12)])
+def test_kahan_thermal_stream_strip_heights_equal_tile_engine(name, slot, strip):
+    scene = scenes.SCENES[name]()
+    a = run_device(scene, form='fp32_kahan', tile=1)
+    b = run_device(scene, form='fp32_kahan', tile=_stream_indices()[slot], strip_height=strip)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg='%s %d %d %s' % (name, slot, strip, k))
+
+
 @pytest.mark.parametrize('form', ['fp32', 'fp32_kahan'])
 @pytest.mark.parametrize('name', LOSSLESS_2D)
 def test_fp32_stream_engine_equals_tile_engine(name, form):
@@ -304,7 +314,7 @@ def test_fp64_multi_tile_grids_match_oracle_bitwise(model, mu, nx, ny):
 
 @pytest.mark.parametrize('tile', [None, 'stream0', 'stream2', 'stream3', 'stream6', 'stream8',
                                   'stream9', 'stream11', 'stream5/12', 'stream9/24',
-                                  'stream3/20'])
+                                  'stream3/20', 'stream6/24', 'stream9/20'])
 @pytest.mark.parametrize('model', ['Thermal2D', 'Thermal3DAxi'])
 def test_fp64_thermal_multi_tile_matches_oracle_bitwise(model, tile):
     nx, ny, dx = 190, 150, 1e-3
