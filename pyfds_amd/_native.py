@@ -21,7 +21,7 @@ NUM_COEFF, NUM_SCALARS = 8, 8
 
 EXPORTS = ('fds_create', 'fds_destroy', 'fds_set_scalars', 'fds_set_coeff', 'fds_commit',
            'fds_set_state', 'fds_get_state', 'fds_set_boundaries', 'fds_set_outputs',
-           'fds_run', 'fds_run_async', 'fds_run_take', 'fds_read_outputs', 'fds_set_tile', 'fds_tile_count', 'fds_tile_shape',
+           'fds_run', 'fds_run_async', 'fds_run_take', 'fds_prepare_run', 'fds_read_outputs', 'fds_set_tile', 'fds_tile_count', 'fds_tile_shape',
            'fds_tile_engine', 'fds_tile_traits', 'fds_set_strip_height', 'fds_engine', 'fds_steps_per_launch', 'fds_set_line_shape', 'fds_set_flow',
            'fds_couple_loss_run',
            'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_batch_create', 'fds_batch_run',
@@ -80,6 +80,7 @@ def load():
         'fds_run': (ct.c_int, [_p, _i64, _i64, ct.c_int]),
         'fds_run_async': (ct.c_int, [_p, _i64, _i64, ct.c_int]),
         'fds_run_take': (ct.c_int, [_p, _dp, _dp, _dp]),
+        'fds_prepare_run': (ct.c_int, [_p, _i64, _i64, ct.c_int]),
         'fds_read_outputs': (ct.c_int, [_p, ct.c_int, _i64, _i64, _dp]),
         'fds_set_tile': (ct.c_int, [_p, ct.c_int]),
         'fds_last_run_ms': (ct.c_int, [_p, _dp]),

@@ -1293,7 +1293,25 @@ int couple_run(fds_ctx *ac, fds_ctx *th, const double *rho, const double *loss, 
 
 // The steps of a prepared run (prepare_run) enqueued on the context's stream, between events
 // e0 and e1.
+// Capture the chunk graphs a run of nsteps steps replays (at most two: one per buffer parity
+// it starts a chunk on).  Capturing is set-up: it waits for the stream to be idle, then builds
+// the graph on the host.
+int capture_run_graphs(fds_ctx *c, int64_t nsteps, int chunk) {
+  if (c->line || chunk <= 0) return 0;
+  const int g = step_group(c);
+  const int ch = std::max(g, chunk - chunk % g);
+  int parity = c->parity;
+  for (int64_t left = nsteps, n = 0; left > ch && n < 2; left -= ch, ++n) {
+    hipGraphExec_t ex;
+    if (int rc = get_graph(c, ch, parity, &ex)) return rc;
+    parity ^= ((ch / g) & 1);
+  }
+  return 0;
+}
+
 int enqueue_steps(fds_ctx *c, int64_t nsteps, int chunk, hipEvent_t e0, hipEvent_t e1) {
+  // graphs first, so that the events bracket the steps only
+  if (int rc = capture_run_graphs(c, nsteps, chunk)) return rc;
   HIP_TRY(hipEventRecord(e0, c->stream));
   int64_t done = 0;
   if (c->line) {
@@ -1446,6 +1464,16 @@ int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_run_ms = ms;
+  return 0;
+}
+
+int fds_prepare_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
+  if (int rc = check_idle(c)) return rc;
+  if (!c->committed) return fail(FDS_E_STATE, "coefficients not committed");
+  if (nsteps < 1 || start < 0) return fail(FDS_E_ARG, "a run needs steps");
+  if (int rc = prepare_run(c, start, nsteps)) return rc;
+  if (int rc = capture_run_graphs(c, nsteps, chunk)) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
 }
 

@@ -392,6 +392,16 @@ class DeviceRunner:
             raise IndexError('index {} is out of bounds for axis 0 with size {}'
                              .format(start + n_ok, valid))
 
+    def prepare(self, field, num_steps):
+        """Do the one-time set-up of ``simulate(num_steps)`` from the field's current step
+        (tables, output rows, the graphs its runs replay) without stepping."""
+        self.sync(field)
+        seg = self._segment(num_steps, None)
+        sizes = pipeline_sizes(num_steps, seg, int(self.chunk), self._output_count(),
+                               nat.check(self.lib.fds_steps_per_launch(self.ctx)))
+        longest = max(sizes) if sizes else min(seg, num_steps)
+        nat.check(self.lib.fds_prepare_run(self.ctx, field.step, longest, int(self.chunk)))
+
     def single_step(self, field):
         """One ``sim_step`` at ``field.step`` (the caller advances the step counter)."""
         self.sync(field)

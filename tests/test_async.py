@@ -210,3 +210,21 @@ def test_batch_async_abi_rules():
     batch.close()
     for f in fields:
         f._runner().close()
+
+
+def test_prepare_run_changes_nothing():
+    """fds_prepare_run (tables, output rows, chunk graphs; no steps) leaves the field as it was:
+    the run that follows gives the same bits as one without it."""
+    scene = _scene('a2d_main')
+    res = []
+    for prep in (True, False):
+        fld = _build(scene)
+        fld.simulate(3)
+        if prep:
+            fld._runner().prepare(fld, 137)
+            assert fld.step == 3
+        fld.simulate(137)
+        res.append(scenes.collect(fld, scene))
+        fld._runner().close()
+    for k in res[0]:
+        np.testing.assert_array_equal(res[0][k], res[1][k], err_msg=k)
