@@ -149,12 +149,13 @@ int fds_read_outputs(fds_ctx *ctx, int comp, int64_t row0, int64_t nrows, double
  * FDS_E_STATE; fds_get_state waits for the context's own runs and fails while a batch run of
  * the context is in flight. */
 int fds_run_async(fds_ctx *ctx, int64_t start_step, int64_t nsteps, int chunk);
+int fds_run_take(fds_ctx *ctx, double *dst0, double *dst1, double *dst2);
+
 /* Everything a run of nsteps steps from start_step sets up before its first step, without
  * stepping: event tables, output rows, the chunk graphs it replays (no reference counterpart:
  * lets a caller keep one-time set-up out of a timed region).  A run of at most nsteps steps
  * that follows then starts at once. */
 int fds_prepare_run(fds_ctx *ctx, int64_t start_step, int64_t nsteps, int chunk);
-int fds_run_take(fds_ctx *ctx, double *dst0, double *dst1, double *dst2);
 
 /* Tile shape of the 2D step kernels (autotuner hook): index into the compiled shape table,
  * or -1 for the default.  fds_tile_count reports the table size, fds_tile_shape its entries
