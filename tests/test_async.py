@@ -31,21 +31,6 @@ def _scene(name):
     return scenes.SCENES[name]()
 
 
-def test_pipeline_sizes():
-    assert device.pipeline_sizes(2000, 2000, 50, 64) == [250] * 8
-    assert device.pipeline_sizes(1000, 1000, 50, 4096) == [150] * 6 + [100]
-    assert device.pipeline_sizes(150, 150, 5, 3) == [20] * 7 + [10]
-    assert device.pipeline_sizes(150, 150, 50, 3) is None          # under four graph chunks
-    assert device.pipeline_sizes(5000, 5000, 50, 0) is None        # nothing to overlap
-    assert device.pipeline_sizes(5000, 300, 50, 8) == [300] * 16 + [200]   # caller's segment
-    # the staging cap: 16 MiB of outputs per run
-    assert device.pipeline_sizes(100000, 100000, 1, 1 << 16) == [32] * 3125
-    # whole fused launches: chunks of 50 with three steps per launch are 48 steps
-    assert device.pipeline_sizes(2000, 2000, 50, 64, 3) == [288] * 6 + [272]
-    assert device.pipeline_sizes(210, 210, 50, 64, 3) == [48] * 4 + [18]
-    assert device.pipeline_sizes(150, 150, 0, 4, 2) == [20] * 7 + [10]
-
-
 # IdealGas1D's ideal-gas state law calls pow (device libm): its FP64 bits follow the
 # synchronous run of the same engine, and the oracle within relative L2 1e-10
 POW_SCENES = {'ig1d_cnl'}

@@ -342,3 +342,21 @@ def test_material_fill_of_rectangles_equals_scatter():
         g._fill(a, idx, 2.5)
         b[idx] = 2.5
         assert np.array_equal(a, b), (trial, kind)
+
+
+def test_pipeline_sizes():
+    """Segments of a pipelined run: whole graph chunks rounded to whole fused launches, about
+    eight of them, no more than the caller's segment or 16 MiB of outputs each."""
+    from pyfds_amd import device
+    assert device.pipeline_sizes(2000, 2000, 50, 64) == [250] * 8
+    assert device.pipeline_sizes(1000, 1000, 50, 4096) == [150] * 6 + [100]
+    assert device.pipeline_sizes(150, 150, 5, 3) == [20] * 7 + [10]
+    assert device.pipeline_sizes(150, 150, 50, 3) is None          # under four graph chunks
+    assert device.pipeline_sizes(5000, 5000, 50, 0) is None        # nothing to overlap
+    assert device.pipeline_sizes(5000, 300, 50, 8) == [300] * 16 + [200]   # caller's segment
+    # the staging cap: 16 MiB of outputs per run
+    assert device.pipeline_sizes(100000, 100000, 1, 1 << 16) == [32] * 3125
+    # whole fused launches: chunks of 50 with three steps per launch are 48 steps
+    assert device.pipeline_sizes(2000, 2000, 50, 64, 3) == [288] * 6 + [272]
+    assert device.pipeline_sizes(210, 210, 50, 64, 3) == [48] * 4 + [18]
+    assert device.pipeline_sizes(150, 150, 0, 4, 2) == [20] * 7 + [10]
