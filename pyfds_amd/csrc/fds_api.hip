@@ -793,6 +793,8 @@ int get_graph(fds_ctx *c, int chunk, int parity, hipGraphExec_t *out) {
   }
   c->graph_objs.push_back(g);
   c->graphs[key] = ex;
+  // the graph's device-side copy now, not on its first launch inside a run
+  HIP_TRY(hipGraphUpload(ex, c->stream));
   *out = ex;
   return 0;
 }
@@ -1473,6 +1475,12 @@ int fds_prepare_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
   if (nsteps < 1 || start < 0) return fail(FDS_E_ARG, "a run needs steps");
   if (int rc = prepare_run(c, start, nsteps)) return rc;
   if (int rc = capture_run_graphs(c, nsteps, chunk)) return rc;
+  // pinned staging of asynchronous runs of this length (both slots)
+  for (int slot = 0; slot < kMaxFlights; ++slot)
+    for (int k = 0; k < 3; ++k) {
+      const size_t bytes = c->comp[k].out_cells.size() * (size_t)nsteps * 8;
+      if (bytes) HIP_TRY(c->comp[k].stage[slot].ensure(bytes));
+    }
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1693,6 +1701,7 @@ int batch_graph(fds_batch *b, int chunk, int rel_parity, hipGraphExec_t *out) {
   }
   b->graph_objs.push_back(g);
   b->graphs[key] = ex;
+  HIP_TRY(hipGraphUpload(ex, b->stream));
   *out = ex;
   return 0;
 }
