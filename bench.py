@@ -664,7 +664,14 @@ def main():
             'algorithmic_bytes_per_launch': BYTES_PER_CELL[main_form] * NX * NY *
             ref['tune']['steps_per_launch'],
             'traffic': 'HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/), '
-                       'null unless this shape was profiled'},
+                       'null unless this shape was profiled',
+            'physical_gbs': (None if measured_traffic(main_form, ref['tune']) is None else
+                             round(measured_traffic(main_form, ref['tune']) /
+                                   (ref['kernel_us'] * ref['tune']['steps_per_launch']) / 1e3,
+                                   1)),
+            'note': 'achieved counts the algorithmic bytes of every fused step; a launch of '
+                    'several steps makes one pass over HBM, so achieved may pass the peak '
+                    'while physical_gbs (measured traffic / launch time) stays below it'},
         'cpu_baseline': cpu,
         'forms': out_forms,
     }
