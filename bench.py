@@ -147,27 +147,148 @@ def tuned(fld, fixed=None):
     return dict(tile=tile, strip=getattr(fld, 'strip_height', 0) or 0, chunk=chunk,
                 shape='%s %dx%d, %d step(s) per launch%s' % (
                     info['engine'], info['tx'], getattr(fld, 'strip_height', 0) or info['ty'],
-                    spl, ', non-temporal' if info['nontemporal'] else ''),
+                    spl, ', non-temporal' if info['nontemporal'] else '')
+                + (', prefetch %d rows' % info['prefetch'] if info['prefetch'] > 1 else ''),
                 steps_per_launch=spl, table=table)
 
 
-# HBM bytes per launch of the step kernel measured with rocprofv3 (FETCH_SIZE scaled x2 and
-# WRITE_SIZE x1 after calibration against a plain copy kernel) for the configurations profiled
-# in profiles/: (form, tile index, strip height) -> (FETCH_SIZE KiB, WRITE_SIZE KiB).
-MEASURED_TRAFFIC_KIB = {
-    ('fp64', 14, 24): (109680.25, 99588.06),
-    ('fp64', 15, 24): (139184.53, 99810.06),
-}
+COPY_CAL_BYTES = 1 << 30      # plain copy used to scale the PMC byte counters
+COPY_CAL_VARIANT = 0          # fds_stream_copy_bench variant (grid-stride copy)
 
 
-def measured_traffic(form, tune):
-    kib = MEASURED_TRAFFIC_KIB.get((form, tune['tile'], tune['strip']))
-    return None if kib is None else int(round((2.0 * kib[0] + kib[1]) * 1024))
+def _pmc_rows(outdir):
+    """(kernel name, counter value) of every dispatch in a rocprofv3 counter CSV."""
+    import csv
+    import glob
+    rows = []
+    for path in glob.glob(os.path.join(outdir, '**', '*counter_collection.csv'), recursive=True):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                rows.append((r.get('Kernel_Name', ''), float(r.get('Counter_Value', 'nan'))))
+    return rows
+
+
+def measure_traffic(kind, form, tile, strip, launches=20):
+    """HBM bytes per launch of the step kernel of ``kind`` ('acoustic2d' or 'thermal') at the
+    shape ``tile`` / ``strip``: two rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE) of this
+    script's --traffic-probe mode, which runs ``launches`` full launches after a 1 GiB plain
+    copy; each counter is scaled by what it reads for that copy (known bytes).  Returns a dict
+    (bytes per launch, the raw KiB and the scale factors) or None when rocprofv3 is missing or
+    a run fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which('rocprofv3') or '/opt/rocm/bin/rocprofv3'
+    if not os.path.exists(prof):
+        return None
+    family = {'acoustic2d': 'k_stream_acoustic<', 'thermal': 'k_stream_thermal<'}[kind]
+    out = {}
+    for counter in ('FETCH_SIZE', 'WRITE_SIZE'):
+        d = tempfile.mkdtemp(prefix='pyfds_pmc_', dir='/tmp')
+        cmd = ['timeout', '-s', 'KILL', '150', prof, '--pmc', counter, '-d', d, '-o', 'pmc',
+               '--output-format', 'csv', '--', sys.executable, os.path.abspath(__file__),
+               '--traffic-probe', '%s:%s:%d:%d:%d' % (kind, form, tile, strip, launches)]
+        env = dict(os.environ, TMPDIR='/tmp')
+        res = subprocess.run(cmd, cwd='/tmp', env=env, capture_output=True, text=True)
+        rows = _pmc_rows(d)
+        shutil.rmtree(d, ignore_errors=True)
+        # rocprofv3 may end with a signal after the program finished: use complete results
+        if 'traffic-probe done' not in res.stdout or not rows:
+            print('traffic probe (%s) failed: rc %d %s' % (counter, res.returncode,
+                                                          res.stderr[-400:]), file=sys.stderr)
+            return None
+        copy = [v for k, v in rows if 'k_stream_copy' in k]
+        kern = {}
+        for k, v in rows:
+            if family in k:
+                kern.setdefault(k, []).append(v)
+        if not copy or not kern:
+            return None
+        name, vals = max(kern.items(), key=lambda kv: len(kv[1]))
+        scale = COPY_CAL_BYTES / (float(np.median(copy)) * 1024.0)
+        out[counter] = dict(kib=float(np.median(vals)), scale=round(scale, 4), kernel=name,
+                            dispatches=len(vals))
+    fetch, write = out['FETCH_SIZE'], out['WRITE_SIZE']
+    read_b = fetch['kib'] * 1024 * fetch['scale']
+    write_b = write['kib'] * 1024 * write['scale']
+    return dict(bytes=int(round(read_b + write_b)), read_bytes=int(round(read_b)),
+                write_bytes=int(round(write_b)), fetch_kib=round(fetch['kib'], 1),
+                write_kib=round(write['kib'], 1), fetch_scale=fetch['scale'],
+                write_scale=write['scale'], kernel=fetch['kernel'],
+                dispatches=fetch['dispatches'])
+
+
+def traffic_probe(spec):
+    """Child of measure_traffic (runs under rocprofv3): a 1 GiB calibration copy, then
+    ``launches`` full launches of the step kernel at the given shape."""
+    import ctypes as ct
+    import pyfds_amd as ns
+    from pyfds_amd import _native as nat
+    kind, form, tile, strip, launches = spec.split(':')
+    tile, strip, launches = int(tile), int(strip), int(launches)
+    lib = nat.load()
+    gbs, ms = ct.c_double(), ct.c_double()
+    nat.check(lib.fds_stream_copy_bench(COPY_CAL_BYTES, 2, COPY_CAL_VARIANT, ct.byref(gbs),
+                                        ct.byref(ms)))
+    if kind == 'acoustic2d':
+        fld = build_field(ns, workload(steps=400), form, t_samples=400)
+    else:
+        fld = thermal_field(ns, form, 400)
+    fld.tile, fld.strip_height, fld.graph_chunk = tile, strip, 50
+    fld.assemble_matrices()
+    runner = fld._runner()
+    runner.sync(fld)
+    spl = nat.check(lib.fds_steps_per_launch(runner.ctx))
+    fld.simulate(spl * launches)
+    print('traffic-probe done', flush=True)
+
+
+def launch_timing(fld, launches=30):
+    """Average duration (us) of one full launch of the field's step kernel: HIP events on the
+    library's stream around ``launches`` launches (graph-replayed, run set up beforehand)."""
+    import ctypes as ct
+    runner = fld._runner()
+    runner.sync(fld)
+    spl = runner.lib.fds_steps_per_launch(runner.ctx)
+    n = spl * launches
+    runner.prepare(fld, n)
+    fld.simulate(n)
+    ms = ct.c_double()
+    runner.lib.fds_last_run_ms(runner.ctx, ct.byref(ms))
+    return ms.value * 1e3 / launches, spl
+
+
+def roofline_entry(bytes_per_update, cells, launch_us, spl, traffic):
+    """The roofline object: physical HBM bytes per launch (measured) over the launch time
+    against 8 TB/s; without a measurement the one-pass compulsory bytes (a lower bound of the
+    physical traffic).  Detail: the compulsory and the per-step algorithmic rates."""
+    compulsory = bytes_per_update * cells            # state r+w + coefficients, one pass
+    algorithmic = bytes_per_update * cells * spl     # the per-step figure x cell updates
+    detail = dict(launch_us=round(launch_us, 3), steps_per_launch=spl,
+                  compulsory_bytes_per_launch=compulsory,
+                  compulsory_gbs=round(compulsory / launch_us / 1e3, 1),
+                  compulsory_frac=round(compulsory / launch_us / 1e3 / HBM_PEAK_GBS, 4),
+                  algorithmic_bytes_per_launch=algorithmic,
+                  algorithmic_gbs=round(algorithmic / launch_us / 1e3, 1),
+                  algorithmic_frac=round(algorithmic / launch_us / 1e3 / HBM_PEAK_GBS, 4))
+    if traffic is not None:
+        phys = traffic['bytes'] / launch_us / 1e3
+        detail.update(traffic_over_compulsory=round(traffic['bytes'] / compulsory, 3),
+                      pmc=traffic)
+        roof = dict(bound='hbm', achieved=round(phys, 1), peak=HBM_PEAK_GBS, unit='GB/s',
+                    frac=round(phys / HBM_PEAK_GBS, 4), traffic=traffic['bytes'],
+                    basis='measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, '
+                          'copy-calibrated) / launch time')
+    else:
+        roof = dict(bound='hbm', achieved=detail['compulsory_gbs'], peak=HBM_PEAK_GBS,
+                    unit='GB/s', frac=detail['compulsory_frac'], traffic=None,
+                    basis='one-pass compulsory bytes / launch time (traffic not measured)')
+    return roof, detail
 
 
 def time_form(ns, wl, form, steps, warmup, fixed=None):
     import torch
-    fld = build_field(ns, wl, form, t_samples=steps + warmup + 50)
+    fld = build_field(ns, wl, form, t_samples=steps + warmup + 200)
     fld.assemble_matrices()
     tune = tuned(fld, fixed)
     if warmup:
@@ -186,18 +307,18 @@ def time_form(ns, wl, form, steps, warmup, fixed=None):
     kernel_s = ms.value / 1e3 / steps       # per step, summed over the run's segments
     achieved = BYTES_PER_CELL[form] * cells / kernel_s / 1e9
     p = np.array(fld.pressure.values)
+    launch_us, _ = launch_timing(fld)
     return dict(gcells=cells * steps / wall / 1e9, wall_s=wall, ms_per_step=wall / steps * 1e3,
-                kernel_us=kernel_s * 1e6, achieved_gbs=achieved, p=p, tune=tune)
+                kernel_us=kernel_s * 1e6, achieved_gbs=achieved, p=p, tune=tune,
+                launch_us=launch_us)
 
 
-def batched_fields(ns, form, n_inst=256, seed=2, steps=1000):
-    """The 256-instance parameter sweep (see module docstring)."""
+def batched_params(n_inst=256, seed=2):
+    """Per-instance parameters of the sweep: (nx, ny, axi, inclusions, probe seed)."""
     rng = np.random.default_rng(seed)
-    dx = 1e-4
-    fields = []
+    out = []
     for i in range(n_inst):
         nx, ny = (int(v) for v in rng.integers(128, 1025, size=2))
-        cls = ns.Acoustic2D if i % 2 == 0 else ns.Acoustic3DAxi
         incl = []
         for _ in range(int(rng.integers(1, 5))):
             w = int(rng.integers(max(2, nx // 10), max(3, nx // 3)))
@@ -205,23 +326,35 @@ def batched_fields(ns, form, n_inst=256, seed=2, steps=1000):
             x0, y0 = int(rng.integers(0, nx - w)), int(rng.integers(0, ny - h))
             incl.append((x0, y0, w, h, float(rng.uniform(1000, 6000)),
                          float(rng.uniform(800, 8000))))
-        cmax = max([1500.0] + [c for *_, c, _ in incl])
-        dt = 0.5 * dx / cmax
-        fld = cls(x_samples=nx, x_delta=dx, y_samples=ny, y_delta=dx, t_samples=steps,
-                  t_delta=dt, material=ns.AcousticMaterial(1500.0, 1000.0))
-        fld.form = form
-        for x0, y0, w, h, c, rho in incl:
-            xs, ys = np.arange(x0, x0 + w), np.arange(y0, y0 + h)
-            fld.add_material_region(ns.regions.RectRegion(
-                (xs[:, None] + ys[None, :] * nx).ravel(), (x0, y0, w, h)),
-                ns.AcousticMaterial(c, rho))
-        t = np.arange(steps + 64) * dt
-        fld.pressure.add_boundary(ns.regions.PointRegion([nx // 2 + nx * (ny // 2)], None),
-                                  value=np.sin(2 * np.pi * 1e6 * t) * (t < 5e-6), additive=True)
-        fld.pressure.add_output(ns.regions.Region(
-            [int(v) for v in rng.choice(nx * ny, 16, replace=False)]))
-        fields.append(fld)
-    return fields
+        probes = [int(v) for v in rng.choice(nx * ny, 16, replace=False)]
+        out.append((nx, ny, i % 2 == 1, incl, probes))
+    return out
+
+
+def batched_field(ns, form, params, steps):
+    nx, ny, axi, incl, probes = params
+    dx = 1e-4
+    cmax = max([1500.0] + [c for *_, c, _ in incl])
+    dt = 0.5 * dx / cmax
+    cls = ns.Acoustic3DAxi if axi else ns.Acoustic2D
+    fld = cls(x_samples=nx, x_delta=dx, y_samples=ny, y_delta=dx, t_samples=steps,
+              t_delta=dt, material=ns.AcousticMaterial(1500.0, 1000.0))
+    fld.form = form
+    for x0, y0, w, h, c, rho in incl:
+        xs, ys = np.arange(x0, x0 + w), np.arange(y0, y0 + h)
+        fld.add_material_region(ns.regions.RectRegion(
+            (xs[:, None] + ys[None, :] * nx).ravel(), (x0, y0, w, h)),
+            ns.AcousticMaterial(c, rho))
+    t = np.arange(steps + 64) * dt
+    fld.pressure.add_boundary(ns.regions.PointRegion([nx // 2 + nx * (ny // 2)], None),
+                              value=np.sin(2 * np.pi * 1e6 * t) * (t < 5e-6), additive=True)
+    fld.pressure.add_output(ns.regions.Region(probes))
+    return fld
+
+
+def batched_fields(ns, form, n_inst=256, seed=2, steps=1000):
+    """The 256-instance parameter sweep (see module docstring)."""
+    return [batched_field(ns, form, p, steps) for p in batched_params(n_inst, seed)]
 
 
 def time_batched(ns, form, steps, warmup, chunk=50, fixed=None):
@@ -301,7 +434,7 @@ def thermal_field(ns, form, steps, nx=4096, seed=3):
 
 def time_thermal(ns, form, steps, warmup, fixed=None):
     import torch
-    fld = thermal_field(ns, form, steps + warmup + 50)
+    fld = thermal_field(ns, form, steps + warmup + 200)
     fld.assemble_matrices()
     tune = tuned(fld, fixed)
     if warmup:
@@ -319,9 +452,11 @@ def time_thermal(ns, form, steps, warmup, fixed=None):
     cells = fld.num_points
     kernel_s = ms.value / 1e3 / steps
     achieved = THERMAL_BYTES_PER_CELL[form] * cells / kernel_s / 1e9
+    t = np.array(fld.temperature.values)
+    launch_us, _ = launch_timing(fld)
     return dict(gcells=cells * steps / wall / 1e9, ms_per_step=wall / steps * 1e3,
-                kernel_us=kernel_s * 1e6, achieved_gbs=achieved,
-                t=np.array(fld.temperature.values), tune=tune)
+                kernel_us=kernel_s * 1e6, achieved_gbs=achieved, t=t, tune=tune,
+                launch_us=launch_us, cells=cells)
 
 
 A1D_N, A1D_STEPS = 2000, 5000
@@ -399,8 +534,8 @@ def acoustic1d_section(ns, steps, no_cpu):
     if not no_cpu:
         out['cpu_baseline'] = dict(value=round(cpu_baseline_acoustic1d(steps), 5),
                                    unit='Gcell-updates/s', cores=1, kind='port',
-                                   sample='numpy oracle, the same Acoustic1D run, all %d steps'
-                                          % steps)
+                                   sample='oracle (scipy dia_matrix products), the same '
+                                          'Acoustic1D run, all %d steps' % steps)
     return out
 
 
@@ -442,6 +577,7 @@ def thermoacoustic_section(ns, steps, host_steps):
 
 def cpu_baseline_acoustic1d(steps):
     import oracle
+    oracle.Dia.use_scipy = True
     n = A1D_N
     o = oracle.PlainAcoustic1D(n, 1e-4, steps, 0.5 * 1e-4 / 1500.0,
                                [(np.arange(n), dict(sound_velocity=1500.0, density=1000.0,
@@ -457,35 +593,60 @@ def cpu_baseline_acoustic1d(steps):
     return n * steps / (time.perf_counter() - t0) / 1e9
 
 
-def cpu_baseline_batched(steps, n_inst=32):
-    """Oracle over the first ``n_inst`` instances of the sweep, ``steps`` steps each."""
+def _oracle_of(f, steps):
+    """The oracle of one sweep member (same materials, source and probes)."""
+    import oracle
+    mats = [(np.asarray(m.region.indices), dict(sound_velocity=m.materials[0].sound_velocity,
+                                                 density=m.materials[0].density,
+                                                 absorption_coef=0))
+            for m in f.material_regions]
+    cls = oracle.PlainAcoustic2D if type(f).__name__ == 'Acoustic2D' \
+        else oracle.PlainAcoustic3DAxi
+    o = cls(f.x.samples, f.x.increment, steps, f.t.increment, mats, ny=f.y.samples,
+            dy=f.y.increment)
+    b = f.pressure.boundaries[0]
+    o.boundaries['pressure'] = [dict(indices=b.region.indices, value=b.value, additive=True)]
+    o.outputs['pressure'] = [dict(indices=f.pressure.outputs[0].region.indices, signals=[])]
+    o.assemble()
+    return o
+
+
+def _batched_worker(job):
+    """Pool worker: step the sweep members ``ids`` on the oracle; (cell updates, seconds)."""
+    ids, steps = job
     import oracle
     import pyfds_amd as ns
-    fields = batched_fields(ns, 'fp64', steps=steps)[:n_inst]
-    cells, wall = 0, 0.0
-    for f in fields:
-        n = f.num_points
-        mats = [(np.asarray(m.region.indices), dict(sound_velocity=m.materials[0].sound_velocity,
-                                                     density=m.materials[0].density,
-                                                     absorption_coef=0))
-                for m in f.material_regions]
-        cls = oracle.PlainAcoustic2D if type(f).__name__ == 'Acoustic2D' \
-            else oracle.PlainAcoustic3DAxi
-        o = cls(f.x.samples, f.x.increment, steps, f.t.increment, mats, ny=f.y.samples,
-                dy=f.y.increment)
-        b = f.pressure.boundaries[0]
-        o.boundaries['pressure'] = [dict(indices=b.region.indices, value=b.value, additive=True)]
-        o.outputs['pressure'] = [dict(indices=f.pressure.outputs[0].region.indices, signals=[])]
-        o.assemble()
+    oracle.Dia.use_scipy = True
+    params = batched_params()
+    cells, busy = 0, 0.0
+    for i in ids:
+        o = _oracle_of(batched_field(ns, 'fp64', params[i], steps), steps)
         t0 = time.perf_counter()
         o.simulate(steps)
-        wall += time.perf_counter() - t0
-        cells += n * steps
-    return cells / wall / 1e9
+        busy += time.perf_counter() - t0
+        cells += o.n * steps
+    return cells, busy
+
+
+def cpu_baseline_batched(steps, n_inst=32):
+    """Oracle over the first ``n_inst`` instances of the sweep, ``steps`` steps each, one core."""
+    cells, busy = _batched_worker((list(range(n_inst)), steps))
+    return cells / busy / 1e9
+
+
+def cpu_baseline_batched_pool(steps, workers):
+    """All instances of the sweep, ``steps`` steps each, on a pool of ``workers`` processes
+    (one per core; instances are independent): cell updates over the busiest worker's time."""
+    import multiprocessing as mp
+    jobs = [(list(range(w, 256, workers)), steps) for w in range(workers)]
+    with mp.get_context('spawn').Pool(workers) as pool:
+        res = pool.map(_batched_worker, jobs)
+    return sum(c for c, _ in res) / max(b for _, b in res) / 1e9
 
 
 def cpu_baseline_thermal(steps):
     import oracle
+    oracle.Dia.use_scipy = True
     import pyfds_amd as ns
     f = thermal_field(ns, 'fp64', steps)
     mats = [(np.asarray(m.region.indices), dict(heat_capacity=m.materials[0].heat_capacity,
@@ -533,12 +694,14 @@ def learned_section(no_cpu, runs=3, iters=1500):
     if not no_cpu:
         out['cpu_baseline'] = dict(value=round(cpu_baseline_enlarged(lb, scene, 3), 5),
                                    unit='Gcell-updates/s', cores=1, kind='port',
-                                   sample='numpy oracle, plain 3072x3072 enlarged run, 3 steps')
+                                   sample='oracle (scipy dia_matrix products), plain 3072x3072 enlarged '
+                                          'run, 3 steps')
     return out
 
 
 def cpu_baseline_enlarged(lb, scene, steps):
     import oracle
+    oracle.Dia.use_scipy = True
     f = lb.field(scene, lb.big, lb.off, steps)
     mats = [(np.asarray(m.region.indices), dict(sound_velocity=m.materials[0].sound_velocity,
                                                  density=m.materials[0].density,
@@ -553,9 +716,31 @@ def cpu_baseline_enlarged(lb, scene, steps):
     return lb.big ** 2 * steps / (time.perf_counter() - t0) / 1e9
 
 
+def cpu_host():
+    """The host the CPU baseline ran on: model, logical CPUs, the affinity mask's size and the
+    threads the baseline used (numpy's element-wise operations and scipy's dia_matvec are
+    single-threaded; no BLAS call is made)."""
+    model = None
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = sorted(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else None
+    return dict(model=model, cpu_count=os.cpu_count(),
+                affinity=len(aff) if aff is not None else None,
+                affinity_first=aff[:4] if aff else None, threads=1,
+                omp_num_threads=os.environ.get('OMP_NUM_THREADS'))
+
+
 def cpu_baseline(wl, steps):
-    """The numpy oracle (plain FP64 method) on the same workload, single core."""
+    """The oracle (plain FP64 method, scipy dia_matrix products as the reference uses) on the
+    same workload, single core."""
     import oracle
+    oracle.Dia.use_scipy = True
     n = wl['nx'] * wl['ny']
     x0, y0, w, h = wl['incl']
     xs = np.arange(x0, x0 + w)
@@ -595,7 +780,7 @@ def main():
     ap.add_argument('--batch-steps', type=int, default=1000)
     ap.add_argument('--batch-forms', default='fp32,fp32_kahan')
     ap.add_argument('--thermal-steps', type=int, default=2000)
-    ap.add_argument('--thermal-forms', default='fp64,fp32_kahan')
+    ap.add_argument('--thermal-forms', default='fp64,fp32,fp32_kahan')
     ap.add_argument('--tile', type=int, default=None,
                     help='Acoustic2D section: use this shape index instead of autotuning')
     ap.add_argument('--strip', type=int, default=0,
@@ -604,14 +789,20 @@ def main():
     ap.add_argument('--thermal-strip', type=int, default=0)
     ap.add_argument('--batch-tile', type=int, default=None)
     ap.add_argument('--batch-strip', type=int, default=0)
+    ap.add_argument('--no-traffic', action='store_true',
+                    help='skip the rocprofv3 HBM traffic measurement of the chosen shapes')
+    ap.add_argument('--traffic-probe', default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(args.traffic_probe)
+        return
     if args.gpus != 1:
         print('bench.py measures one GPU only: pyfds_amd runs each field on one GPU and has '
               'no multi-GPU path (run with --gpus 1)', file=sys.stderr)
         sys.exit(2)
 
     import pyfds_amd as ns
-    wl = workload(steps=args.steps + args.warmup)
+    wl = workload(steps=args.steps + args.warmup + 200)
     forms = [f for f in args.forms.split(',') if f]
     fixed = None if args.tile is None else (args.tile, args.strip)
     res = {f: time_form(ns, wl, f, args.steps, args.warmup, fixed) for f in forms}
@@ -620,24 +811,33 @@ def main():
     out_forms = {}
     for f, r in res.items():
         spl = r['tune']['steps_per_launch']
+        cells = NX * NY
         entry = dict(value=round(r['gcells'], 3), ms_per_step=round(r['ms_per_step'], 4),
-                     kernel_us=round(r['kernel_us'], 2),
-                     launch_us=round(r['kernel_us'] * spl, 2),
-                     roofline_frac=round(r['achieved_gbs'] / HBM_PEAK_GBS, 4),
+                     kernel_us=round(r['kernel_us'], 2), launch_us=round(r['launch_us'], 2),
+                     compulsory_frac=round(BYTES_PER_CELL[f] * cells / r['launch_us'] / 1e3
+                                           / HBM_PEAK_GBS, 4),
+                     algorithmic_frac=round(r['achieved_gbs'] / HBM_PEAK_GBS, 4),
                      bytes_per_cell=BYTES_PER_CELL[f], autotune=r['tune'])
-        tr = measured_traffic(f, r['tune'])
-        if tr is not None:
-            entry['hbm_bytes_per_step'] = tr // spl
         if f != main_form and main_form == 'fp64':
             den = np.linalg.norm(ref['p'])
             entry['rel_l2_vs_fp64'] = float(np.linalg.norm(r['p'] - ref['p']) / den)
         out_forms[f] = entry
+    traffic = None if args.no_traffic else measure_traffic(
+        'acoustic2d', main_form, ref['tune']['tile'], ref['tune']['strip'])
+    roof, detail = roofline_entry(BYTES_PER_CELL[main_form], NX * NY, ref['launch_us'],
+                                  ref['tune']['steps_per_launch'], traffic)
+    detail['kernel'] = ref['tune']['shape']
+    detail['note'] = ('frac: measured HBM bytes per launch / launch time / 8 TB/s; a launch '
+                      'advances steps_per_launch fused steps in one pass, so the per-step '
+                      'algorithmic rate (algorithmic_frac, 64 B per cell update) may pass 1 '
+                      'while the physical one cannot')
     cpu = None
     if not args.no_cpu:
         cpu_val = cpu_baseline(wl, args.cpu_steps)
         cpu = dict(value=round(cpu_val, 5), unit='Gcell-updates/s', cores=1, kind='port',
-                   sample='numpy oracle (plain FP64 method), Acoustic2D 2048x2048, same scene, '
-                          '%d steps (per-step cost is constant)' % args.cpu_steps)
+                   sample='oracle (plain FP64 method with scipy dia_matrix products, as the '
+                          'reference), Acoustic2D 2048x2048, same scene, %d steps (per-step '
+                          'cost is constant)' % args.cpu_steps, host=cpu_host())
     line = {
         'metric': METRIC,
         'value': round(ref['gcells'], 3),
@@ -654,24 +854,8 @@ def main():
         'config': {'workload': 'Acoustic2D 2048x2048 %s drop-in, steel inclusion, 1 MHz '
                                'burst source, 64 recorded points' % main_form.upper(),
                    'nx': NX, 'ny': NY, 'form': main_form, 'recorded_points': 64},
-        'roofline': {'bound': 'hbm', 'achieved': round(ref['achieved_gbs'], 1),
-                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': round(ref['achieved_gbs'] / HBM_PEAK_GBS, 4),
-                     'traffic': measured_traffic(main_form, ref['tune'])},
-        'roofline_detail': {
-            'kernel': ref['tune']['shape'],
-            'launch_us': round(ref['kernel_us'] * ref['tune']['steps_per_launch'], 2),
-            'algorithmic_bytes_per_launch': BYTES_PER_CELL[main_form] * NX * NY *
-            ref['tune']['steps_per_launch'],
-            'traffic': 'HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/), '
-                       'null unless this shape was profiled',
-            'physical_gbs': (None if measured_traffic(main_form, ref['tune']) is None else
-                             round(measured_traffic(main_form, ref['tune']) /
-                                   (ref['kernel_us'] * ref['tune']['steps_per_launch']) / 1e3,
-                                   1)),
-            'note': 'achieved counts the algorithmic bytes of every fused step; a launch of '
-                    'several steps makes one pass over HBM, so achieved may pass the peak '
-                    'while physical_gbs (measured traffic / launch time) stays below it'},
+        'roofline': roof,
+        'roofline_detail': detail,
         'cpu_baseline': cpu,
         'forms': out_forms,
     }
@@ -706,10 +890,18 @@ def main():
         bat['config'] = ('256 Acoustic2D/Acoustic3DAxi instances, sides 128-1024, %d steps, '
                          'HIP-graph chunks of 50' % args.batch_steps)
         if not args.no_cpu:
+            host = cpu_host()
+            workers = max(1, min(16, host['affinity'] or 1))
             bat['cpu_baseline'] = dict(value=round(cpu_baseline_batched(2), 5),
                                        unit='Gcell-updates/s', cores=1, kind='port',
-                                       sample='numpy oracle (plain FP64 method), first 32 '
-                                              'of the 256 instances, 2 steps each')
+                                       sample='oracle (plain FP64 method, scipy dia_matrix '
+                                              'products), first 32 of the 256 instances, '
+                                              '2 steps each', host=host)
+            bat['cpu_baseline_all_cores'] = dict(
+                value=round(cpu_baseline_batched_pool(2, workers), 5), unit='Gcell-updates/s',
+                cores=workers, kind='port',
+                sample='the same oracle on all 256 instances, 2 steps each, one process per '
+                       'core (instances are independent)')
         line['batched'] = bat
     if 'thermal' in sections:
         th = {}
@@ -719,10 +911,15 @@ def main():
                              None if args.thermal_tile is None
                              else (args.thermal_tile, args.thermal_strip))
             tvals[f] = r['t']
+            traffic = None
+            if f == 'fp64' and not args.no_traffic:
+                traffic = measure_traffic('thermal', f, r['tune']['tile'], r['tune']['strip'])
+            roof, detail = roofline_entry(THERMAL_BYTES_PER_CELL[f], r['cells'], r['launch_us'],
+                                          r['tune']['steps_per_launch'], traffic)
             th[f] = dict(value=round(r['gcells'], 3), ms_per_step=round(r['ms_per_step'], 4),
                          kernel_us=round(r['kernel_us'], 2),
-                         achieved_gbs=round(r['achieved_gbs'], 1),
-                         roofline_frac=round(r['achieved_gbs'] / HBM_PEAK_GBS, 4),
+                         algorithmic_frac=round(r['achieved_gbs'] / HBM_PEAK_GBS, 4),
+                         roofline=roof, roofline_detail=detail,
                          bytes_per_cell=THERMAL_BYTES_PER_CELL[f], autotune=r['tune'])
         if 'fp64' in tvals:
             for f, t in tvals.items():
@@ -734,7 +931,8 @@ def main():
         if not args.no_cpu:
             th['cpu_baseline'] = dict(value=round(cpu_baseline_thermal(3), 5),
                                       unit='Gcell-updates/s', cores=1, kind='port',
-                                      sample='numpy oracle, Thermal2D 4096x4096, 3 steps')
+                                      sample='oracle (scipy dia_matrix products), Thermal2D '
+                                             '4096x4096, 3 steps')
         line['thermal2d'] = th
     if 'learned' in sections:
         line['learned_bc'] = learned_section(args.no_cpu, iters=args.learned_iters)

@@ -171,8 +171,9 @@ int fds_set_tile(fds_ctx *ctx, int tile_index);
 int fds_tile_count(void);
 int fds_tile_shape(int tile_index, int *tx, int *ty);
 int fds_tile_engine(int tile_index);
-/* Time steps per launch and cache policy (1: non-temporal state traffic) of a shape. */
-int fds_tile_traits(int tile_index, int *steps, int *nontemporal);
+/* Time steps per launch, cache policy (1: non-temporal state traffic) and prefetch depth (input
+ * rows a streaming workgroup loads ahead; 0 for LDS-tile shapes) of a shape. */
+int fds_tile_traits(int tile_index, int *steps, int *nontemporal, int *prefetch);
 /* Strip height (rows per workgroup) of a streaming shape; 0 restores the shape's own height.
  * Ignored by LDS-tile shapes.  For the resident 1D engine the value is the number of cells per
  * thread (1, 2, 4 or 8; 0 = chosen from the line length). */
@@ -209,7 +210,22 @@ int fds_batch_run_async(fds_batch *b, int64_t start_step, int64_t nsteps, int ch
 int fds_batch_take(fds_batch *b, double *const *dst);
 /* Time steps one launch of the batch kernel advances (as fds_steps_per_launch). */
 int fds_batch_steps_per_launch(fds_batch *b);
+/* Tile scheduling of a streaming batch (no reference counterpart).  0 (default): one
+ * workgroup per tile, the hardware dispatcher hands them out (XCD-banded order).  1: a
+ * persistent launch of as many workgroups as the device holds at once, taking tiles from a
+ * work queue (a device counter, largest tiles first) until none is left; results are the same
+ * bits.  FDS_E_ARG for a batch on the LDS-tile engine. */
+int fds_batch_set_schedule(fds_batch *b, int schedule);
 int fds_batch_destroy(fds_batch *b);
+
+/* Non-finite detection (no reference counterpart: pyfds propagates NaN and infinity
+ * silently; failure detection for long runs).  enable != 0: every run of the context scans the
+ * first component's state for NaN and infinity after every graph chunk of steps (the resident
+ * 1D engine: after launches at least 50 steps apart) and every component after the run; a
+ * batch run scans its members that enable it after the run.  A run that found one completes
+ * (state and outputs read back as usual) and returns FDS_E_NONFINITE; fds_last_error names
+ * the component and the step count by which it was found. */
+int fds_set_nonfinite_check(fds_ctx *ctx, int enable);
 
 /* Flow along x (AcousticFlow2D, pyfds/acoustic_flow.py:49-57 apply_flow): after every step n,
  * each row r with period[r] == 0 or n % |period[r]| == 0 moves one cell to +x (a zero enters at
@@ -277,11 +293,12 @@ int fds_window_sq_diff(fds_ctx *a, int comp_a, fds_ctx *b, int comp_b, int wx, i
  * in-process consumers that keep them on the GPU. */
 int fds_output_device_ptr(fds_ctx *ctx, int comp, double **ptr, int64_t *rows, int64_t *m);
 
-/* Calibration (no reference counterpart): copies `bytes` bytes device-to-device |reps| times
- * with a plain grid-stride streaming kernel (reps > 0) or an unrolled kernel with four loads in
- * flight per thread and non-temporal stores (reps < 0); returns read+write GB/s and
- * milliseconds per copy. */
-int fds_stream_copy_bench(int64_t bytes, int reps, double *gbs, double *ms_each);
+/* Calibration (no reference counterpart): copies `bytes` bytes device-to-device `reps` times
+ * (after one untimed copy) and returns read+write GB/s and milliseconds per copy.  variant 0:
+ * grid-stride loop of 16-byte elements (4096 workgroups); 1: one pass, 8 loads in flight per
+ * thread; 2: as 1 with non-temporal loads and stores; 3: 16 loads, non-temporal; 4: 4 loads,
+ * non-temporal. */
+int fds_stream_copy_bench(int64_t bytes, int reps, int variant, double *gbs, double *ms_each);
 
 /* Last error message of the calling thread ("" if none). */
 const char *fds_last_error(void);

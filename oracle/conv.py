@@ -7,7 +7,11 @@ per boundary side, its two backward products and the weighted MSE loss:
 
 Pinned in tests/test_learned.py by the adjoint identities
 <conv(x, w), dy> = <x, bwd_data(dy, w)> = <w, bwd_weight(dy, x)>, by central finite
-differences of the loss, and by a dense-matrix brute force of the convolution.
+differences of the loss, and by a dense-matrix brute force of the convolution.  adam_step is
+pinned against torch.optim.Adam over three steps and against its first-step closed form
+w - lr*g/(abs(g) + eps); to_bf16_bits against torch's float32 to bfloat16 cast on values where
+round-to-nearest-even and truncation differ.  Planted mistakes (no second-moment bias
+correction, a truncating conversion) fail the same comparisons.
 
 Also: bf16 bit conversion (round to nearest even) for feeding exact bf16 values to the device.
 """

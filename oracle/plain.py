@@ -25,7 +25,16 @@ What is restated (reference = emtpb/pyfds, files cited as ``pyfds/<file>:<line>`
 Coefficient expressions keep the reference's evaluation order so FP64 results
 are bit-identical to the reference wherever the reference itself is
 deterministic (all arithmetic here is IEEE add/sub/mul/div except ``**gamma``
-in IdealGas1D, which goes through libm ``pow``).
+in IdealGas1D).
+
+The reference's ``**gamma`` is numpy's power, whose bits depend on the host: on
+a CPU with AVX-512 numpy 2.2 computes it with its bundled SVML routine (which
+rounds about 5 % of these calls differently from the nearest double), elsewhere
+with glibc's ``pow`` (about 0.07 %).  :func:`pow_cr` is the correctly rounded
+power, written independently of both (long double power, decided by Python's
+``decimal`` wherever the long double result lies near a rounding midpoint);
+``PlainIdealGas1D.power`` selects which one the oracle's equation of state uses
+(numpy's by default: the reference on this host).
 
 How each function is pinned (checked by tests/test_oracle.py):
 
@@ -45,8 +54,50 @@ __all__ = [
     'Dia', 'd_first', 'd_second', 'material_vector', 'apply_boundaries',
     'record_outputs', 'PlainField', 'PlainAcoustic1D', 'PlainAcoustic2D',
     'PlainAcoustic3DAxi', 'PlainThermal1D', 'PlainThermal2D', 'PlainThermal3DAxi',
-    'PlainIdealGas1D', 'PlainAcoustic2ndOrder1D', 'absorption_coef', 'MODELS',
+    'PlainIdealGas1D', 'PlainAcoustic2ndOrder1D', 'absorption_coef', 'MODELS', 'pow_cr',
+    'numpy_power_is_libm',
 ]
+
+
+# --------------------------------------------------------------------------------------
+# correctly rounded power
+# --------------------------------------------------------------------------------------
+def pow_cr(x, y):
+    """x**y rounded to the nearest double (ties to even), elementwise, for finite x > 0.
+
+    The long double (x87, 64-bit significand) power is within a few long double ulps of the
+    exact value, i.e. within 2^-9 of a double ulp.  Where it lies farther than 2^-8 double
+    ulps from the midpoint between the two doubles around it, rounding it gives the
+    correctly rounded double; the remaining ~0.8 % of inputs are decided by ``decimal`` at
+    80 significant digits (exp(y ln x)), which is far from any midpoint for these inputs.
+    Pinned against ``decimal`` on random inputs and against glibc's pow where glibc is
+    correctly rounded (tests/test_crmath.py).
+    """
+    from decimal import Decimal, localcontext
+    x = np.asarray(x, np.float64)
+    y = np.broadcast_to(np.asarray(y, np.float64), x.shape)
+    if np.any(~np.isfinite(x) | (x <= 0) | ~np.isfinite(y)):
+        raise ValueError('pow_cr: finite x > 0 and finite y only')
+    ld = np.power(x.astype(np.longdouble), y.astype(np.longdouble))
+    d = ld.astype(np.float64)
+    other = np.nextafter(d, np.where(ld > d, np.inf, -np.inf))
+    dl, ol = d.astype(np.longdouble), other.astype(np.longdouble)
+    mid = (dl + ol) / 2
+    ulp = np.abs(ol - dl)
+    near = ~(np.abs(ld - mid) > ulp * np.longdouble(2.0) ** -8) | ~np.isfinite(d)
+    out = d.copy()
+    with localcontext() as ctx:
+        ctx.prec = 80
+        for i in np.flatnonzero(near):
+            out.flat[i] = float((Decimal(float(y.flat[i])) * Decimal(float(x.flat[i])).ln()).exp())
+    return out
+
+
+def numpy_power_is_libm():
+    """True when this host's numpy power rounds like glibc's pow (no SVML dispatch)."""
+    import math
+    probe = 1.0 + np.linspace(-1e-3, 1e-3, 4001)
+    return bool(np.array_equal(probe ** 1.4, np.array([math.pow(v, 1.4) for v in probe])))
 
 
 # --------------------------------------------------------------------------------------
@@ -85,7 +136,20 @@ class Dia:
                 y[-off:] += data[:n + off] * x[:n + off]
         return y
 
+    # When True, ``dot`` multiplies with scipy's own ``dia_matrix`` built from the same stored
+    # diagonals (scipy's compiled ``dia_matvec``, the reference's CPU kernel; same bits as
+    # ``matvec``): the timed CPU baseline uses it.
+    use_scipy = False
+
     def dot(self, x):
+        if Dia.use_scipy:
+            m = self.__dict__.get('_scipy')
+            if m is None:
+                import scipy.sparse as sp
+                m = sp.dia_matrix((np.array([d for _, d in self.diags]), self.offsets),
+                                  shape=(self.n, self.n))
+                self._scipy = m
+            return m.dot(np.asarray(x, dtype=float))
         return self.matvec(x)
 
     def __add__(self, other):
@@ -424,8 +488,12 @@ class PlainThermal3DAxi(PlainThermal2D):
 
 
 class PlainIdealGas1D(PlainField):
-    """``IdealGas1D`` (``pyfds/nonlinear_acoustics.py:10-89``)."""
+    """``IdealGas1D`` (``pyfds/nonlinear_acoustics.py:10-89``).
+
+    ``power`` is the ``**`` of the nonlinear equation of state: ``'numpy'`` (the reference's
+    numpy power on this host) or ``'cr'`` (:func:`pow_cr`)."""
     component_names = ('pressure', 'velocity', 'density')
+    power = 'numpy'
 
     def __init__(self, *args, convective=True, nl_state=True, **kwargs):
         super().__init__(*args, **kwargs)
@@ -462,8 +530,11 @@ class PlainIdealGas1D(PlainField):
     def equation_of_state(self):
         c, rho0 = self.components, self.stat_density
         if self.nl_state:
+            base = (rho0 + c['density']) / rho0
+            pw = (pow_cr(base, self.heat_cap_ratio) if self.power == 'cr'
+                  else base ** self.heat_cap_ratio)
             c['pressure'] = (rho0 * self.sound_velocity ** 2 / self.heat_cap_ratio
-                             * (((rho0 + c['density']) / rho0) ** self.heat_cap_ratio - 1))
+                             * (pw - 1))
         else:
             c['pressure'] = self.sound_velocity ** 2 * c['density']
 

@@ -26,15 +26,39 @@ EXPORTS = ('fds_create', 'fds_destroy', 'fds_set_scalars', 'fds_set_coeff', 'fds
            'fds_couple_loss_run',
            'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_batch_create', 'fds_batch_run',
            'fds_batch_run_async', 'fds_batch_take', 'fds_batch_steps_per_launch',
-           'fds_batch_last_run_ms', 'fds_batch_destroy', 'fds_set_learned_boundary',
+           'fds_batch_last_run_ms', 'fds_batch_destroy', 'fds_batch_set_schedule',
+           'fds_set_learned_boundary',
            'fds_lbc_conv_fwd', 'fds_lbc_conv_bwd_data', 'fds_lbc_conv_bwd_weight',
            'fds_lbc_reduce_weight', 'fds_lbc_mse', 'fds_lbc_adam', 'fds_lbc_pack',
            'fds_lbc_gather', 'fds_lbc_band_rms', 'fds_window_sq_diff', 'fds_output_device_ptr',
-           'fds_last_error', 'fds_version')
+           'fds_set_nonfinite_check', 'fds_last_error', 'fds_version')
 
 
 class NativeError(RuntimeError):
     """The HIP library is missing, failed to load, or a device call failed."""
+
+
+def _preload_hip_runtime():
+    """Make the HIP runtime that PyTorch-ROCm ships (when it is installed) the process's
+    libamdhip64 before this library pulls in the system one, without importing torch.
+
+    Both carry the soname libamdhip64.so.7; whichever is loaded first serves the whole process,
+    and torch reports no GPU when the other one is already loaded.  Loading torch's file by
+    path first keeps a later ``import torch`` (the learned boundary, user code) working in
+    the same process; without torch the system runtime is used."""
+    import importlib.util
+    import sys
+    if 'torch' in sys.modules:
+        return
+    try:
+        spec = importlib.util.find_spec('torch')
+    except (ImportError, ValueError):
+        return
+    if spec is None or not spec.submodule_search_locations:
+        return
+    path = os.path.join(list(spec.submodule_search_locations)[0], 'lib', 'libamdhip64.so')
+    if os.path.exists(path):
+        ct.CDLL(path, mode=ct.RTLD_GLOBAL)
 
 
 _lib = None
@@ -56,13 +80,7 @@ def load():
         raise NativeError('pyfds_amd HIP library not built (%s missing); run '
                           '`python -m pyfds_amd.build`' % LIB_PATH)
     try:
-        # PyTorch-ROCm ships its own libamdhip64 under the same soname as /opt/rocm's.  Whichever
-        # is loaded first serves the whole process, and torch reports no GPU when it finds the
-        # other one already loaded; so when torch is installed, let it load first and share it.
-        import torch  # noqa: F401
-    except ImportError:
-        pass
-    try:
+        _preload_hip_runtime()
         lib = ct.CDLL(LIB_PATH)
     except OSError as exc:
         raise NativeError('cannot load %s: %s' % (LIB_PATH, exc)) from exc
@@ -84,13 +102,14 @@ def load():
         'fds_read_outputs': (ct.c_int, [_p, ct.c_int, _i64, _i64, _dp]),
         'fds_set_tile': (ct.c_int, [_p, ct.c_int]),
         'fds_last_run_ms': (ct.c_int, [_p, _dp]),
-        'fds_stream_copy_bench': (ct.c_int, [_i64, ct.c_int, _dp, _dp]),
+        'fds_stream_copy_bench': (ct.c_int, [_i64, ct.c_int, ct.c_int, _dp, _dp]),
         'fds_batch_create': (ct.c_int, [ct.POINTER(_p), ct.POINTER(_p), ct.c_int, ct.c_int]),
         'fds_batch_run': (ct.c_int, [_p, _i64, _i64, ct.c_int]),
         'fds_batch_run_async': (ct.c_int, [_p, _i64, _i64, ct.c_int]),
         'fds_batch_take': (ct.c_int, [_p, ct.POINTER(_p)]),
         'fds_batch_last_run_ms': (ct.c_int, [_p, _dp]),
         'fds_batch_destroy': (ct.c_int, [_p]),
+        'fds_batch_set_schedule': (ct.c_int, [_p, ct.c_int]),
         'fds_set_learned_boundary': (ct.c_int, [_p, _p, _p, ct.c_int, _p]),
         'fds_lbc_conv_fwd': (ct.c_int, [_p, _p, _p, _p, _p] + [ct.c_int] * 5 + [_p]),
         'fds_lbc_conv_bwd_data': (ct.c_int, [_p, _p, _p, _p, _p] + [ct.c_int] * 5 + [_p]),
@@ -107,7 +126,8 @@ def load():
                                              ct.POINTER(_i64)]),
         'fds_tile_count': (ct.c_int, []),
         'fds_tile_engine': (ct.c_int, [ct.c_int]),
-        'fds_tile_traits': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
+        'fds_tile_traits': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int),
+                                       ct.POINTER(ct.c_int)]),
         'fds_set_strip_height': (ct.c_int, [_p, ct.c_int]),
         'fds_engine': (ct.c_int, [_p]),
         'fds_steps_per_launch': (ct.c_int, [_p]),
@@ -116,6 +136,7 @@ def load():
         'fds_couple_loss_run': (ct.c_int, [_p, _p, _dp, _dp, ct.c_double, ct.c_int64, ct.c_int,
                                            _dp, ct.c_int64, ct.c_int64]),
         'fds_set_flow': (ct.c_int, [_p, _lp, ct.c_int64]),
+        'fds_set_nonfinite_check': (ct.c_int, [_p, ct.c_int]),
         'fds_tile_shape': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         'fds_last_error': (ct.c_char_p, []),
         'fds_version': (ct.c_char_p, []),
@@ -128,6 +149,11 @@ def load():
     return lib
 
 
+def error_message():
+    """The library's last error message on this thread."""
+    return load().fds_last_error().decode(errors='replace')
+
+
 def check(rc):
     """Turn a negative return code into the reference's exception types."""
     if rc >= 0:
@@ -137,6 +163,8 @@ def check(rc):
         raise IndexError(msg)
     if rc == E_ARG:
         raise ValueError(msg)
+    if rc == E_NONFINITE:
+        raise FloatingPointError(msg)
     raise NativeError('pyfds_amd device call failed (%d): %s' % (rc, msg))
 
 
@@ -171,15 +199,15 @@ def tile_shapes():
 
 
 def tile_table():
-    """Every shape of the step kernels: dicts with index, engine, tx, ty, steps (per launch)
-    and nontemporal."""
+    """Every shape of the step kernels: dicts with index, engine, tx, ty, steps (per launch),
+    nontemporal and prefetch (rows loaded ahead by a streaming workgroup)."""
     lib = load()
     out = []
     for i, ((tx, ty), eng) in enumerate(zip(tile_shapes(), tile_engines())):
-        st, nt = ct.c_int(), ct.c_int()
-        check(lib.fds_tile_traits(i, ct.byref(st), ct.byref(nt)))
+        st, nt, pf = ct.c_int(), ct.c_int(), ct.c_int()
+        check(lib.fds_tile_traits(i, ct.byref(st), ct.byref(nt), ct.byref(pf)))
         out.append(dict(index=i, engine=eng, tx=tx, ty=ty, steps=st.value,
-                        nontemporal=bool(nt.value)))
+                        nontemporal=bool(nt.value), prefetch=pf.value))
     return out
 
 

@@ -16,10 +16,21 @@ from . import _native as nat
 from .device import pipeline_sizes
 
 
-class BatchRunner:
-    """Advance ``fields`` together; ``tile`` picks one shape of the tile table for all."""
+SCHEDULES = {'dispatch': 0, 'queue': 1}
 
-    def __init__(self, fields, tile=None, chunk=50):
+
+class BatchRunner:
+    """Advance ``fields`` together; ``tile`` picks one shape of the tile table for all.
+
+    ``schedule`` (streaming shapes): ``'dispatch'`` launches one workgroup per tile and lets
+    the hardware dispatcher hand them out; ``'queue'`` launches a persistent grid (as many
+    workgroups as the device holds at once) that takes tiles from a device work queue,
+    largest first.  Both give the same bits."""
+
+    def __init__(self, fields, tile=None, chunk=50, schedule='dispatch'):
+        if schedule not in SCHEDULES:
+            raise ValueError('schedule is one of {}'.format(sorted(SCHEDULES)))
+        self.schedule = schedule
         self.fields = list(fields)
         if not self.fields:
             raise ValueError('empty batch')
@@ -59,6 +70,8 @@ class BatchRunner:
             nat.check(self.lib.fds_batch_create(ct.byref(handle), arr, len(members),
                                                 self.tile))
             self.handle, self._members = handle, members
+            if self.schedule != 'dispatch':
+                nat.check(self.lib.fds_batch_set_schedule(handle, SCHEDULES[self.schedule]))
         return runners
 
     def simulate(self, num_steps):
@@ -77,10 +90,15 @@ class BatchRunner:
         if sizes:
             self._run_pipelined(runners, start, sizes)
             return
-        nat.check(self.lib.fds_batch_run(self.handle, start, num_steps, self.chunk))
+        rc = self.lib.fds_batch_run(self.handle, start, num_steps, self.chunk)
+        if rc != nat.E_NONFINITE:
+            nat.check(rc)
+        msg = nat.error_message() if rc == nat.E_NONFINITE else None
         for f, r in zip(self.fields, runners):
             r.collect(f, num_steps)
             f.step += num_steps
+        if msg is not None:        # the run completed; a member met a NaN or an infinity
+            raise FloatingPointError(msg)
 
     def _run_pipelined(self, runners, start, sizes):
         """Asynchronous batch runs of ``sizes`` steps, the next one enqueued before the outputs
@@ -95,27 +113,38 @@ class BatchRunner:
             queue.append(k)
             nxt[0] += k
 
+        nonfinite = []
+
         def take():
             k = queue[0]
             data = [[np.empty((k, m)) if m else None for m in cs] for cs in counts]
             ptrs = (ct.c_void_p * (3 * len(data)))(
                 *[None if d is None else d.ctypes.data for ds in data for d in ds])
-            nat.check(lib.fds_batch_take(handle, ptrs))
+            rc = lib.fds_batch_take(handle, ptrs)
+            if rc != nat.E_NONFINITE:
+                nat.check(rc)
             queue.popleft()
             for f in self.fields:
                 f.step += k
             for f, r, ds in zip(self.fields, runners, data):
                 r._append_outputs(f, ds)
+            if rc == nat.E_NONFINITE:
+                nonfinite.append(nat.error_message())
 
         try:
             enqueue(sizes[0])
             for k in sizes[1:]:
+                if nonfinite:
+                    break
                 enqueue(k)
                 take()
-            take()
+            if queue:
+                take()
         finally:
             while queue:
                 take()
+        if nonfinite:
+            raise FloatingPointError(nonfinite[0])
 
     def last_run_ms(self):
         ms = ct.c_double()

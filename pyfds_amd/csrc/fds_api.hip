@@ -209,6 +209,10 @@ struct fds_ctx {
   const uint16_t *lbc_w1 = nullptr, *lbc_w2 = nullptr;
   int lbc_layer = 0;
   float lbc_cs[3] = {1.f, 1.f, 1.f};
+  // non-finite detection: scans in every graph chunk and after every run; the result
+  // (dyn[kDynNonfinite]) is copied to nf_stage[slot] (slot kMaxFlights: synchronous runs)
+  bool nf_check = false;
+  HostBuf nf_stage;
 };
 
 struct fds_batch {
@@ -234,6 +238,10 @@ struct fds_batch {
   std::deque<Flight> flights;     // asynchronous batch runs in flight (oldest first)
   int next_slot = 0;
   hipEvent_t fev[kMaxFlights][3] = {};
+  // tile scheduling of streaming batches: 0 the hardware dispatcher (one workgroup per tile,
+  // XCD-banded), 1 a persistent grid taking tiles from a work queue in `order`
+  int schedule = 0;
+  DevBuf order, queue;
 };
 
 namespace {
@@ -618,7 +626,7 @@ void fill_args(const fds_ctx *c, fds::StepArgs<T> &a, int parity) {
   a.ghost = c->ghost;
 }
 
-fds::LaunchFn pick_launch(const fds_ctx *c, bool fused) {
+fds::LaunchFn pick_launch(const fds_ctx *c, int stages) {
   const int t = c->shape;
   if (c->streaming && c->lossy) {
     // variant: axial-symmetric, isotropic (B, K and M y slots alias the x ones), plain
@@ -626,7 +634,7 @@ fds::LaunchFn pick_launch(const fds_ctx *c, bool fused) {
     return c->ks->stream_lossy[t - fds::kNumTiles][v];
   }
   if (c->streaming) {
-    const int s = t - fds::kNumTiles, f = fused ? 1 : 0;
+    const int s = t - fds::kNumTiles, f = stages - 1;
     // variant: axial-symmetric, isotropic (y coefficient slots alias the x ones), plain
     const int v = c->axi ? 1 : (c->iso_b && c->iso_k) ? 2 : 0;
     return c->family == kThermal ? c->ks->stream_thermal[s][v][f]
@@ -684,6 +692,37 @@ __global__ void k_advance(int64_t *dyn, int64_t by) {
   }
 }
 
+__global__ void k_queue_reset(unsigned long long *q) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) q[0] = 0ull;
+}
+
+// Non-finite detection.  dyn[kDynNonfinite] holds the smallest (step * 4 + component) over the
+// scans that met a NaN or an infinity in a component's state (all bits set: none), where step
+// counts the steps done when the scan ran.  A scan reads the state once (grid-stride loop);
+// only lanes that meet a non-finite value touch the flag.
+constexpr int kDynNonfinite = 4;
+constexpr uint64_t kNoNonfinite = ~0ull;
+constexpr int64_t kNonfiniteEvery = 50;   // resident 1D engine: steps between scans (at least)
+
+__global__ void k_nonfinite_reset(int64_t *dyn) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    reinterpret_cast<unsigned long long *>(dyn)[kDynNonfinite] = kNoNonfinite;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_nonfinite_scan(const T *__restrict__ x, int64_t n,
+                                                        int comp,
+                                                        const int64_t *__restrict__ base,
+                                                        int64_t add, int64_t *flag_dyn) {
+  bool bad = false;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    bad |= !__builtin_isfinite(x[i]);
+  if (bad)
+    atomicMin(reinterpret_cast<unsigned long long *>(flag_dyn) + kDynNonfinite,
+              (unsigned long long)((base[0] + add) * 4 + comp));
+}
+
 // Launch one step: reads state buffer `parity`, writes parity^1.
 template <typename T>
 void launch_lbc(fds_ctx *, const fds::StepArgs<T> &) {}
@@ -710,16 +749,16 @@ int step_group(const fds_ctx *c) {
   return (c->streaming && !c->lbc_w1 && !c->flow) ? c->fuse : 1;
 }
 
-// Launch one step (or, fused, step_group(c) steps) starting at step k of the launch sequence:
-// reads state buffer `parity`, writes parity^1.
+// Launch `stages` fused steps (1 <= stages <= step_group(c)) starting at step k of the launch
+// sequence: reads state buffer `parity`, writes parity^1.
 template <typename T>
-int launch_step(fds_ctx *c, int parity, int k, bool store, bool fused) {
+int launch_step(fds_ctx *c, int parity, int k, bool store, int stages) {
   fds::StepArgs<T> a;
   fill_args<T>(c, a, parity);
   a.k = k;
   if (store) a.flags |= fds::kFlagStore;
   const int nb = fds::xcd_grid(c->ntiles);
-  pick_launch(c, fused)(&a, nb, lds_bytes(c), c->stream);
+  pick_launch(c, stages)(&a, nb, lds_bytes(c), c->stream);
   HIP_TRY(hipGetLastError());
   if (c->flow) {
     hipLaunchKernelGGL(k_flow_shift<T>, dim3((unsigned)c->ny), dim3(256), 0, c->stream,
@@ -751,9 +790,56 @@ int launch_line(fds_ctx *c, int parity, int k, int steps) {
   return 0;
 }
 
-int launch_step_any(fds_ctx *c, int parity, int k, bool store, bool fused = false) {
-  return c->elem == 8 ? launch_step<double>(c, parity, k, store, fused)
-                      : launch_step<float>(c, parity, k, store, fused);
+int launch_step_any(fds_ctx *c, int parity, int k, bool store, int stages = 1) {
+  return c->elem == 8 ? launch_step<double>(c, parity, k, store, stages)
+                      : launch_step<float>(c, parity, k, store, stages);
+}
+
+// Scan component k's state (buffer `parity` where the component ping-pongs) for non-finite
+// values on `stream`; a hit records step base[0] + add in flag_dyn.
+int scan_nonfinite(const fds_ctx *c, int k, int parity, const int64_t *base, int64_t add,
+                   int64_t *flag_dyn, hipStream_t stream) {
+  const Comp &cp = c->comp[k];
+  if (!cp.present) return 0;
+  const bool pp = cp.accumulated || c->family != kThermal || k == 0;
+  const void *x = cp.buf[pp ? parity : 0].p;
+  const unsigned nb = (unsigned)std::min<int64_t>((c->n + 1023) / 1024, 2048);
+  if (c->elem == 8)
+    hipLaunchKernelGGL(k_nonfinite_scan<double>, dim3(nb), dim3(256), 0, stream,
+                       static_cast<const double *>(x), c->n, k, base, add, flag_dyn);
+  else
+    hipLaunchKernelGGL(k_nonfinite_scan<float>, dim3(nb), dim3(256), 0, stream,
+                       static_cast<const float *>(x), c->n, k, base, add, flag_dyn);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int scan_all_nonfinite(const fds_ctx *c, int parity, const int64_t *base, int64_t add,
+                       hipStream_t stream) {
+  for (int k = 0; k < 3; ++k)
+    if (int rc = scan_nonfinite(c, k, parity, base, add, c->dyn.as<int64_t>(), stream))
+      return rc;
+  return 0;
+}
+
+// Copy the context's non-finite result to staging slot `slot` (stream order).
+int stage_nonfinite(fds_ctx *c, int slot, hipStream_t stream) {
+  if (!c->nf_check) return 0;
+  HIP_TRY(c->nf_stage.ensure((kMaxFlights + 1) * sizeof(uint64_t)));
+  HIP_TRY(hipMemcpyAsync(static_cast<uint64_t *>(c->nf_stage.p) + slot,
+                         c->dyn.as<uint64_t>() + kDynNonfinite, sizeof(uint64_t),
+                         hipMemcpyDeviceToHost, stream));
+  return 0;
+}
+
+// After the copy of stage_nonfinite(c, slot) completed: FDS_E_NONFINITE if a scan met a NaN
+// or an infinity.
+int check_nonfinite(const fds_ctx *c, int slot) {
+  if (!c->nf_check || !c->nf_stage.p) return 0;
+  const uint64_t v = static_cast<const uint64_t *>(c->nf_stage.p)[slot];
+  if (v == kNoNonfinite) return 0;
+  return fail(FDS_E_NONFINITE, "non-finite value in component %d, found after step %lld",
+              (int)(v & 3), (long long)(v >> 2));
 }
 
 int get_graph(fds_ctx *c, int chunk, int parity, hipGraphExec_t *out) {
@@ -773,7 +859,10 @@ int get_graph(fds_ctx *c, int chunk, int parity, hipGraphExec_t *out) {
   int rc = 0;
   const int grp = step_group(c);   // chunk is a multiple of grp
   for (int k = 0; k < chunk && rc == 0; k += grp)
-    rc = launch_step_any(c, parity ^ ((k / grp) & 1), k, false, grp > 1);
+    rc = launch_step_any(c, parity ^ ((k / grp) & 1), k, false, grp);
+  if (rc == 0 && c->nf_check)       // the first component after the chunk
+    rc = scan_nonfinite(c, 0, parity ^ ((chunk / grp) & 1), c->dyn.as<const int64_t>(), chunk,
+                        c->dyn.as<int64_t>(), c->stream);
   if (rc == 0) {
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->dyn.as<int64_t>(),
                        (int64_t)chunk);
@@ -848,12 +937,14 @@ int fds_tile_engine(int i) {
   return i >= fds::kNumTiles ? 1 : 0;
 }
 
-int fds_tile_traits(int i, int *steps, int *nontemporal) {
+int fds_tile_traits(int i, int *steps, int *nontemporal, int *prefetch) {
   if (i < 0 || i >= fds::kNumTiles + fds::kNumStream)
     return fail(FDS_E_ARG, "tile index %d out of range", i);
+  if (!steps || !nontemporal || !prefetch) return fail(FDS_E_ARG, "null argument");
   const bool st = i >= fds::kNumTiles;
   *steps = st ? fds::kStreamShapes[i - fds::kNumTiles][2] : 1;
   *nontemporal = st ? fds::kStreamShapes[i - fds::kNumTiles][3] : 0;
+  *prefetch = st ? fds::kStreamShapes[i - fds::kNumTiles][4] : 0;
   return 0;
 }
 
@@ -1220,6 +1311,10 @@ int prepare_run(fds_ctx *c, int64_t start, int64_t nsteps) {
   }
   hipLaunchKernelGGL(k_set_dyn, dim3(1), dim3(64), 0, c->stream, c->dyn.as<int64_t>(), start);
   HIP_TRY(hipGetLastError());
+  if (c->nf_check) {
+    hipLaunchKernelGGL(k_nonfinite_reset, dim3(1), dim3(64), 0, c->stream, c->dyn.as<int64_t>());
+    HIP_TRY(hipGetLastError());
+  }
   if (!c->ev0) HIP_TRY(hipEventCreate(&c->ev0));
   if (!c->ev1) HIP_TRY(hipEventCreate(&c->ev1));
   if (c->line) {
@@ -1323,6 +1418,7 @@ int enqueue_steps(fds_ctx *c, int64_t nsteps, int chunk, hipEvent_t e0, hipEvent
     // terms, the nonlinear mass flux), one more cell on the right: ghost - 1 steps per launch.
     const int per = c->line_wgs > 1 ? std::min(c->line_ghost - 1, fds::kLineSteps)
                                     : fds::kLineSteps;
+    int64_t scanned = 0;
     while (done < nsteps) {
       const int st = (int)std::min<int64_t>(nsteps - done, per);
       const int rc = c->elem == 8 ? launch_line<double>(c, c->parity, (int)done, st)
@@ -1330,10 +1426,18 @@ int enqueue_steps(fds_ctx *c, int64_t nsteps, int chunk, hipEvent_t e0, hipEvent
       if (rc) return rc;
       c->parity ^= 1;
       done += st;
+      // the first component every kNonfiniteEvery steps or more (launch granularity)
+      if (c->nf_check && done < nsteps && done - scanned >= kNonfiniteEvery) {
+        if (int rc2 = scan_nonfinite(c, 0, c->parity, c->dyn.as<const int64_t>(), done,
+                                     c->dyn.as<int64_t>(), c->stream))
+          return rc2;
+        scanned = done;
+      }
     }
     chunk = 0;
   }
   const int g = step_group(c);
+  int64_t graph_done = 0;    // steps the device step counter advanced by (graph chunks)
   if (chunk > 0) {
     // graph-replayed full chunks (whole launches of g steps), never including the run's last
     // step (which stores the assigned components)
@@ -1344,19 +1448,21 @@ int enqueue_steps(fds_ctx *c, int64_t nsteps, int chunk, hipEvent_t e0, hipEvent
       HIP_TRY(hipGraphLaunch(ex, c->stream));
       c->parity ^= ((chunk / g) & 1);
       done += chunk;
+      graph_done += chunk;
     }
   }
-  // the rest as plain launches (k counts from the device step counter's current base)
-  int64_t k = 0;
-  for (; g > 1 && nsteps - done - k >= g; k += g) {
-    if (int rc = launch_step_any(c, c->parity, (int)k, done + k + g == nsteps, true)) return rc;
+  // the rest as plain launches of g stages, then one launch of the remaining stages (k counts
+  // from the device step counter's current base)
+  for (int64_t k = 0; done + k < nsteps;) {
+    const int st = (int)std::min<int64_t>(g, nsteps - done - k);
+    if (int rc = launch_step_any(c, c->parity, (int)k, done + k + st == nsteps, st)) return rc;
     c->parity ^= 1;
+    k += st;
   }
-  for (; done + k < nsteps; ++k) {
-    const bool last = (done + k == nsteps - 1);
-    if (int rc = launch_step_any(c, c->parity, (int)k, last)) return rc;
-    c->parity ^= 1;
-  }
+  if (c->nf_check)   // every component after the run
+    if (int rc = scan_all_nonfinite(c, c->parity, c->dyn.as<const int64_t>(),
+                                    nsteps - graph_done, c->stream))
+      return rc;
   HIP_TRY(hipEventRecord(e1, c->stream));
   return 0;
 }
@@ -1462,11 +1568,12 @@ int fds_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
   if (nsteps == 0) return 0;
   if (int rc = prepare_run(c, start, nsteps)) return rc;
   if (int rc = enqueue_steps(c, nsteps, chunk, c->ev0, c->ev1)) return rc;
+  if (int rc = stage_nonfinite(c, kMaxFlights, c->stream)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_run_ms = ms;
-  return 0;
+  return check_nonfinite(c, kMaxFlights);
 }
 
 int fds_prepare_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
@@ -1500,6 +1607,7 @@ int fds_run_async(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
   if (int rc = ensure_events(c->fev[slot])) return rc;
   if (int rc = enqueue_steps(c, nsteps, chunk, c->fev[slot][0], c->fev[slot][1])) return rc;
   if (int rc = stage_outputs(c, slot, nsteps, c->stream)) return rc;
+  if (int rc = stage_nonfinite(c, slot, c->stream)) return rc;
   HIP_TRY(hipEventRecord(c->fev[slot][2], c->stream));
   if (c->flights.empty()) c->last_run_ms = 0.0;   // a new sequence of asynchronous runs
   c->flights.push_back(Flight{nsteps, slot});
@@ -1518,6 +1626,15 @@ int fds_run_take(fds_ctx *c, double *dst0, double *dst1, double *dst2) {
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, c->fev[f.slot][0], c->fev[f.slot][1]));
   c->last_run_ms += ms;
+  return check_nonfinite(c, f.slot);
+}
+
+int fds_set_nonfinite_check(fds_ctx *c, int enable) {
+  if (int rc = check_idle(c)) return rc;
+  if (c->nf_check != (enable != 0)) {
+    c->nf_check = enable != 0;
+    touch(c);   // the chunk graphs hold the scans
+  }
   return 0;
 }
 
@@ -1654,15 +1771,21 @@ int batch_build(fds_batch *b, int64_t nsteps) {
   return 0;
 }
 
-int batch_launch(fds_batch *b, int rel_parity, int k, bool store, bool fused = false) {
+int batch_launch(fds_batch *b, int rel_parity, int k, bool store, int stages = 1) {
   fds::BatchLaunch L;
   L.descs = b->descs[rel_parity][store ? 1 : 0].p;
   L.tile_inst = b->tile_inst.as<const int32_t>();
   L.inst_base = b->inst_base.as<const int32_t>();
   L.ntiles = b->ntiles;
   L.k = k;
-  if (b->streaming)
-    b->ks->stream_batch[b->shape - fds::kNumTiles][fused ? 1 : 0](L, fds::xcd_grid(b->ntiles), 0,
+  L.order = b->order.as<const int32_t>();
+  L.queue = b->queue.as<unsigned long long>();
+  L.spl = b->fuse;
+  L.num_cu = b->ctxs[0]->num_cu;
+  if (b->streaming && b->schedule == 1)
+    b->ks->stream_batch_wq[b->shape - fds::kNumTiles][stages - 1](L, 0, 0, b->stream);
+  else if (b->streaming)
+    b->ks->stream_batch[b->shape - fds::kNumTiles][stages - 1](L, fds::xcd_grid(b->ntiles), 0,
                                                                  b->stream);
   else
     b->ks->batch[b->shape](L, fds::xcd_grid(b->ntiles), b->lds, b->stream);
@@ -1682,7 +1805,7 @@ int batch_graph(fds_batch *b, int chunk, int rel_parity, hipGraphExec_t *out) {
   int rc = 0;
   const int grp = b->fuse;   // chunk is a multiple of grp
   for (int k = 0; k < chunk && rc == 0; k += grp)
-    rc = batch_launch(b, rel_parity ^ ((k / grp) & 1), k, false, grp > 1);
+    rc = batch_launch(b, rel_parity ^ ((k / grp) & 1), k, false, grp);
   if (rc == 0)
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, b->stream, b->dyn.as<int64_t>(),
                        (int64_t)chunk);
@@ -1723,6 +1846,17 @@ int batch_prepare(fds_batch *b, int64_t start, int64_t nsteps) {
   if (rc) return rc;
   hipLaunchKernelGGL(k_set_dyn, dim3(1), dim3(64), 0, b->stream, b->dyn.as<int64_t>(), start);
   HIP_TRY(hipGetLastError());
+  if (b->schedule == 1) {   // the work queue starts the run empty
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, b->stream,
+                       b->queue.as<unsigned long long>());
+    HIP_TRY(hipGetLastError());
+  }
+  for (auto *c : b->ctxs)
+    if (c->nf_check) {
+      hipLaunchKernelGGL(k_nonfinite_reset, dim3(1), dim3(64), 0, b->stream,
+                         c->dyn.as<int64_t>());
+      HIP_TRY(hipGetLastError());
+    }
   return 0;
 }
 
@@ -1750,14 +1884,12 @@ int batch_enqueue(fds_batch *b, int64_t nsteps, int chunk, hipEvent_t e0, hipEve
       done += chunk;
     }
   }
-  int64_t k = 0;
-  for (; g > 1 && nsteps - done - k >= g; k += g) {
-    if (int rc2 = batch_launch(b, rel, (int)k, done + k + g == nsteps, true)) return rc2;
+  const int64_t graph_done = done;   // steps the batch step counter advanced by
+  for (int64_t k = 0; done + k < nsteps;) {
+    const int st = (int)std::min<int64_t>(g, nsteps - done - k);
+    if (int rc2 = batch_launch(b, rel, (int)k, done + k + st == nsteps, st)) return rc2;
     rel ^= 1;
-  }
-  for (; done + k < nsteps; ++k) {
-    if (int rc2 = batch_launch(b, rel, (int)k, done + k == nsteps - 1)) return rc2;
-    rel ^= 1;
+    k += st;
   }
   HIP_TRY(hipEventRecord(e1, b->stream));
   // members advanced nsteps steps in lock step, flipping buffers once per launch
@@ -1765,6 +1897,22 @@ int batch_enqueue(fds_batch *b, int64_t nsteps, int chunk, hipEvent_t e0, hipEve
     for (auto *c : b->ctxs) c->parity ^= 1;
     b->flip ^= 1;
   }
+  // members that check for non-finite values: every component after the run
+  for (auto *c : b->ctxs)
+    if (c->nf_check)
+      if (int rc2 = scan_all_nonfinite(c, c->parity, b->dyn.as<const int64_t>(),
+                                       nsteps - graph_done, b->stream))
+        return rc2;
+  return 0;
+}
+
+// The first member whose non-finite check of staging slot `slot` failed, or 0.
+int batch_check_nonfinite(const fds_batch *b, int slot) {
+  for (size_t i = 0; i < b->ctxs.size(); ++i)
+    if (int rc = check_nonfinite(b->ctxs[i], slot)) {
+      const std::string msg = g_last_error;
+      return fail(rc, "batch member %d: %s", (int)i, msg.c_str());
+    }
   return 0;
 }
 
@@ -1796,6 +1944,13 @@ int fds_batch_create(fds_batch **out, fds_ctx *const *ctxs, int n, int tile) {
     if (c->family == kNonlinear) {
       delete b;
       return fail(FDS_E_ARG, "nonlinear 1D models cannot be batched");
+    }
+    // the batch launches only the step kernels: a member's per-step flow shift or learned
+    // boundary would be skipped
+    if (c->flow || c->lbc_w1) {
+      delete b;
+      return fail(FDS_E_ARG, "members with flow (AcousticFlow2D) or a learned boundary "
+                             "cannot be batched");
     }
   }
   hipError_t e = hipSetDevice(b->device);
@@ -1832,12 +1987,44 @@ int fds_batch_create(fds_batch **out, fds_ctx *const *ctxs, int n, int tile) {
   shape_of(t, &b->tx, &b->ty);
   b->lds = lds;
   b->ntiles = (int)tinst.size();
+  // work-queue order: largest tiles first (owned cells plus ghosts; edge tiles are smaller)
+  std::vector<int64_t> cost(b->ntiles);
+  for (int g = 0; g < b->ntiles; ++g) {
+    const fds_ctx *c = b->ctxs[tinst[g]];
+    const int t = g - base[tinst[g]];
+    const int64_t ty = t / std::max(1, c->tiles_x), tx = t - ty * c->tiles_x;
+    const int64_t sx = std::max(1, c->sx), h = std::max(1, c->ty);
+    const int64_t cols = std::min<int64_t>(sx, c->nx - tx * sx);
+    const int64_t rows = std::min<int64_t>(h, c->ny - ty * h);
+    cost[g] = (cols + 2 * c->ghost) * (rows + 2 * c->ghost_rows);
+  }
+  std::vector<int32_t> order(b->ntiles);
+  for (int g = 0; g < b->ntiles; ++g) order[g] = g;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
   if (upload(b->tile_inst, tinst, b->stream) != hipSuccess ||
-      upload(b->inst_base, base, b->stream) != hipSuccess || b->dyn.ensure(64) != hipSuccess) {
+      upload(b->inst_base, base, b->stream) != hipSuccess || b->dyn.ensure(64) != hipSuccess ||
+      upload(b->order, order, b->stream) != hipSuccess ||
+      b->queue.ensure(sizeof(unsigned long long)) != hipSuccess) {
     fds_batch_destroy(b);
     return fail(FDS_E_HIP, "out of device memory for the batch tables");
   }
   *out = b;
+  return 0;
+}
+
+int fds_batch_set_schedule(fds_batch *b, int schedule) {
+  if (!b) return fail(FDS_E_ARG, "null batch");
+  if (schedule != 0 && schedule != 1) return fail(FDS_E_ARG, "schedule is 0 or 1");
+  if (!b->flights.empty()) return fail(FDS_E_STATE, "asynchronous runs in flight");
+  if (schedule == 1 && !b->streaming)
+    return fail(FDS_E_ARG, "the work-queue schedule needs a streaming shape");
+  if (b->schedule != schedule) {
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    batch_drop_graphs(b);
+    b->schedule = schedule;
+  }
   return 0;
 }
 
@@ -1869,11 +2056,13 @@ int fds_batch_run(fds_batch *b, int64_t start, int64_t nsteps, int chunk) {
   if (!b->ev0) HIP_TRY(hipEventCreate(&b->ev0));
   if (!b->ev1) HIP_TRY(hipEventCreate(&b->ev1));
   if (int rc = batch_enqueue(b, nsteps, chunk, b->ev0, b->ev1)) return rc;
+  for (auto *c : b->ctxs)
+    if (int rc = stage_nonfinite(c, kMaxFlights, b->stream)) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->last_run_ms = ms;
-  return 0;
+  return batch_check_nonfinite(b, kMaxFlights);
 }
 
 int fds_batch_run_async(fds_batch *b, int64_t start, int64_t nsteps, int chunk) {
@@ -1887,8 +2076,10 @@ int fds_batch_run_async(fds_batch *b, int64_t start, int64_t nsteps, int chunk) 
   const int slot = b->next_slot;
   if (int rc = ensure_events(b->fev[slot])) return rc;
   if (int rc = batch_enqueue(b, nsteps, chunk, b->fev[slot][0], b->fev[slot][1])) return rc;
-  for (auto *c : b->ctxs)
+  for (auto *c : b->ctxs) {
     if (int rc = stage_outputs(c, slot, nsteps, b->stream)) return rc;
+    if (int rc = stage_nonfinite(c, slot, b->stream)) return rc;
+  }
   HIP_TRY(hipEventRecord(b->fev[slot][2], b->stream));
   if (b->flights.empty()) b->last_run_ms = 0.0;
   b->flights.push_back(Flight{nsteps, slot});
@@ -1911,7 +2102,7 @@ int fds_batch_take(fds_batch *b, double *const *dst) {
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, b->fev[f.slot][0], b->fev[f.slot][1]));
   b->last_run_ms += ms;
-  return 0;
+  return batch_check_nonfinite(b, f.slot);
 }
 
 int fds_batch_last_run_ms(fds_batch *b, double *ms) {

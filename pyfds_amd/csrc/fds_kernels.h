@@ -15,17 +15,21 @@ constexpr int kDefaultLossless2D = kNumTiles + 8;   // lossless 2D: streaming 64
 constexpr int kDefaultLossy2D = kNumTiles + 7;      // lossy 2D: streaming 64 x 16, one step
 constexpr int kTileRow = 5;
 
-// Row-streaming engine (2D models; lossy acoustics one step per launch): TX threads = columns, strips of TY rows, S time
-// steps fused per launch, NT = non-temporal state reads and writes; TX = 64 runs one wave per
+// Row-streaming engine (2D models; lossy acoustics one step per launch): TX threads = columns,
+// strips of TY rows, S time steps fused per launch, NT = non-temporal state reads and writes,
+// PF = input rows prefetched ahead (loads in flight per wave); TX = 64 runs one wave per
 // workgroup and exchanges neighbours by lane shuffles.  Shape indices continue the tile table:
-// index kNumTiles + i is kStreamShapes[i] = {TX, TY, S, NT}.
-constexpr int kNumStream = 12;
-constexpr int kStreamShapes[kNumStream][4] = {
-    {256, 16, 1, 0}, {256, 16, 2, 0}, {128, 32, 2, 0}, {256, 32, 3, 0}, {256, 16, 1, 1},
-    {256, 16, 2, 1}, {256, 32, 3, 1}, {64, 16, 1, 0},  {64, 16, 2, 0},  {64, 32, 3, 0},
-    {64, 16, 1, 1},  {64, 16, 2, 1}};
+// index kNumTiles + i is kStreamShapes[i] = {TX, TY, S, NT, PF}.
+constexpr int kNumStream = 16;
+constexpr int kStreamShapes[kNumStream][5] = {
+    {256, 16, 1, 0, 1}, {256, 16, 2, 0, 1}, {128, 32, 2, 0, 1}, {256, 32, 3, 0, 1},
+    {256, 16, 1, 1, 1}, {256, 16, 2, 1, 1}, {256, 32, 3, 1, 1}, {64, 16, 1, 0, 1},
+    {64, 16, 2, 0, 1},  {64, 32, 3, 0, 1},  {64, 16, 1, 1, 1},  {64, 16, 2, 1, 1},
+    {64, 32, 3, 0, 2},  {64, 16, 2, 0, 2},  {256, 32, 3, 1, 2}, {64, 32, 3, 1, 2}};
+constexpr int kMaxStages = 3;   // largest S in kStreamShapes
 // X-macro over the streaming shape indices (each shape is its own translation unit).
-#define FDS_STREAM_SHAPES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11)
+#define FDS_STREAM_SHAPES(X) \
+  X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15)
 
 using LaunchFn = void (*)(const void *args, int nblocks, size_t lds_bytes, hipStream_t s);
 // resident 1D launches: `grid` workgroups (segments) of `threads` threads; the coupled one
@@ -43,6 +47,12 @@ struct BatchLaunch {
   const int32_t *inst_base;
   int ntiles;
   int k;
+  // tile work queue (persistent launches): hand-out order of the global tiles, the claim
+  // counter, the steps of a full launch and the device's compute units
+  const int32_t *order;
+  unsigned long long *queue;
+  int spl;
+  int num_cu;
 };
 using BatchFn = void (*)(const BatchLaunch &b, int nblocks, size_t lds_bytes, hipStream_t s);
 
@@ -51,9 +61,12 @@ struct KernelSet {
   LaunchFn thermal[kNumTiles][2];       // [tile][axi]
   LaunchFn nonlinear;
   BatchFn batch[kNumTiles];
-  LaunchFn stream_acoustic[kNumStream][3][2];   // [shape][plain, axi, isotropic][fused]
-  LaunchFn stream_thermal[kNumStream][3][2];
-  BatchFn stream_batch[kNumStream][2];          // [shape][fused]
+  // [shape][plain, axi, isotropic][stages - 1]: every stage count up to the shape's own
+  LaunchFn stream_acoustic[kNumStream][3][kMaxStages];
+  LaunchFn stream_thermal[kNumStream][3][kMaxStages];
+  BatchFn stream_batch[kNumStream][kMaxStages];          // [shape][stages - 1]
+  // persistent variant: a resident grid takes tiles from a work queue (grid chosen inside)
+  BatchFn stream_batch_wq[kNumStream][kMaxStages];
   LaunchFn stream_lossy[kNumStream][3];         // lossy acoustics, one step: [shape][variant]
   // resident 1D engine: [cells per thread][acoustic, lossy acoustic, thermal, nonlinear]
   LineFn line[4][4];

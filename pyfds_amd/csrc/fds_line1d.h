@@ -42,7 +42,7 @@ constexpr int kLineCpt[kNumLineCpt] = {1, 2, 4, 8};   // cells per thread of the
 constexpr int kLineMaxThreads[kNumLineCpt] = {1024, 1024, 512, 512};
 // the coupled kernel holds two fields: half the workgroup from 2 cells per thread on
 constexpr int kCoupleMaxThreads[kNumLineCpt] = {1024, 512, 512, 256};
-// the nonlinear models hold three components and call pow / expm1 / log1p
+// the nonlinear models hold three components and call pow_cr / expm1 / log1p
 constexpr int kNlLineMaxThreads[kNumLineCpt] = {512, 512, 512, 256};
 constexpr int kLineSteps = 2048;                   // time steps per launch at most
 // default layout: one workgroup up to kLineSegmentAbove cells; longer lines in one-wave
@@ -514,7 +514,7 @@ __device__ void line1d_nonlinear(const StepArgs<T> &a) {
       } else if (mode & kNlState) {
         if (sizeof(T) == 8) {
           const T ratio = (r0[m] + dr) / r0[m];
-          const T pw = (T)pow((double)ratio, (double)e2[m]);
+          const T pw = (T)crm::pow_cr((double)ratio, (double)e2[m]);  // correctly rounded
           pn = e1[m] * (pw - T(1));
         } else {
           const double xr = (double)dr / (double)r0[m];

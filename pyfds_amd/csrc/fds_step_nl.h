@@ -12,6 +12,7 @@
 #pragma once
 
 #include "fds_common.h"
+#include "fds_crmath.h"
 #include "../../include/pyfds_amd.h"
 
 namespace fds {
@@ -162,9 +163,9 @@ __device__ void nonlinear1d_tile(const StepArgs<T> &a, int tile, int64_t step, i
     } else if (mode & kNlState) {
       const T r0 = s0[c];
       if (sizeof(T) == 8) {
-        // float64: the reference's expression, term by term
+        // float64: the reference's expression, term by term, with the correctly rounded power
         const T ratio = (r0 + dr) / r0;
-        const T pw = (T)pow((double)ratio, (double)e2[j]);
+        const T pw = (T)crm::pow_cr((double)ratio, (double)e2[j]);
         p = e1[j] * (pw - T(1));
       } else {
         // float32: ((rho0 + rho')/rho0)**gamma - 1 cancels catastrophically for small rho';

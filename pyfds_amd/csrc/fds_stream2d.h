@@ -231,7 +231,8 @@ struct AcVel {
 // temporal allocation (pays off when the working set is far larger than the caches); EV: the
 // tile has events; CHK: some reads fall outside the arrays; ISO: the y coefficients are the x
 // ones (isotropic material arrays), so they are neither loaded nor held twice.
-template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool EV, bool CHK, bool ISO>
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool EV, bool CHK, bool ISO,
+          int PF>
 __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
                                StreamLds<T, TX, S> &L) {
   auto &sbp = L.bwd;    // Bx*p of the row, columns -1..TX
@@ -272,7 +273,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
 #pragma unroll
     for (int s = 0; s < S; ++s) cur[s] = stage_cursors(a, tile);
   }
-  constexpr int PF = 1, RS = PF + 1;   // stage 0 loads its rows one iteration ahead
+  constexpr int RS = PF + 1;   // stage 0 loads its rows PF iterations ahead
   AcRow<T> nin[RS] = {};       // stage 0 input rows (ring)
   AcHalo<T> nh[RS] = {};       // stage 0 halo
   T pp[2] = {}, cpp[2] = {};   // stage 0 input p after its events (previous-row p)
@@ -480,7 +481,8 @@ struct ThHalo {        // left thread: T, Q of column -1; right thread: T, Q, G 
   T t, ct, q, g;
 };
 
-template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool EV, bool CHK, bool ISO>
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool EV, bool CHK, bool ISO,
+          int PF>
 __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
                               StreamLds<T, TX, S> &L) {
   auto &sqt = L.bwd;    // Qx*T of the row
@@ -522,7 +524,7 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
 #pragma unroll
     for (int s = 0; s < S; ++s) cur[s] = stage_cursors(a, tile);
   }
-  constexpr int PF = 1, RS = PF + 1;
+  constexpr int RS = PF + 1;
   ThRow<T> nin[RS] = {};
   ThHalo<T> nh[RS] = {};
   T tp[2] = {}, ctp[2] = {};
@@ -690,14 +692,19 @@ __device__ __forceinline__ int strip_path(const StepArgs<T> &a, int tile) {
   return __builtin_amdgcn_readfirstlane((tile_has_events(a, tile) ? 2 : 0) | (inside ? 1 : 0));
 }
 
-#define FDS_STRIP_CASES(FN, AXI, ISO, BASE)                                                    \
-  case BASE + 0: FN<T, KAHAN, TX, AXI, S, NT, false, true, ISO>(a, tile, step, row, L); break;  \
-  case BASE + 1: FN<T, KAHAN, TX, AXI, S, NT, false, false, ISO>(a, tile, step, row, L); break; \
-  case BASE + 2: FN<T, KAHAN, TX, AXI, S, NT, true, true, ISO>(a, tile, step, row, L); break;   \
-  case BASE + 3: FN<T, KAHAN, TX, AXI, S, NT, true, false, ISO>(a, tile, step, row, L); break;
+#define FDS_STRIP_CASES(FN, AXI, ISO, BASE)                                                  \
+  case BASE + 0: FN<T, KAHAN, TX, AXI, S, NT, false, true, ISO, PF>(a, tile, step, row, L);   \
+    break;                                                                                     \
+  case BASE + 1: FN<T, KAHAN, TX, AXI, S, NT, false, false, ISO, PF>(a, tile, step, row, L);  \
+    break;                                                                                     \
+  case BASE + 2: FN<T, KAHAN, TX, AXI, S, NT, true, true, ISO, PF>(a, tile, step, row, L);    \
+    break;                                                                                     \
+  case BASE + 3: FN<T, KAHAN, TX, AXI, S, NT, true, false, ISO, PF>(a, tile, step, row, L);   \
+    break;
 
-// ISO kernels serve fields whose y coefficient arrays are the x ones (the host checks).
-template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool ISO>
+// ISO kernels serve fields whose y coefficient arrays are the x ones (the host checks).  PF:
+// rows of stage-0 input prefetched ahead (more loads in flight per wave, more registers).
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool ISO, int PF>
 __device__ __forceinline__ void acoustic_stream(const StepArgs<T> &a, int tile, int64_t step,
                                                 int64_t row, StreamLds<T, TX, S> &L) {
   switch (strip_path<T, TX>(a, tile)) {
@@ -706,7 +713,7 @@ __device__ __forceinline__ void acoustic_stream(const StepArgs<T> &a, int tile, 
   }
 }
 
-template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool ISO>
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool ISO, int PF>
 __device__ __forceinline__ void thermal_stream(const StepArgs<T> &a, int tile, int64_t step,
                                                int64_t row, StreamLds<T, TX, S> &L) {
   switch (strip_path<T, TX>(a, tile)) {
@@ -717,7 +724,7 @@ __device__ __forceinline__ void thermal_stream(const StepArgs<T> &a, int tile, i
 
 // One tile of instance `a` in a batched streaming launch (lossless models only): one switch
 // over model variant and path.
-template <typename T, bool KAHAN, int TX, int S, bool NT>
+template <typename T, bool KAHAN, int TX, int S, bool NT, int PF>
 __device__ __forceinline__ void any_stream(const StepArgs<T> &a, int tile, int64_t step,
                                            int64_t row, StreamLds<T, TX, S> &L) {
   const int f = a.flags;

@@ -1,6 +1,8 @@
 // Streaming step kernels of one numeric form and one shape of the streaming table.  The build
 // compiles this file once per form (FDS_FORM 0: float64, 1: float32, 2: float32 + Kahan) and
 // per shape index (FDS_SHAPE), so the shapes compile in parallel.
+#include <algorithm>
+
 #include "fds_kernels.h"
 #include "fds_stream2d.h"
 #include "fds_stream_lossy.h"
@@ -31,13 +33,14 @@ namespace FDS_FORM_NS {
 
 using T = FDS_T;
 constexpr bool K = FDS_KAHAN;
+constexpr int PF = kStreamShapes[FDS_SHAPE][4];   // rows prefetched ahead by stage 0
 
 template <int TX, bool AXI, int S, bool NT, bool ISO>
 __global__ __launch_bounds__(TX) void k_stream_acoustic(StepArgs<T> a) {
   __shared__ StreamLds<T, TX, S> L;
   const int tile = xcd_tile(blockIdx.x, gridDim.x, a.ntiles);
   if (tile < 0) return;
-  acoustic_stream<T, K, TX, AXI, S, NT, ISO>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
+  acoustic_stream<T, K, TX, AXI, S, NT, ISO, PF>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
 }
 
 template <int TX, bool AXI, int S, bool NT, bool ISO>
@@ -45,7 +48,7 @@ __global__ __launch_bounds__(TX) void k_stream_thermal(StepArgs<T> a) {
   __shared__ StreamLds<T, TX, S> L;
   const int tile = xcd_tile(blockIdx.x, gridDim.x, a.ntiles);
   if (tile < 0) return;
-  thermal_stream<T, K, TX, AXI, S, NT, ISO>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
+  thermal_stream<T, K, TX, AXI, S, NT, ISO, PF>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
 }
 
 template <int TX, bool AXI, bool NT, bool ISO>
@@ -66,7 +69,39 @@ __global__ __launch_bounds__(TX) void k_stream_batch(const StepArgs<T> *__restri
   if (g < 0) return;
   const int inst = tile_inst[g];
   const StepArgs<T> &a = descs[inst];
-  any_stream<T, K, TX, S, NT>(a, g - inst_base[inst], a.dyn[0] + k, a.dyn[1] + k, L);
+  any_stream<T, K, TX, S, NT, PF>(a, g - inst_base[inst], a.dyn[0] + k, a.dyn[1] + k, L);
+}
+
+// Persistent batch launch with a tile work queue: a resident grid of workgroups claims tiles
+// from a 64-bit counter until the launch's tiles are gone, in the order `order` (largest tiles
+// first).  Every workgroup makes one claim per tile it steps plus one final claim past the
+// end, so a launch advances the counter by exactly ntiles + gridDim.x: launch L of a run
+// (L = its first stage's step offset in the run / spl) owns the counter values from
+// L * (ntiles + gridDim.x) on, and no reset is needed between launches (the run's start
+// clears the counter).  Each workgroup exits when its claim passes the end.
+template <int TX, int S, bool NT>
+__global__ __launch_bounds__(TX) void k_stream_batch_wq(const StepArgs<T> *__restrict__ descs,
+                                                        const int32_t *__restrict__ tile_inst,
+                                                        const int32_t *__restrict__ inst_base,
+                                                        const int32_t *__restrict__ order,
+                                                        int ntiles, int k, int spl,
+                                                        unsigned long long *queue) {
+  __shared__ StreamLds<T, TX, S> L;
+  __shared__ long long claim;
+  const int64_t row0 = descs[0].dyn[1] + k;
+  const unsigned long long base =
+      (unsigned long long)(row0 / spl) * (unsigned long long)(ntiles + (int)gridDim.x);
+  for (;;) {
+    if (threadIdx.x == 0) claim = (long long)(atomicAdd(queue, 1ull) - base);
+    __syncthreads();
+    const long long t = claim;
+    __syncthreads();   // every thread has its claim (and left the previous tile's LDS rows)
+    if (t >= ntiles) break;
+    const int g = order[t];
+    const int inst = tile_inst[g];
+    const StepArgs<T> &a = descs[inst];
+    any_stream<T, K, TX, S, NT, PF>(a, g - inst_base[inst], a.dyn[0] + k, a.dyn[1] + k, L);
+  }
 }
 
 template <int TX, bool AXI, int S, bool NT, bool ISO>
@@ -94,8 +129,23 @@ void launch_stream_batch(const BatchLaunch &b, int nblocks, size_t, hipStream_t 
                      b.ntiles, b.k);
 }
 
+// Resident grid: as many workgroups as the device holds at once, never more than the tiles.
+template <int TX, int S, bool NT>
+void launch_stream_batch_wq(const BatchLaunch &b, int, size_t, hipStream_t s) {
+  static int per_cu = 0;
+  if (per_cu == 0 &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stream_batch_wq<TX, S, NT>, TX, 0)
+          != hipSuccess)
+    per_cu = 1;
+  const int grid = std::max(1, std::min(b.ntiles, std::max(1, per_cu) * b.num_cu));
+  hipLaunchKernelGGL((k_stream_batch_wq<TX, S, NT>), dim3(grid), dim3(TX), 0, s,
+                     static_cast<const StepArgs<T> *>(b.descs), b.tile_inst, b.inst_base,
+                     b.order, b.ntiles, b.k, b.spl, b.queue);
+}
+
 // Variants [v]: 0 plain, 1 axial-symmetric, 2 isotropic coefficients (y arrays = x arrays, not
-// axial-symmetric).  Fused [.][1]: the shape's stage count; [.][0]: one stage (remainder steps).
+// axial-symmetric).  [.][f]: f + 1 stages per launch, for f + 1 from 1 (remainder steps) to the
+// shape's own stage count; fewer stages than the tiling's ghost depth allows are exact too.
 template <int TX, int S, bool NT>
 void fill_variants(KernelSet &ks, int I, int f) {
   ks.stream_acoustic[I][0][f] = launch_stream_acoustic<TX, false, S, NT, false>;
@@ -105,6 +155,7 @@ void fill_variants(KernelSet &ks, int I, int f) {
   ks.stream_thermal[I][1][f] = launch_stream_thermal<TX, true, S, NT, false>;
   ks.stream_thermal[I][2][f] = launch_stream_thermal<TX, false, S, NT, true>;
   ks.stream_batch[I][f] = launch_stream_batch<TX, S, NT>;
+  ks.stream_batch_wq[I][f] = launch_stream_batch_wq<TX, S, NT>;
   if (f == 0) {   // lossy acoustics: one step per launch on every shape
     ks.stream_lossy[I][0] = launch_stream_lossy<TX, false, NT, false>;
     ks.stream_lossy[I][1] = launch_stream_lossy<TX, true, NT, false>;
@@ -118,8 +169,10 @@ void FDS_CAT(fill_stream_, FDS_SHAPE)(KernelSet &ks) {
   constexpr int TX = kStreamShapes[I][0];
   constexpr int S = kStreamShapes[I][2];
   constexpr bool NT = kStreamShapes[I][3] != 0;
-  fill_variants<TX, S, NT>(ks, I, 1);
+  static_assert(S >= 1 && S <= kMaxStages, "stage count out of range");
   fill_variants<TX, 1, NT>(ks, I, 0);
+  if constexpr (S >= 2) fill_variants<TX, 2, NT>(ks, I, 1);
+  if constexpr (S >= 3) fill_variants<TX, 3, NT>(ks, I, 2);
 }
 
 }  // namespace FDS_FORM_NS

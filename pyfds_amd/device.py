@@ -144,6 +144,58 @@ def compile_boundaries(boundaries, n):
                            np.ascontiguousarray(np.concatenate(table), dtype=np.float64))
 
 
+def _copy_value(v):
+    """A boundary value as the program compiler reads it, copied (scalar, signal array or
+    per-point signals), to detect in-place changes between runs."""
+    if np.ndim(v) == 0:
+        return ('s', float(v))
+    if isinstance(v, np.ndarray):
+        return ('a', v.copy())
+    return ('l', [np.array(x, dtype=np.float64, copy=True) for x in v])
+
+
+def _same_value(v, snap):
+    kind, data = snap
+    if kind == 's':
+        return np.ndim(v) == 0 and float(v) == data
+    if kind == 'a':
+        return isinstance(v, np.ndarray) and v.shape == data.shape and v.dtype == data.dtype \
+            and np.array_equal(v, data)
+    if np.ndim(v) == 0 or isinstance(v, np.ndarray) or len(v) != len(data):
+        return False
+    return all(np.array_equal(np.asarray(x, dtype=np.float64), d) for x, d in zip(v, data))
+
+
+def _snapshot(items, value_of=None):
+    """What a boundary or output list holds, by identity and by content: the objects, their
+    regions, copies of the index arrays and (boundaries) of the values."""
+    snap = []
+    for it in items:
+        idx = it.region.indices
+        entry = (it, it.region, None, np.array(idx, copy=True))
+        if value_of is not None:
+            entry += (bool(it.additive), it.value, _copy_value(it.value))
+        snap.append(entry)
+    return snap
+
+
+def _unchanged(items, snap, boundaries):
+    """True when ``items`` still hold exactly what ``snap`` recorded (same objects, regions,
+    index values and, for boundaries, additive flags and values)."""
+    if snap is None or len(items) != len(snap):
+        return False
+    for it, entry in zip(items, snap):
+        if it is not entry[0] or it.region is not entry[1]:
+            return False
+        arr = np.asarray(it.region.indices)
+        if arr.shape != entry[3].shape or not np.array_equal(arr, entry[3]):
+            return False
+        if boundaries:
+            if bool(it.additive) != entry[4] or not _same_value(it.value, entry[6]):
+                return False
+    return True
+
+
 class DeviceRunner:
     """Owns the ``fds_ctx`` of one field."""
 
@@ -152,10 +204,13 @@ class DeviceRunner:
         self.ctx = None
         self.coef_version = None
         self.programs = [None, None, None]
+        self.bsnap = [None, None, None]     # boundary lists the programs were compiled from
+        self.osnap = [None, None, None]     # output lists the recorded cells came from
         self.out_cells = [None, None, None]
         self.out_layout = [[], [], []]
         self.owners = [None, None, None]
         self.flow = None
+        self.nf_check = False
         self.chunk = getattr(field, 'graph_chunk', 50)
         self.tile = getattr(field, 'tile', None)
         self.strip_height = getattr(field, 'strip_height', None)
@@ -215,6 +270,10 @@ class DeviceRunner:
             nat.check(lib.fds_set_scalars(ctx, nat.dptr(s), s.size))
             nat.check(lib.fds_commit(ctx))
             self.coef_version = field._coef_version
+        want = bool(getattr(field, 'check_finite', False))
+        if want != self.nf_check:
+            nat.check(lib.fds_set_nonfinite_check(ctx, int(want)))
+            self.nf_check = want
         periods = getattr(field, '_device_flow_periods', None)
         if periods is not None:
             p = nat.i64(periods())
@@ -235,25 +294,33 @@ class DeviceRunner:
                 comp._host_dirty = False
                 comp._device = None
                 self.owners[k] = comp
-            prog = compile_boundaries(comp.boundaries, self.n)
-            if not prog.same_as(self.programs[k]):
-                nat.check(lib.fds_set_boundaries(
-                    ctx, k, prog.cells.size, nat.lptr(prog.cells), nat.iptr(prog.ptr),
-                    prog.a.size, nat.dptr(prog.a), nat.lptr(prog.off), nat.lptr(prog.stride),
-                    nat.lptr(prog.len), prog.table.size, nat.dptr(prog.table)))
-                self.programs[k] = prog
-            layout, cells, start = [], [], 0
-            for out in comp.outputs:
-                idx = normalise_indices(out.region.indices, self.n, 'output')
-                layout.append((out, start, idx.size))
-                cells.append(idx)
-                start += idx.size
-            cells = np.concatenate(cells) if cells else np.zeros(0, np.int64)
-            if self.out_cells[k] is None or not np.array_equal(cells, self.out_cells[k]):
-                cells = nat.i64(cells)
-                nat.check(lib.fds_set_outputs(ctx, k, cells.size, nat.lptr(cells)))
-                self.out_cells[k] = cells
-            self.out_layout[k] = layout
+            # boundary program: recompiled only when the list, a region, its indices, a flag
+            # or a value (signals are compared by content: users may edit them in place)
+            # changed since the last upload
+            if not _unchanged(comp.boundaries, self.bsnap[k], True):
+                prog = compile_boundaries(comp.boundaries, self.n)
+                if not prog.same_as(self.programs[k]):
+                    nat.check(lib.fds_set_boundaries(
+                        ctx, k, prog.cells.size, nat.lptr(prog.cells), nat.iptr(prog.ptr),
+                        prog.a.size, nat.dptr(prog.a), nat.lptr(prog.off),
+                        nat.lptr(prog.stride), nat.lptr(prog.len), prog.table.size,
+                        nat.dptr(prog.table)))
+                    self.programs[k] = prog
+                self.bsnap[k] = _snapshot(comp.boundaries, True)
+            if not _unchanged(comp.outputs, self.osnap[k], False) or self.out_cells[k] is None:
+                layout, cells, start = [], [], 0
+                for out in comp.outputs:
+                    idx = normalise_indices(out.region.indices, self.n, 'output')
+                    layout.append((out, start, idx.size))
+                    cells.append(idx)
+                    start += idx.size
+                cells = np.concatenate(cells) if cells else np.zeros(0, np.int64)
+                if self.out_cells[k] is None or not np.array_equal(cells, self.out_cells[k]):
+                    cells = nat.i64(cells)
+                    nat.check(lib.fds_set_outputs(ctx, k, cells.size, nat.lptr(cells)))
+                    self.out_cells[k] = cells
+                self.out_layout[k] = layout
+                self.osnap[k] = _snapshot(comp.outputs)
         return comps
 
     # -- running --------------------------------------------------------------------------
@@ -285,11 +352,14 @@ class DeviceRunner:
             queue.append(k)
             start[0] += k
 
+        nonfinite = []
+
         def take():
             k = queue[0]
             data = [np.empty((k, m)) if m else None for m in counts]
-            nat.check(lib.fds_run_take(ctx, *[None if d is None else nat.dptr(d)
-                                               for d in data]))
+            rc = lib.fds_run_take(ctx, *[None if d is None else nat.dptr(d) for d in data])
+            if rc != nat.E_NONFINITE:
+                nat.check(rc)
             # the device has run these steps: count them before anything else can fail
             queue.popleft()
             first = field.step
@@ -298,22 +368,38 @@ class DeviceRunner:
             if progress is not None:
                 for s in range(first, first + k):
                     progress.log(s)
+            if rc == nat.E_NONFINITE:
+                nonfinite.append(nat.error_message())
 
         try:
             enqueue(sizes[0])
             for k in sizes[1:]:
+                if nonfinite:          # a non-finite value was found: enqueue no more steps
+                    break
                 enqueue(k)
                 take()
-            take()
+            if queue:
+                take()
         finally:
             # after an error, keep the host in step with what the device ran
             while queue:
                 take()
+        if nonfinite:
+            raise FloatingPointError(nonfinite[0])
 
     def run_steps(self, field, start, nsteps):
         """Run ``nsteps`` steps from pyfds step ``start``; append the recorded outputs."""
         self.sync(field)
-        nat.check(self.lib.fds_run(self.ctx, start, nsteps, int(self.chunk)))
+        self._run(field, start, nsteps)
+
+    def _run(self, field, start, nsteps):
+        """``run_steps`` after ``sync``."""
+        rc = self.lib.fds_run(self.ctx, start, nsteps, int(self.chunk))
+        if rc == nat.E_NONFINITE:       # the run completed: keep its outputs, then raise
+            msg = nat.error_message()
+            self.collect(field, nsteps)
+            raise FloatingPointError(msg)
+        nat.check(rc)
         self.collect(field, nsteps)
 
     def collect(self, field, nsteps):
@@ -375,14 +461,23 @@ class DeviceRunner:
         valid = self._valid_steps()
         n_ok = num_steps if valid is None else max(0, min(num_steps, valid - start))
         seg = self._segment(num_steps, progress)
-        sizes = pipeline_sizes(n_ok, seg, int(self.chunk), self._output_count(),
+        m = self._output_count()
+        sizes = pipeline_sizes(n_ok, seg, int(self.chunk), m,
                                nat.check(self.lib.fds_steps_per_launch(self.ctx)))
+        if sizes is None and m and n_ok:
+            # runs with outputs take the asynchronous path even unpipelined: the outputs come
+            # back through pinned staging behind the last step instead of a copy afterwards
+            sizes = [seg] * (n_ok // seg) + ([n_ok % seg] if n_ok % seg else [])
         if sizes:
             self._run_pipelined(field, sizes, progress)
         done = n_ok if sizes else 0
         while done < n_ok:
             k = min(seg, n_ok - done)
-            self.run_steps(field, field.step, k)
+            try:
+                self._run(field, field.step, k)
+            except FloatingPointError:
+                field.step += k          # the device ran these steps
+                raise
             if progress is not None:
                 for s in range(field.step, field.step + k):
                     progress.log(s)

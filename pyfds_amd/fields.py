@@ -15,6 +15,13 @@ The classes keep the reference's constructors, attributes and methods
   reference's order.
 * two optional keyword-only constructor arguments select the numeric form
   (``form='fp64' | 'fp32' | 'fp32_kahan'``) and the HIP device.
+* ``check_finite = True`` on a field makes its device runs scan the state for NaN and
+  infinity (the first component after every graph chunk of steps, every component after the
+  run); ``simulate`` then raises ``FloatingPointError`` naming the component and the step by
+  which it was found, after the steps run so far are counted and their outputs recorded (a
+  pipelined run stops enqueueing further steps).  Off by default: the reference propagates
+  non-finite values silently.
+* ``sim`` is a short name of ``simulate``.
 """
 import logging as lo
 
@@ -28,6 +35,8 @@ logger = lo.getLogger('pyfds')
 
 class Field:
     """Base class for all fields (``pyfds/fields.py:9-127``)."""
+
+    check_finite = False
 
     def __init__(self):
         self.material_regions = []
@@ -102,6 +111,10 @@ class Field:
                 self.step += 1
 
         logger.info('Simulation of {} steps completed.'.format(num_steps))
+
+    def sim(self, num_steps=None):
+        """Short name of :meth:`simulate` (same arguments and behaviour)."""
+        return self.simulate(num_steps)
 
     def _runner(self):
         runner = self.__dict__.get('_device_runner')

@@ -3,7 +3,12 @@ tests/scenes.py and tests/coupled_scenes.py.
 
 Run in the development container only (it reads the reference checkout):
 
-    python tests/golden/make_golden.py /root/reference
+    python tests/golden/make_golden.py /root/reference [--only NAME ...]
+
+The scenes in ``scenes.POW_SCENES`` (IdealGas1D's nonlinear equation of state, whose numpy
+power rounds differently on AVX-512 hosts) are written a second time as ``<name>_libm.npz``
+by a child process that starts numpy with its AVX-512 dispatch disabled
+(``NPY_DISABLE_CPU_FEATURES``), so numpy's power is glibc's ``pow`` there.
 
 What it pins: time-evolved FP64 results of every hot-path model (component
 values after the run plan and every recorded output signal), including
@@ -17,31 +22,50 @@ against stale data.  Fixtures are data only: inputs are rebuilt from
 tests/scenes.py.
 """
 import os
+import subprocess
 import sys
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 
-def main(reference_root):
+AVX512 = 'AVX512F AVX512CD AVX512_SKX AVX512_CLX AVX512_CNL AVX512_ICL'
+
+
+def main(reference_root, only=None, libm=False):
     import matplotlib
     matplotlib.use('Agg')
     sys.path.insert(0, reference_root)
     import pyfds
     import pyfds.regions as regions
     import scenes
+    from oracle import numpy_power_is_libm
 
-    for name, make in scenes.SCENES.items():
-        scene = make()
+    if libm and not numpy_power_is_libm():
+        raise SystemExit('numpy power is not glibc pow in this process')
+    names = scenes.POW_SCENES if libm else list(scenes.SCENES)
+    for name in names:
+        if only and name not in only:
+            continue
+        scene = scenes.SCENES[name]()
         fld = scenes.build_field(pyfds, regions, scene)
         scenes.run_plan(fld, scene)
         res = scenes.collect(fld, scene)
         res['fingerprint'] = np.array(scenes.fingerprint(scene))
-        path = os.path.join(HERE, name + '.npz')
+        path = os.path.join(HERE, name + ('_libm' if libm else '') + '.npz')
         np.savez_compressed(path, **res)
         print('wrote', os.path.relpath(path), 'step', int(res['step']))
+    if libm:
+        return
+    if not only or any(n in only for n in scenes.POW_SCENES):
+        env = dict(os.environ, NPY_DISABLE_CPU_FEATURES=AVX512)
+        subprocess.run([sys.executable, os.path.abspath(__file__), reference_root, '--libm']
+                       + (['--only'] + list(only) if only else []), env=env, check=True)
+    if only and not any(n.startswith('coupled_') for n in only):
+        return
 
     # coupled fields and flow (tests/coupled_scenes.py)
     import coupled_scenes as cs
@@ -56,4 +80,11 @@ def main(reference_root):
 
 
 if __name__ == '__main__':
-    main(sys.argv[1] if len(sys.argv) > 1 else '/root/reference')
+    args = sys.argv[1:]
+    libm = '--libm' in args
+    only = None
+    if '--only' in args:
+        only = args[args.index('--only') + 1:]
+        args = args[:args.index('--only')]
+    args = [a for a in args if a != '--libm']
+    main(args[0] if args else '/root/reference', only, libm)

@@ -40,14 +40,17 @@ def test_tile_table():
     assert all(ty > 1 for _, ty in shapes[first:])
     lib = _native.load()
     assert lib.fds_tile_engine(len(shapes)) < 0 and lib.fds_tile_engine(-1) < 0
-    # the streaming table the GPU tests pick slots from: (tx, ty, steps per launch, nt)
-    stream = [(d['tx'], d['ty'], d['steps'], d['nontemporal'])
+    # the streaming table the GPU tests pick slots from: (tx, ty, steps per launch, nt,
+    # prefetch depth)
+    stream = [(d['tx'], d['ty'], d['steps'], d['nontemporal'], d['prefetch'])
               for d in _native.tile_table() if d['engine'] == 'stream']
-    assert stream == [(256, 16, 1, False), (256, 16, 2, False), (128, 32, 2, False),
-                      (256, 32, 3, False), (256, 16, 1, True), (256, 16, 2, True),
-                      (256, 32, 3, True), (64, 16, 1, False), (64, 16, 2, False),
-                      (64, 32, 3, False), (64, 16, 1, True), (64, 16, 2, True)]
-    assert all(d['steps'] == 1 and not d['nontemporal']
+    assert stream == [(256, 16, 1, False, 1), (256, 16, 2, False, 1), (128, 32, 2, False, 1),
+                      (256, 32, 3, False, 1), (256, 16, 1, True, 1), (256, 16, 2, True, 1),
+                      (256, 32, 3, True, 1), (64, 16, 1, False, 1), (64, 16, 2, False, 1),
+                      (64, 32, 3, False, 1), (64, 16, 1, True, 1), (64, 16, 2, True, 1),
+                      (64, 32, 3, False, 2), (64, 16, 2, False, 2), (256, 32, 3, True, 2),
+                      (64, 32, 3, True, 2)]
+    assert all(d['steps'] == 1 and not d['nontemporal'] and d['prefetch'] == 0
                for d in _native.tile_table() if d['engine'] == 'tile')
 
 

@@ -1,5 +1,7 @@
 """Learned absorbing boundary: the CPU conv oracle is pinned (adjoints, finite differences,
-dense brute force); the BF16-MFMA training kernels match it; the fused per-step inference
+dense brute force; Adam against torch.optim.Adam and its first-step closed form; bf16
+rounding against torch's cast, on values where truncation differs); the BF16-MFMA training
+kernels match it; the fused per-step inference
 kernel writes the layer the oracle predicts; a trained network beats the same grid without
 it."""
 import ctypes as ct
@@ -45,11 +47,88 @@ def test_mse_gradient_finite_differences():
         assert np.isclose(fd, grad[idx], rtol=1e-4, atol=1e-8)
 
 
-def test_bf16_round_trip():
-    a = np.array([1.0, -2.5, 3.14159, 1e-3, 65504.0], np.float32)
-    r = conv.from_bf16_bits(conv.to_bf16_bits(a))
-    assert np.all(np.abs(r - a) <= np.abs(a) * 2 ** -8)
-    assert conv.from_bf16_bits(conv.to_bf16_bits(np.float32(1.0)))[()] == 1.0
+def _torch_bf16_bits(a):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.bfloat16)
+    return t.view(torch.int16).numpy().view(np.uint16)
+
+
+# values where round-to-nearest-even, round-half-up and truncation disagree
+BF16_EDGES = np.array([65504.0, 1 + 2 ** -8 + 2 ** -16, 1 + 2 ** -8, 1 + 3 * 2 ** -8,
+                       -(1 + 2 ** -8 + 2 ** -16), 3.14159, 1e-3, -2.5, 1.0, 2 ** -130,
+                       3.0e38], np.float32)
+
+
+def _bf16_cases():
+    rng = np.random.default_rng(3)
+    return np.concatenate([BF16_EDGES, rng.standard_normal(20000).astype(np.float32) * 1e3,
+                           rng.standard_normal(20000).astype(np.float32) * 1e-20])
+
+
+def test_bf16_rounding_matches_torch():
+    a = _bf16_cases()
+    np.testing.assert_array_equal(conv.to_bf16_bits(a), _torch_bf16_bits(a))
+    r = conv.from_bf16_bits(conv.to_bf16_bits(BF16_EDGES[:4]))
+    np.testing.assert_array_equal(r, np.array([65536.0, 1 + 2 ** -7, 1.0, 1 + 2 ** -6],
+                                              np.float32))
+
+
+def test_bf16_planted_truncation_is_caught():
+    a = _bf16_cases()
+    truncated = (np.ascontiguousarray(a).view(np.uint32) >> 16).astype(np.uint16)
+    assert np.count_nonzero(truncated != _torch_bf16_bits(a)) > 0
+
+
+def _torch_adam(w, grads, lr, b1, b2, eps):
+    import torch
+    p = torch.nn.Parameter(torch.from_numpy(w.copy()))
+    opt = torch.optim.Adam([p], lr=lr, betas=(b1, b2), eps=eps)
+    out = []
+    for g in grads:
+        p.grad = torch.from_numpy(g.copy())
+        opt.step()
+        out.append(p.detach().numpy().copy())
+    return out
+
+
+def _oracle_adam(w, grads, lr, b1, b2, eps, step_fn=None):
+    step_fn = step_fn or conv.adam_step
+    m1, m2, out = np.zeros_like(w), np.zeros_like(w), []
+    for i, g in enumerate(grads):
+        w, m1, m2 = step_fn(w, m1, m2, g, lr, b1, b2, eps, i + 1)
+        out.append(w.copy())
+    return out
+
+
+ADAM = dict(lr=3e-3, b1=0.9, b2=0.999, eps=1e-8)
+
+
+def _adam_case():
+    rng = np.random.default_rng(4)
+    w = rng.standard_normal(500)
+    return w, [rng.standard_normal(500) * s for s in (1.0, 0.3, 2.0)]
+
+
+def test_adam_oracle_matches_torch_and_first_step_closed_form():
+    w, grads = _adam_case()
+    ours, ref = _oracle_adam(w, grads, **ADAM), _torch_adam(w, grads, **ADAM)
+    for a, b in zip(ours, ref):
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-14)
+    # step 1: bias-corrected moments are g and g**2, so w1 = w - lr g / (|g| + eps)
+    g = grads[0]
+    np.testing.assert_allclose(ours[0], w - ADAM['lr'] * g / (np.abs(g) + ADAM['eps']),
+                               rtol=0, atol=1e-15)
+
+
+def test_adam_planted_mistake_is_caught():
+    # the second moment without its bias correction: the same comparison must fail
+    def planted(w, m1, m2, g, lr, b1, b2, eps, step):
+        m1 = b1 * m1 + (1 - b1) * g
+        m2 = b2 * m2 + (1 - b2) * g * g
+        return w - lr * (m1 / (1 - b1 ** step)) / (np.sqrt(m2) + eps), m1, m2
+    w, grads = _adam_case()
+    ours, ref = _oracle_adam(w, grads, step_fn=planted, **ADAM), _torch_adam(w, grads, **ADAM)
+    assert max(np.abs(a - b).max() for a, b in zip(ours, ref)) > 1e-6
 
 
 def test_layer_cells_are_written_once():

@@ -8,9 +8,10 @@ recorded points per thread, runs that cross the 2048-step launch limit, ``sim_st
 equal the LDS-tile engine's bits (both evaluate the same expressions in the same order); the
 engine choice follows the line length and the tile setting.  The nonlinear models: FP64
 bit-identical to the oracle (linear and second-order equations of state), and every form equal
-to the nonlinear tile kernel's bits (also the ideal-gas state, whose pow() comes from the device
-libm and is not pinned by the host oracle).  The reference's own 1D scenes (golden fixtures)
-run on this engine by default in ``test_device_parity.py``.
+to the nonlinear tile kernel's bits.  The ideal-gas state's power is the correctly rounded one
+(``fds_crmath.h``): its FP64 runs equal the oracle with ``pow_cr`` bit for bit, on one
+workgroup and on segments.  The reference's own 1D scenes (golden fixtures) run on this engine
+by default in ``test_device_parity.py``.
 """
 import numpy as np
 import pytest
@@ -132,7 +133,7 @@ NL_SCENES = {
     'ig1d_clin_700': lambda: line_nonlinear('IdealGas1D', 700, 2300, 7, True, False),
     'ig1d_nlin_2048': lambda: line_nonlinear('IdealGas1D', 2048, 2200, 8, False, False),
     'a2o1d_1000': lambda: line_nonlinear('Acoustic2ndOrder1D', 1000, 2200, 9),
-    # the nonlinear equation of state calls pow (device libm): engine against engine only
+    # the nonlinear equation of state (correctly rounded power)
     'ig1d_cnl_900': lambda: line_nonlinear('IdealGas1D', 900, 2200, 10, True, True),
 }
 NL_MAX_THREADS = {1: 512, 2: 512, 4: 512, 8: 256}
@@ -214,7 +215,7 @@ def test_engine_choice():
         field(100, strip_height=3)
 
 
-NL_CASES = [(name, c) for name, make in NL_SCENES.items() if 'cnl' not in name
+NL_CASES = [(name, c) for name, make in NL_SCENES.items()
             for c in (0,) + CPT if c == 0 or -(-_n(make()) // c) <= NL_MAX_THREADS[c]]
 
 
@@ -222,7 +223,7 @@ NL_CASES = [(name, c) for name, make in NL_SCENES.items() if 'cnl' not in name
 def test_nonlinear_line_fp64_matches_oracle_bitwise(name, cpt):
     scene = NL_SCENES[name]()
     print('nonlinear line scene', name, 'cells per thread', cpt, flush=True)
-    ref = scenes.run_oracle(scene)
+    ref = scenes.run_oracle(scene, power='cr')
     res, eng = run(scene, strip_height=cpt)
     assert eng == 'line'
     for k in ref:
@@ -264,6 +265,7 @@ SEG_SCENES = {
     'a1d_lossy_9000': lambda: line_acoustic(9000, 2200, True, 11),
     'ig1d_clin_700': NL_SCENES['ig1d_clin_700'],
     'a2o1d_1000': NL_SCENES['a2o1d_1000'],
+    'ig1d_cnl_900': NL_SCENES['ig1d_cnl_900'],
 }
 
 
@@ -272,7 +274,7 @@ SEG_SCENES = {
 def test_segmented_line_fp64_matches_oracle_bitwise(name, cfg):
     scene = SEG_SCENES[name]()
     print('segmented line scene', name, cfg, flush=True)
-    ref = scenes.run_oracle(scene)
+    ref = scenes.run_oracle(scene, power='cr')
     res, eng = run(scene, line_shape=cfg)
     assert eng == 'line'
     for k in ref:
@@ -290,3 +292,17 @@ def test_segmented_line_fp32_forms_equal_one_segment(name, form):
         for k in one:
             np.testing.assert_array_equal(seg[k], one[k],
                                           err_msg='%s %s %s %s' % (name, form, shape, k))
+
+
+@pytest.mark.parametrize('engine', ['line', 'nonlinear'])
+def test_ideal_gas_long_run_matches_cr_oracle_bitwise(engine):
+    # IdealGas1D, nonlinear state, 2048 cells x 2000 steps: every one of the ~4 M powers must
+    # be the correctly rounded one, or the run drifts (a one-ulp power error grows to ~1e-10
+    # relative L2 over this run)
+    scene = scenes.scene_ig1d_long()
+    print('ideal gas long run', engine, flush=True)
+    ref = scenes.run_oracle(scene, power='cr')
+    res, eng = run(scene, **({} if engine == 'line' else dict(tile=0)))
+    assert eng == engine
+    for k in ref:
+        np.testing.assert_array_equal(res[k], ref[k], err_msg='%s %s' % (engine, k))
