@@ -242,6 +242,7 @@ struct fds_batch {
   // XCD-banded), 1 a persistent grid taking tiles from a work queue in `order`
   int schedule = 0;
   DevBuf order, queue;
+  int nq = 0;             // work-queue counters the queue buffer holds (one per launch)
 };
 
 namespace {
@@ -434,6 +435,12 @@ int choose_tiles(fds_ctx *c) {
                                                              : fds::kDefaultLossless2D);
     if (t >= fds::kNumTiles && !can_stream(c))
       t = (c->ny == 1) ? fds::kTileRow : fds::kDefaultTile2D;
+    // lossy acoustics streams one step per launch with its own row ring: shapes that only add
+    // prefetch depth have no lossy kernels
+    if (t >= fds::kNumTiles && c->lossy && c->family == kAcoustic &&
+        fds::kStreamShapes[t - fds::kNumTiles][4] > 1)
+      return fail(FDS_E_ARG, "shape %d (prefetch %d rows) serves lossless fields only", t,
+                  fds::kStreamShapes[t - fds::kNumTiles][4]);
     c->shape = t;
     c->streaming = t >= fds::kNumTiles;
     shape_of(t, &c->tx, &c->ty);
@@ -692,8 +699,9 @@ __global__ void k_advance(int64_t *dyn, int64_t by) {
   }
 }
 
-__global__ void k_queue_reset(unsigned long long *q) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) q[0] = 0ull;
+// Zero the first n work-queue counters (one per launch of a run).
+__global__ __launch_bounds__(256) void k_queue_reset(unsigned *q, int n) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) q[i] = 0u;
 }
 
 // Non-finite detection.  dyn[kDynNonfinite] holds the smallest (step * 4 + component) over the
@@ -1779,7 +1787,8 @@ int batch_launch(fds_batch *b, int rel_parity, int k, bool store, int stages = 1
   L.ntiles = b->ntiles;
   L.k = k;
   L.order = b->order.as<const int32_t>();
-  L.queue = b->queue.as<unsigned long long>();
+  L.queue = b->queue.as<unsigned>();
+  L.nq = b->nq;
   L.spl = b->fuse;
   L.num_cu = b->ctxs[0]->num_cu;
   if (b->streaming && b->schedule == 1)
@@ -1846,9 +1855,19 @@ int batch_prepare(fds_batch *b, int64_t start, int64_t nsteps) {
   if (rc) return rc;
   hipLaunchKernelGGL(k_set_dyn, dim3(1), dim3(64), 0, b->stream, b->dyn.as<int64_t>(), start);
   HIP_TRY(hipGetLastError());
-  if (b->schedule == 1) {   // the work queue starts the run empty
-    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, b->stream,
-                       b->queue.as<unsigned long long>());
+  if (b->schedule == 1) {   // one empty counter per launch of the run
+    const int64_t need = nsteps / std::max(1, b->fuse) + 2;
+    if (need > std::numeric_limits<int32_t>::max())
+      return fail(FDS_E_ARG, "run too long for the work queue");
+    if (b->nq < need) {
+      // a larger buffer (batch_steady made runs in flight finish first)
+      HIP_TRY(b->queue.ensure((size_t)need * 2 * sizeof(unsigned)));
+      b->nq = (int)(b->queue.bytes / sizeof(unsigned));
+      batch_drop_graphs(b);   // captured launches hold the old buffer
+    }
+    const unsigned grid = (unsigned)std::min<int64_t>((need + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_queue_reset, dim3(grid), dim3(256), 0, b->stream,
+                       b->queue.as<unsigned>(), (int)need);
     HIP_TRY(hipGetLastError());
   }
   for (auto *c : b->ctxs)
@@ -1864,6 +1883,7 @@ int batch_prepare(fds_batch *b, int64_t start, int64_t nsteps) {
 bool batch_steady(const fds_batch *b, int64_t nsteps) {
   for (const auto *c : b->ctxs)
     if (!run_steady(c, nsteps)) return false;
+  if (b->schedule == 1 && b->nq < nsteps / std::max(1, b->fuse) + 2) return false;
   return !batch_stale(b);
 }
 
@@ -1885,8 +1905,10 @@ int batch_enqueue(fds_batch *b, int64_t nsteps, int chunk, hipEvent_t e0, hipEve
     }
   }
   const int64_t graph_done = done;   // steps the batch step counter advanced by
+  // full launches, then the remainder one stage at a time (batch kernels exist for the shape's
+  // stage count and for one stage)
   for (int64_t k = 0; done + k < nsteps;) {
-    const int st = (int)std::min<int64_t>(g, nsteps - done - k);
+    const int st = nsteps - done - k >= g ? g : 1;
     if (int rc2 = batch_launch(b, rel, (int)k, done + k + st == nsteps, st)) return rc2;
     rel ^= 1;
     k += st;
@@ -1966,6 +1988,11 @@ int fds_batch_create(fds_batch **out, fds_ctx *const *ctxs, int n, int tile) {
     return fail(FDS_E_ARG, "tile index %d out of range", t);
   }
   if (t >= fds::kNumTiles && !all_stream) t = fds::kDefaultTile2D;
+  if (t >= fds::kNumTiles && fds::kStreamShapes[t - fds::kNumTiles][4] > 1) {
+    fds_batch_destroy(b);
+    return fail(FDS_E_ARG, "shape %d (prefetch %d rows) serves single fields only", t,
+                fds::kStreamShapes[t - fds::kNumTiles][4]);
+  }
   std::vector<int32_t> tinst, base;
   size_t lds = 0;
   for (int i = 0; i < n; ++i) {
@@ -2004,8 +2031,7 @@ int fds_batch_create(fds_batch **out, fds_ctx *const *ctxs, int n, int tile) {
                    [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
   if (upload(b->tile_inst, tinst, b->stream) != hipSuccess ||
       upload(b->inst_base, base, b->stream) != hipSuccess || b->dyn.ensure(64) != hipSuccess ||
-      upload(b->order, order, b->stream) != hipSuccess ||
-      b->queue.ensure(sizeof(unsigned long long)) != hipSuccess) {
+      upload(b->order, order, b->stream) != hipSuccess) {
     fds_batch_destroy(b);
     return fail(FDS_E_HIP, "out of device memory for the batch tables");
   }

@@ -20,16 +20,16 @@ constexpr int kTileRow = 5;
 // PF = input rows prefetched ahead (loads in flight per wave); TX = 64 runs one wave per
 // workgroup and exchanges neighbours by lane shuffles.  Shape indices continue the tile table:
 // index kNumTiles + i is kStreamShapes[i] = {TX, TY, S, NT, PF}.
-constexpr int kNumStream = 16;
+constexpr int kNumStream = 15;
 constexpr int kStreamShapes[kNumStream][5] = {
     {256, 16, 1, 0, 1}, {256, 16, 2, 0, 1}, {128, 32, 2, 0, 1}, {256, 32, 3, 0, 1},
     {256, 16, 1, 1, 1}, {256, 16, 2, 1, 1}, {256, 32, 3, 1, 1}, {64, 16, 1, 0, 1},
     {64, 16, 2, 0, 1},  {64, 32, 3, 0, 1},  {64, 16, 1, 1, 1},  {64, 16, 2, 1, 1},
-    {64, 32, 3, 0, 2},  {64, 16, 2, 0, 2},  {256, 32, 3, 1, 2}, {64, 32, 3, 1, 2}};
+    {64, 32, 3, 0, 2},  {64, 16, 2, 0, 2},  {256, 32, 3, 1, 2}};
 constexpr int kMaxStages = 3;   // largest S in kStreamShapes
 // X-macro over the streaming shape indices (each shape is its own translation unit).
 #define FDS_STREAM_SHAPES(X) \
-  X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15)
+  X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14)
 
 using LaunchFn = void (*)(const void *args, int nblocks, size_t lds_bytes, hipStream_t s);
 // resident 1D launches: `grid` workgroups (segments) of `threads` threads; the coupled one
@@ -47,10 +47,12 @@ struct BatchLaunch {
   const int32_t *inst_base;
   int ntiles;
   int k;
-  // tile work queue (persistent launches): hand-out order of the global tiles, the claim
-  // counter, the steps of a full launch and the device's compute units
+  // tile work queue (persistent launches): hand-out order of the global tiles, one claim
+  // counter per launch of the run (nq of them), the steps of a full launch and the device's
+  // compute units
   const int32_t *order;
-  unsigned long long *queue;
+  unsigned *queue;
+  int nq;
   int spl;
   int num_cu;
 };

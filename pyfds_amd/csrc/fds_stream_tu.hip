@@ -33,9 +33,9 @@ namespace FDS_FORM_NS {
 
 using T = FDS_T;
 constexpr bool K = FDS_KAHAN;
-constexpr int PF = kStreamShapes[FDS_SHAPE][4];   // rows prefetched ahead by stage 0
-
-template <int TX, bool AXI, int S, bool NT, bool ISO>
+// PF: rows of input prefetched ahead by stage 0 (the shape's own depth for its full stage
+// count; remainder launches of fewer stages use one row)
+template <int TX, bool AXI, int S, bool NT, bool ISO, int PF>
 __global__ __launch_bounds__(TX) void k_stream_acoustic(StepArgs<T> a) {
   __shared__ StreamLds<T, TX, S> L;
   const int tile = xcd_tile(blockIdx.x, gridDim.x, a.ntiles);
@@ -43,7 +43,7 @@ __global__ __launch_bounds__(TX) void k_stream_acoustic(StepArgs<T> a) {
   acoustic_stream<T, K, TX, AXI, S, NT, ISO, PF>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
 }
 
-template <int TX, bool AXI, int S, bool NT, bool ISO>
+template <int TX, bool AXI, int S, bool NT, bool ISO, int PF>
 __global__ __launch_bounds__(TX) void k_stream_thermal(StepArgs<T> a) {
   __shared__ StreamLds<T, TX, S> L;
   const int tile = xcd_tile(blockIdx.x, gridDim.x, a.ntiles);
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(TX) void k_stream_lossy(StepArgs<T> a) {
   lossy_stream<T, K, TX, AXI, NT, ISO>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
 }
 
-template <int TX, int S, bool NT>
+template <int TX, int S, bool NT, int PF>
 __global__ __launch_bounds__(TX) void k_stream_batch(const StepArgs<T> *__restrict__ descs,
                                                      const int32_t *__restrict__ tile_inst,
                                                      const int32_t *__restrict__ inst_base,
@@ -73,30 +73,29 @@ __global__ __launch_bounds__(TX) void k_stream_batch(const StepArgs<T> *__restri
 }
 
 // Persistent batch launch with a tile work queue: a resident grid of workgroups claims tiles
-// from a 64-bit counter until the launch's tiles are gone, in the order `order` (largest tiles
-// first).  Every workgroup makes one claim per tile it steps plus one final claim past the
-// end, so a launch advances the counter by exactly ntiles + gridDim.x: launch L of a run
-// (L = its first stage's step offset in the run / spl) owns the counter values from
-// L * (ntiles + gridDim.x) on, and no reset is needed between launches (the run's start
-// clears the counter).  Each workgroup exits when its claim passes the end.
-template <int TX, int S, bool NT>
+// from a counter until the launch's tiles are gone, in the order `order` (largest tiles
+// first).  Launch L of a run (L = its first stage's step offset in the run / spl) has its own
+// counter queue[L], zeroed at the run's start, so a claim c is tile order[c] for c < ntiles
+// and every workgroup exits at its first claim past the end.  L < nq is guaranteed by the
+// host (the counters cover the run); a launch outside them does no work.
+template <int TX, int S, bool NT, int PF>
 __global__ __launch_bounds__(TX) void k_stream_batch_wq(const StepArgs<T> *__restrict__ descs,
                                                         const int32_t *__restrict__ tile_inst,
                                                         const int32_t *__restrict__ inst_base,
                                                         const int32_t *__restrict__ order,
                                                         int ntiles, int k, int spl,
-                                                        unsigned long long *queue) {
+                                                        unsigned *queue, int nq) {
   __shared__ StreamLds<T, TX, S> L;
-  __shared__ long long claim;
-  const int64_t row0 = descs[0].dyn[1] + k;
-  const unsigned long long base =
-      (unsigned long long)(row0 / spl) * (unsigned long long)(ntiles + (int)gridDim.x);
+  __shared__ unsigned claim;
+  const int64_t launch = (descs[0].dyn[1] + k) / spl;
+  if (launch < 0 || launch >= nq) return;
+  unsigned *q = queue + launch;
   for (;;) {
-    if (threadIdx.x == 0) claim = (long long)(atomicAdd(queue, 1ull) - base);
+    if (threadIdx.x == 0) claim = atomicAdd(q, 1u);
     __syncthreads();
-    const long long t = claim;
+    const unsigned t = claim;
     __syncthreads();   // every thread has its claim (and left the previous tile's LDS rows)
-    if (t >= ntiles) break;
+    if (t >= (unsigned)ntiles) break;
     const int g = order[t];
     const int inst = tile_inst[g];
     const StepArgs<T> &a = descs[inst];
@@ -104,15 +103,15 @@ __global__ __launch_bounds__(TX) void k_stream_batch_wq(const StepArgs<T> *__res
   }
 }
 
-template <int TX, bool AXI, int S, bool NT, bool ISO>
+template <int TX, bool AXI, int S, bool NT, bool ISO, int PF>
 void launch_stream_acoustic(const void *args, int nblocks, size_t, hipStream_t s) {
-  hipLaunchKernelGGL((k_stream_acoustic<TX, AXI, S, NT, ISO>), dim3(nblocks), dim3(TX), 0, s,
+  hipLaunchKernelGGL((k_stream_acoustic<TX, AXI, S, NT, ISO, PF>), dim3(nblocks), dim3(TX), 0, s,
                      *static_cast<const StepArgs<T> *>(args));
 }
 
-template <int TX, bool AXI, int S, bool NT, bool ISO>
+template <int TX, bool AXI, int S, bool NT, bool ISO, int PF>
 void launch_stream_thermal(const void *args, int nblocks, size_t, hipStream_t s) {
-  hipLaunchKernelGGL((k_stream_thermal<TX, AXI, S, NT, ISO>), dim3(nblocks), dim3(TX), 0, s,
+  hipLaunchKernelGGL((k_stream_thermal<TX, AXI, S, NT, ISO, PF>), dim3(nblocks), dim3(TX), 0, s,
                      *static_cast<const StepArgs<T> *>(args));
 }
 
@@ -122,44 +121,48 @@ void launch_stream_lossy(const void *args, int nblocks, size_t, hipStream_t s) {
                      *static_cast<const StepArgs<T> *>(args));
 }
 
-template <int TX, int S, bool NT>
+template <int TX, int S, bool NT, int PF>
 void launch_stream_batch(const BatchLaunch &b, int nblocks, size_t, hipStream_t s) {
-  hipLaunchKernelGGL((k_stream_batch<TX, S, NT>), dim3(nblocks), dim3(TX), 0, s,
+  hipLaunchKernelGGL((k_stream_batch<TX, S, NT, PF>), dim3(nblocks), dim3(TX), 0, s,
                      static_cast<const StepArgs<T> *>(b.descs), b.tile_inst, b.inst_base,
                      b.ntiles, b.k);
 }
 
 // Resident grid: as many workgroups as the device holds at once, never more than the tiles.
-template <int TX, int S, bool NT>
+template <int TX, int S, bool NT, int PF>
 void launch_stream_batch_wq(const BatchLaunch &b, int, size_t, hipStream_t s) {
   static int per_cu = 0;
   if (per_cu == 0 &&
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stream_batch_wq<TX, S, NT>, TX, 0)
-          != hipSuccess)
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stream_batch_wq<TX, S, NT, PF>,
+                                                   TX, 0) != hipSuccess)
     per_cu = 1;
   const int grid = std::max(1, std::min(b.ntiles, std::max(1, per_cu) * b.num_cu));
-  hipLaunchKernelGGL((k_stream_batch_wq<TX, S, NT>), dim3(grid), dim3(TX), 0, s,
+  hipLaunchKernelGGL((k_stream_batch_wq<TX, S, NT, PF>), dim3(grid), dim3(TX), 0, s,
                      static_cast<const StepArgs<T> *>(b.descs), b.tile_inst, b.inst_base,
-                     b.order, b.ntiles, b.k, b.spl, b.queue);
+                     b.order, b.ntiles, b.k, b.spl, b.queue, b.nq);
 }
 
 // Variants [v]: 0 plain, 1 axial-symmetric, 2 isotropic coefficients (y arrays = x arrays, not
 // axial-symmetric).  [.][f]: f + 1 stages per launch, for f + 1 from 1 (remainder steps) to the
 // shape's own stage count; fewer stages than the tiling's ghost depth allows are exact too.
-template <int TX, int S, bool NT>
+// Remainder variants (fewer stages than the shape's) prefetch one row.  Batches (BATCH) run
+// their full launches on the shape's stage count and remainder steps one stage at a time; the
+// work-queue variant (WQ) exists for the full launches (remainder launches of a work-queue
+// batch run on the dispatcher kernel: they only end runs, and a run starts the queue afresh).
+template <int TX, int S, bool NT, int PF, bool BATCH, bool WQ>
 void fill_variants(KernelSet &ks, int I, int f) {
-  ks.stream_acoustic[I][0][f] = launch_stream_acoustic<TX, false, S, NT, false>;
-  ks.stream_acoustic[I][1][f] = launch_stream_acoustic<TX, true, S, NT, false>;
-  ks.stream_acoustic[I][2][f] = launch_stream_acoustic<TX, false, S, NT, true>;
-  ks.stream_thermal[I][0][f] = launch_stream_thermal<TX, false, S, NT, false>;
-  ks.stream_thermal[I][1][f] = launch_stream_thermal<TX, true, S, NT, false>;
-  ks.stream_thermal[I][2][f] = launch_stream_thermal<TX, false, S, NT, true>;
-  ks.stream_batch[I][f] = launch_stream_batch<TX, S, NT>;
-  ks.stream_batch_wq[I][f] = launch_stream_batch_wq<TX, S, NT>;
-  if (f == 0) {   // lossy acoustics: one step per launch on every shape
-    ks.stream_lossy[I][0] = launch_stream_lossy<TX, false, NT, false>;
-    ks.stream_lossy[I][1] = launch_stream_lossy<TX, true, NT, false>;
-    ks.stream_lossy[I][2] = launch_stream_lossy<TX, false, NT, true>;
+  ks.stream_acoustic[I][0][f] = launch_stream_acoustic<TX, false, S, NT, false, PF>;
+  ks.stream_acoustic[I][1][f] = launch_stream_acoustic<TX, true, S, NT, false, PF>;
+  ks.stream_acoustic[I][2][f] = launch_stream_acoustic<TX, false, S, NT, true, PF>;
+  ks.stream_thermal[I][0][f] = launch_stream_thermal<TX, false, S, NT, false, PF>;
+  ks.stream_thermal[I][1][f] = launch_stream_thermal<TX, true, S, NT, false, PF>;
+  ks.stream_thermal[I][2][f] = launch_stream_thermal<TX, false, S, NT, true, PF>;
+  if constexpr (BATCH) {
+    ks.stream_batch[I][f] = launch_stream_batch<TX, S, NT, PF>;
+    if constexpr (WQ)
+      ks.stream_batch_wq[I][f] = launch_stream_batch_wq<TX, S, NT, PF>;
+    else
+      ks.stream_batch_wq[I][f] = launch_stream_batch<TX, S, NT, PF>;
   }
 }
 
@@ -169,10 +172,20 @@ void FDS_CAT(fill_stream_, FDS_SHAPE)(KernelSet &ks) {
   constexpr int TX = kStreamShapes[I][0];
   constexpr int S = kStreamShapes[I][2];
   constexpr bool NT = kStreamShapes[I][3] != 0;
+  constexpr int PF = kStreamShapes[I][4];
   static_assert(S >= 1 && S <= kMaxStages, "stage count out of range");
-  fill_variants<TX, 1, NT>(ks, I, 0);
-  if constexpr (S >= 2) fill_variants<TX, 2, NT>(ks, I, 1);
-  if constexpr (S >= 3) fill_variants<TX, 3, NT>(ks, I, 2);
+  static_assert(PF >= 1, "prefetch depth out of range");
+  // shapes that only add prefetch depth serve single fields (no batch, no lossy kernels)
+  constexpr bool B = PF == 1;
+  fill_variants<TX, 1, NT, S == 1 ? PF : 1, B, B && S == 1>(ks, I, 0);
+  if constexpr (S >= 2) fill_variants<TX, 2, NT, S == 2 ? PF : 1, B && S == 2, B>(ks, I, 1);
+  if constexpr (S >= 3) fill_variants<TX, 3, NT, PF, B, B>(ks, I, 2);
+  // lossy acoustics: one step per launch on every shape (its own row ring)
+  if constexpr (PF == 1) {
+    ks.stream_lossy[I][0] = launch_stream_lossy<TX, false, NT, false>;
+    ks.stream_lossy[I][1] = launch_stream_lossy<TX, true, NT, false>;
+    ks.stream_lossy[I][2] = launch_stream_lossy<TX, false, NT, true>;
+  }
 }
 
 }  // namespace FDS_FORM_NS

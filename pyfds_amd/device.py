@@ -507,8 +507,10 @@ class DeviceRunner:
         self.run_steps(field, field.step, 1)
 
     # -- device -> host -------------------------------------------------------------------
-    def download(self, k):
-        out = np.empty(self.n)
+    def download(self, k, out=None):
+        """Component k's device state into ``out`` (a new array if None)."""
+        if out is None:
+            out = np.empty(self.n)
         nat.check(self.lib.fds_get_state(self.ctx, k, nat.dptr(out), self.n))
         return out
 

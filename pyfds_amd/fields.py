@@ -371,12 +371,19 @@ class FieldComponent:
 
     ``values`` is backed by the device after a run: reading it downloads (once) and marks the
     host copy as the one to upload before the next run, so in-place edits are honoured.
+
+    ``in_place``: the reference updates this component in place (``values -= ...``), so an
+    array obtained before a run holds the new state afterwards; the download then goes into
+    that same array (when it is a writeable float64 array of the right size).  Components the
+    reference assigns anew each step (thermal fluxes, the nonlinear models' pressure) get a
+    new array, as there.
     """
 
-    def __init__(self, num_points):
+    def __init__(self, num_points, in_place=True):
         self._values = np.zeros(num_points)
         self._device = None          # (runner, component index) holding newer values
         self._host_dirty = True      # host copy must be uploaded before the next run
+        self._in_place = in_place
         self.boundaries = []
         self.outputs = []
 
@@ -389,7 +396,12 @@ class FieldComponent:
         if self._device is not None:
             runner, k = self._device
             self._device = None
-            self._values = runner.download(k)
+            v = self._values
+            if self._in_place and isinstance(v, np.ndarray) and v.dtype == np.float64 \
+                    and v.shape == (runner.n,) and v.flags.c_contiguous and v.flags.writeable:
+                runner.download(k, out=v)
+            else:
+                self._values = runner.download(k)
         self._host_dirty = True
         return self._values
 

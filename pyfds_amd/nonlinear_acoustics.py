@@ -59,7 +59,7 @@ class IdealGas1D(_Nonlinear1D):
         super().__init__(*args, **kwargs)
         self.convective = convective
         self.nl_state = nl_state
-        self.pressure = fld.FieldComponent(self.num_points)
+        self.pressure = fld.FieldComponent(self.num_points, in_place=False)
         self.velocity = fld.FieldComponent(self.num_points)
         self.density = fld.FieldComponent(self.num_points)
         self.stat_density = None
@@ -95,7 +95,7 @@ class Acoustic2ndOrder1D(_Nonlinear1D):
             raise RuntimeError('Material with properties d_rho_p and d_rho2_p is required for '
                                '2nd order nonlinear acoustic simulation.')
         super().__init__(*args, **kwargs)
-        self.pressure = fld.FieldComponent(self.num_points)
+        self.pressure = fld.FieldComponent(self.num_points, in_place=False)
         self.velocity = fld.FieldComponent(self.num_points)
         self.density = fld.FieldComponent(self.num_points)
         self.stat_density = None

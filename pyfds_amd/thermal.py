@@ -26,7 +26,7 @@ class Thermal1D(_DeviceModel, fld.Field1D):
     def __init__(self, *args, **kwargs):
         super().__init__(*args, **kwargs)
         self.temperature = fld.FieldComponent(self.num_points)
-        self.heat_flux = fld.FieldComponent(self.num_points)
+        self.heat_flux = fld.FieldComponent(self.num_points, in_place=False)
 
     def _device_grid(self):
         return self.num_points, 1
@@ -56,8 +56,8 @@ class Thermal2D(_DeviceModel, fld.Field2D):
     def __init__(self, *args, **kwargs):
         super().__init__(*args, **kwargs)
         self.temperature = fld.FieldComponent(self.num_points)
-        self.heat_flux_x = fld.FieldComponent(self.num_points)
-        self.heat_flux_y = fld.FieldComponent(self.num_points)
+        self.heat_flux_x = fld.FieldComponent(self.num_points, in_place=False)
+        self.heat_flux_y = fld.FieldComponent(self.num_points, in_place=False)
 
     def _device_grid(self):
         return self.x.samples, self.y.samples

@@ -87,10 +87,9 @@ def test_fp64_tile_shapes_bit_identical(tile):
 # table: 0 (256 x 16, one step per launch), 1 (256 x 16, two fused steps), 2 (128 x 32, two),
 # 3 (256 x 32, three), 4-6 the same with non-temporal state traffic, 7-9 one-wave 64-wide
 # shapes with one, two and three steps, 10-11 64-wide non-temporal with one and two steps
-# (tests/test_host_api.py pins these traits), 12-15 prefetching two rows ahead: 64 x 32 three
-# steps, 64 x 16 two steps, 256 x 32 three steps non-temporal, 64 x 32 three steps
-# non-temporal.
-NSTREAM = 16
+# (tests/test_host_api.py pins these traits), 12-14 prefetching two rows ahead: 64 x 32 three
+# steps, 64 x 16 two steps, 256 x 32 three steps non-temporal.
+NSTREAM = 15
 
 
 def _stream_indices():
@@ -110,6 +109,10 @@ def test_fp64_stream_shapes_match_reference_golden(name, slot):
     scene = scenes.SCENES[name]()
     gold = np.load(os.path.join(GOLDEN, name + '.npz'))
     print('scene', name, 'shape', idx[slot], flush=True)
+    if 'lossy' in name and pyfds_amd._native.tile_table()[idx[slot]]['prefetch'] > 1:
+        with pytest.raises(ValueError):           # prefetch shapes serve lossless fields
+            run_device(scene, tile=idx[slot])
+        return
     res = run_device(scene, tile=idx[slot])
     for k in gold.files:
         if k != 'fingerprint':
@@ -169,7 +172,7 @@ def test_fp32_stream_engine_equals_tile_engine(name, form):
                                               ('Acoustic2D', 333, 101, 12),
                                               ('Acoustic3DAxi', 190, 77, 13),
                                               ('Acoustic2D', 509, 97, 14),
-                                              ('Acoustic3DAxi', 251, 100, 15)])
+                                              ('Acoustic3DAxi', 251, 100, 14)])
 def test_fp64_stream_multi_strip_grids_match_oracle_bitwise(model, nx, ny, slot):
     """Partial column blocks at the right edge and partial strips at the bottom."""
     idx = _stream_indices()[slot]
@@ -289,7 +292,7 @@ def _big_acoustic(nx, ny, steps, form='fp64', seed=1, model='Acoustic2D', mu=0.0
                                                     ('Acoustic2D', 333, 101, 12, 24),
                                                     ('Acoustic3DAxi', 200, 97, 12, 20),
                                                     ('Acoustic2D', 150, 71, 13, 12),
-                                                    ('Acoustic2D', 270, 83, 15, 24)])
+                                                    ('Acoustic2D', 270, 83, 14, 24)])
 def test_fp64_stream_strip_heights_match_oracle_bitwise(model, nx, ny, slot, strip):
     idx = _stream_indices()[slot]
     scene = _big_acoustic(nx, ny, 120, model=model)
@@ -348,7 +351,7 @@ def test_fp64_thermal_multi_tile_matches_oracle_bitwise(model, tile):
 
 
 @pytest.mark.parametrize('model', ['Acoustic2D', 'Acoustic3DAxi'])
-@pytest.mark.parametrize('slot', [3, 8, 9, 12, 13, 15])
+@pytest.mark.parametrize('slot', [3, 8, 9, 12, 13, 14])
 def test_fp64_fused_remainder_launches_match_oracle_bitwise(model, slot):
     # runs whose length is not a multiple of the fused stage count end in one launch of the
     # remaining stages (two-stage launches on three-step shapes)

@@ -48,8 +48,7 @@ def test_tile_table():
                       (256, 32, 3, False, 1), (256, 16, 1, True, 1), (256, 16, 2, True, 1),
                       (256, 32, 3, True, 1), (64, 16, 1, False, 1), (64, 16, 2, False, 1),
                       (64, 32, 3, False, 1), (64, 16, 1, True, 1), (64, 16, 2, True, 1),
-                      (64, 32, 3, False, 2), (64, 16, 2, False, 2), (256, 32, 3, True, 2),
-                      (64, 32, 3, True, 2)]
+                      (64, 32, 3, False, 2), (64, 16, 2, False, 2), (256, 32, 3, True, 2)]
     assert all(d['steps'] == 1 and not d['nontemporal'] and d['prefetch'] == 0
                for d in _native.tile_table() if d['engine'] == 'tile')
 
