@@ -288,6 +288,14 @@ int fds_lbc_band_rms(const float *frames, float *rms, int F, int D, int G, int L
  * (bx0, by0) of context b, num = sum (a - b)^2 and den = sum b^2 of the current states. */
 int fds_window_sq_diff(fds_ctx *a, int comp_a, fds_ctx *b, int comp_b, int wx, int wy, int ax0,
                        int ay0, int bx0, int by0, double *num, double *den);
+/* No counterpart in the reference: fds_window_sq_diff for callers that cannot allocate
+ * device memory (code under graph capture, serving loops that forbid hipMalloc).  Same
+ * arguments, same numbers bit for bit; it allocates nothing: the 2 x 512 float64 partial sums
+ * (one per workgroup, 8 KB, whatever the window size) live in a static device array that each
+ * GPU holds until the process ends.  Calls that overlap on one GPU share that array and race:
+ * serialise them. */
+int fds_window_sq_diff_static(fds_ctx *a, int comp_a, fds_ctx *b, int comp_b, int wx, int wy,
+                              int ax0, int ay0, int bx0, int by0, double *num, double *den);
 
 /* Device pointer of a component's recorded outputs ([rows][m] float64) of the last run, for
  * in-process consumers that keep them on the GPU. */

@@ -30,7 +30,8 @@ EXPORTS = ('fds_create', 'fds_destroy', 'fds_set_scalars', 'fds_set_coeff', 'fds
            'fds_set_learned_boundary',
            'fds_lbc_conv_fwd', 'fds_lbc_conv_bwd_data', 'fds_lbc_conv_bwd_weight',
            'fds_lbc_reduce_weight', 'fds_lbc_mse', 'fds_lbc_adam', 'fds_lbc_pack',
-           'fds_lbc_gather', 'fds_lbc_band_rms', 'fds_window_sq_diff', 'fds_output_device_ptr',
+           'fds_lbc_gather', 'fds_lbc_band_rms', 'fds_window_sq_diff', 'fds_window_sq_diff_static',
+           'fds_output_device_ptr',
            'fds_set_nonfinite_check', 'fds_last_error', 'fds_version')
 
 
@@ -122,6 +123,8 @@ def load():
         'fds_lbc_band_rms': (ct.c_int, [_p, _p] + [ct.c_int] * 4 + [_p]),
         'fds_window_sq_diff': (ct.c_int, [_p, ct.c_int, _p, ct.c_int] + [ct.c_int] * 6
                                + [_dp, _dp]),
+        'fds_window_sq_diff_static': (ct.c_int, [_p, ct.c_int, _p, ct.c_int] + [ct.c_int] * 6
+                                      + [_dp, _dp]),
         'fds_output_device_ptr': (ct.c_int, [_p, ct.c_int, ct.POINTER(_p), ct.POINTER(_i64),
                                              ct.POINTER(_i64)]),
         'fds_tile_count': (ct.c_int, []),

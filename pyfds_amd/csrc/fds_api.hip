@@ -9,6 +9,7 @@
 #include <limits>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2187,23 +2188,41 @@ const void *current_state(const fds_ctx *c, int comp) {
 
 }  // namespace
 
-extern "C" int fds_window_sq_diff(fds_ctx *a, int comp_a, fds_ctx *b, int comp_b, int wx, int wy,
-                                  int ax0, int ay0, int bx0, int by0, double *num, double *den) {
-  if (int rc = check_idle(a)) return rc;
-  if (!b || b->device != a->device) return fail(FDS_E_ARG, "contexts must share a device");
-  if (int rc = check_idle(b)) return rc;
-  if (int rc = check_comp(a, comp_a)) return rc;
-  if (int rc = check_comp(b, comp_b)) return rc;
-  if (wx < 1 || wy < 1 || ax0 < 0 || ay0 < 0 || bx0 < 0 || by0 < 0 || ax0 + wx > a->nx ||
-      ay0 + wy > a->ny || bx0 + wx > b->nx || by0 + wy > b->ny)
-    return fail(FDS_E_INDEX, "window outside a grid");
-  HIP_TRY(hipStreamSynchronize(b->stream));
-  const int blocks = 512;
-  DevBuf part;
-  HIP_TRY(part.ensure(blocks * 2 * sizeof(double)));
+namespace {
+
+constexpr int kWindowBlocks = 512;   // workgroups of the window reduction (partial sums each)
+
+// Partial sums of the no-allocation window reduction: one array per GPU, held by the process
+// until it ends.  Host code uses its device address from hipGetSymbolAddress (window_scratch),
+// never the array's name.
+static __device__ __align__(256) double scratch[2 * kWindowBlocks];
+
+// Device address of `scratch` on `device`, looked up once per device (with it current).
+int window_scratch(int device, double **out) {
+  static std::mutex mu;
+  static std::map<int, double *> ptrs;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = ptrs.find(device);
+  if (it == ptrs.end()) {
+    double *p = nullptr;
+    HIP_TRY(hipSetDevice(device));
+    const hipError_t e = hipGetSymbolAddress(reinterpret_cast<void **>(&p), HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(FDS_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    it = ptrs.emplace(device, p).first;
+  }
+  *out = it->second;
+  return 0;
+}
+
+// The window reduction of fds_window_sq_diff into `part` (2 * kWindowBlocks doubles, device),
+// summed on the host in block order.
+int window_sq_diff(fds_ctx *a, int comp_a, fds_ctx *b, int comp_b, int wx, int wy, int ax0,
+                   int ay0, int bx0, int by0, double *part, double *num, double *den) {
+  const int blocks = kWindowBlocks;
   const void *pa = current_state(a, comp_a), *pb = current_state(b, comp_b);
   const int anx = (int)a->nx, bnx = (int)b->nx;
-  double *o = part.as<double>();
+  double *o = part;
   if (a->elem == 8 && b->elem == 8)
     hipLaunchKernelGGL((k_window_sq_diff<double, double>), dim3(blocks), dim3(256), 0, a->stream,
                        (const double *)pa, anx, (const double *)pb, bnx, wx, wy, ax0, ay0, bx0, by0, o);
@@ -2217,8 +2236,8 @@ extern "C" int fds_window_sq_diff(fds_ctx *a, int comp_a, fds_ctx *b, int comp_b
     hipLaunchKernelGGL((k_window_sq_diff<double, float>), dim3(blocks), dim3(256), 0, a->stream,
                        (const double *)pa, anx, (const float *)pb, bnx, wx, wy, ax0, ay0, bx0, by0, o);
   HIP_TRY(hipGetLastError());
-  std::vector<double> h(blocks * 2);
-  HIP_TRY(hipMemcpyAsync(h.data(), o, h.size() * sizeof(double), hipMemcpyDeviceToHost, a->stream));
+  double h[2 * kWindowBlocks];
+  HIP_TRY(hipMemcpyAsync(h, o, sizeof h, hipMemcpyDeviceToHost, a->stream));
   HIP_TRY(hipStreamSynchronize(a->stream));
   double s0 = 0, s1 = 0;
   for (int i = 0; i < blocks; ++i) {
@@ -2228,6 +2247,43 @@ extern "C" int fds_window_sq_diff(fds_ctx *a, int comp_a, fds_ctx *b, int comp_b
   *num = s0;
   *den = s1;
   return 0;
+}
+
+int window_checks(fds_ctx *a, int comp_a, fds_ctx *b, int comp_b, int wx, int wy, int ax0,
+                  int ay0, int bx0, int by0, double *num, double *den) {
+  if (int rc = check_idle(a)) return rc;
+  if (!b || b->device != a->device) return fail(FDS_E_ARG, "contexts must share a device");
+  if (int rc = check_idle(b)) return rc;
+  if (int rc = check_comp(a, comp_a)) return rc;
+  if (int rc = check_comp(b, comp_b)) return rc;
+  if (!num || !den) return fail(FDS_E_ARG, "null output");
+  if (wx < 1 || wy < 1 || ax0 < 0 || ay0 < 0 || bx0 < 0 || by0 < 0 || ax0 + wx > a->nx ||
+      ay0 + wy > a->ny || bx0 + wx > b->nx || by0 + wy > b->ny)
+    return fail(FDS_E_INDEX, "window outside a grid");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int fds_window_sq_diff(fds_ctx *a, int comp_a, fds_ctx *b, int comp_b, int wx, int wy,
+                                  int ax0, int ay0, int bx0, int by0, double *num, double *den) {
+  if (int rc = window_checks(a, comp_a, b, comp_b, wx, wy, ax0, ay0, bx0, by0, num, den))
+    return rc;
+  DevBuf part;
+  HIP_TRY(part.ensure(2 * kWindowBlocks * sizeof(double)));
+  return window_sq_diff(a, comp_a, b, comp_b, wx, wy, ax0, ay0, bx0, by0, part.as<double>(),
+                        num, den);
+}
+
+extern "C" int fds_window_sq_diff_static(fds_ctx *a, int comp_a, fds_ctx *b, int comp_b, int wx,
+                                         int wy, int ax0, int ay0, int bx0, int by0, double *num,
+                                         double *den) {
+  if (int rc = window_checks(a, comp_a, b, comp_b, wx, wy, ax0, ay0, bx0, by0, num, den))
+    return rc;
+  double *part = nullptr;
+  if (int rc = window_scratch(a->device, &part)) return rc;
+  return window_sq_diff(a, comp_a, b, comp_b, wx, wy, ax0, ay0, bx0, by0, part, num, den);
 }
 
 extern "C" int fds_output_device_ptr(fds_ctx *c, int comp, double **ptr, int64_t *rows, int64_t *m) {
