@@ -250,6 +250,59 @@ int fds_couple_loss_run(fds_ctx *acoustic, fds_ctx *thermal, const double *rho,
                         const double *loss, double dt, int64_t stepping, int accumulate,
                         double *accum, int64_t start_step, int64_t nsteps);
 
+/* Coupled fields on the device: SynchronizedFields (pyfds/coupling.py:14-90) with the shipped
+ * interactions, each step every field's step kernel, then the interactions in list order at
+ * that step; nothing returns to the host until the run ends.  The reference applies them on
+ * the host after every pair of sim_steps (pyfds/coupling.py:81-87). */
+enum {
+  FDS_COUPLE_MATERIAL_EXP = 1,     /* MaterialCouplingExponential  pyfds/coupling.py:229-270 */
+  FDS_COUPLE_MATERIAL_POWER = 2,   /* MaterialCouplingPowerLaw     pyfds/coupling.py:273-300 */
+  FDS_COUPLE_BOUNDARY_LINEAR = 3   /* BoundaryCoupling, transfer = factor * q
+                                    *                              pyfds/coupling.py:93-146  */
+};
+/* material vectors a coupled re-assembly reads (fds_coupling_set_material) */
+enum {
+  FDS_P_SOUND_VELOCITY = 0, FDS_P_DENSITY = 1, FDS_P_ABSORPTION = 2,  /* acoustics */
+  FDS_P_HEAT_CAPACITY = 3, FDS_P_CONDUCTIVITY_X = 4, FDS_P_CONDUCTIVITY_Y = 5  /* thermal */
+};
+typedef struct {
+  int kind;                        /* FDS_COUPLE_*                                            */
+  int source_field, source_comp;   /* field index in the coupling, component (model order)    */
+  int target_field, target_comp;   /* boundary: target component; material: unused           */
+  int target_param;                /* material: FDS_P_* coupled parameter                     */
+  double p0, p1;                   /* exp: a, b; power law: power, factor; linear: factor, -  */
+  int64_t stepping;                /* apply every stepping-th step (step % stepping == 0)     */
+  double threshold;                /* material: relative change to re-assemble (NaN: none)    */
+  int additive, accumulate;        /* boundary                                                */
+} fds_interaction;
+typedef struct fds_coupling fds_coupling;
+/* Fields: committed contexts of one device, form (FP64 or FP32) and cell count; at most one
+ * material coupling per target field (as in the reference, whose wrapped material_vector
+ * supports one).  Material couplings re-assemble acoustic fields (sound velocity, density,
+ * absorption) or thermal fields (density, heat capacity, conductivity x / y). */
+int fds_coupling_create(fds_coupling **out, fds_ctx *const *fields, int nfields,
+                        const fds_interaction *inter, int ninter);
+/* Static material vector `param` of field `field` (float64, n = cells): the target field's
+ * material_vector without the coupling (pyfds/coupling.py:184-197).  A material coupling's
+ * target needs all the vectors its model's assembly reads. */
+int fds_coupling_set_material(fds_coupling *cp, int field, int param, const double *values,
+                              int64_t n);
+/* Expression scalars of a target field, in the reference's order: acoustics dt/dx, dt/dy,
+ * dt/dx**2, dt/dy**2, dt; thermal dt/dx, dt/dy, 1/dx, 1/dy. */
+int fds_coupling_set_scalars(fds_coupling *cp, int field, const double *s, int n);
+/* Interaction state (n = cells float64): a material coupling's last used factors, a boundary
+ * coupling's accumulated transfer; set != 0 uploads, 0 downloads.  Both start at zero. */
+int fds_coupling_state(fds_coupling *cp, int interaction, int set, double *values, int64_t n);
+/* Advance every field nsteps steps from start_step (each field's outputs and state read back
+ * as after fds_run).  Re-assemblies of material couplings with a threshold are logged as
+ * (step, largest relative change) pairs into log[2 * log_cap]; *nlog = their number. */
+int fds_coupling_run(fds_coupling *cp, int64_t start_step, int64_t nsteps, double *log,
+                     int64_t log_cap, int64_t *nlog);
+int fds_coupling_destroy(fds_coupling *cp);
+/* Copy coefficient slot FDS_C_* of a context to the host (no reference counterpart: the
+ * host's assembled factors after a device re-assembly). */
+int fds_get_coeff(fds_ctx *ctx, int slot, double *data, int64_t n);
+
 /* Learned absorbing boundary (no reference counterpart; pyfds has only the plain method).
  * After every step of an FP32 Acoustic2D context on a square G x G grid, a BF16-MFMA network
  * predicts the outer `layer` cells (16) of p, vx, vy on all four sides from the 8 adjacent

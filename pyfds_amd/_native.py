@@ -32,7 +32,9 @@ EXPORTS = ('fds_create', 'fds_destroy', 'fds_set_scalars', 'fds_set_coeff', 'fds
            'fds_lbc_reduce_weight', 'fds_lbc_mse', 'fds_lbc_adam', 'fds_lbc_pack',
            'fds_lbc_gather', 'fds_lbc_band_rms', 'fds_window_sq_diff', 'fds_window_sq_diff_static',
            'fds_output_device_ptr',
-           'fds_set_nonfinite_check', 'fds_last_error', 'fds_version')
+           'fds_set_nonfinite_check', 'fds_coupling_create', 'fds_coupling_set_material',
+           'fds_coupling_set_scalars', 'fds_coupling_state', 'fds_coupling_run',
+           'fds_coupling_destroy', 'fds_get_coeff', 'fds_last_error', 'fds_version')
 
 
 class NativeError(RuntimeError):
@@ -140,6 +142,14 @@ def load():
                                            _dp, ct.c_int64, ct.c_int64]),
         'fds_set_flow': (ct.c_int, [_p, _lp, ct.c_int64]),
         'fds_set_nonfinite_check': (ct.c_int, [_p, ct.c_int]),
+        'fds_coupling_create': (ct.c_int, [ct.POINTER(_p), ct.POINTER(_p), ct.c_int, _p,
+                                           ct.c_int]),
+        'fds_coupling_set_material': (ct.c_int, [_p, ct.c_int, ct.c_int, _dp, _i64]),
+        'fds_coupling_set_scalars': (ct.c_int, [_p, ct.c_int, _dp, ct.c_int]),
+        'fds_coupling_state': (ct.c_int, [_p, ct.c_int, ct.c_int, _dp, _i64]),
+        'fds_coupling_run': (ct.c_int, [_p, _i64, _i64, _dp, _i64, ct.POINTER(_i64)]),
+        'fds_coupling_destroy': (ct.c_int, [_p]),
+        'fds_get_coeff': (ct.c_int, [_p, ct.c_int, _dp, _i64]),
         'fds_tile_shape': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         'fds_last_error': (ct.c_char_p, []),
         'fds_version': (ct.c_char_p, []),

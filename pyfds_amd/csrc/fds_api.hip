@@ -50,6 +50,20 @@ struct DevBuf {
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   ~DevBuf() { release(); }
   void release() {
     if (p) (void)hipFree(p);
@@ -1792,8 +1806,11 @@ int batch_launch(fds_batch *b, int rel_parity, int k, bool store, int stages = 1
   L.nq = b->nq;
   L.spl = b->fuse;
   L.num_cu = b->ctxs[0]->num_cu;
-  if (b->streaming && b->schedule == 1)
-    b->ks->stream_batch_wq[b->shape - fds::kNumTiles][stages - 1](L, 0, 0, b->stream);
+  // the work queue serves full launches; remainder launches (fewer stages) and other shapes
+  // run on the dispatcher kernels
+  if (b->streaming && b->schedule == 1 && stages == b->fuse)
+    b->ks->stream_batch_wq[b->shape - fds::kNumTiles][stages - 1](L, fds::xcd_grid(b->ntiles),
+                                                                    0, b->stream);
   else if (b->streaming)
     b->ks->stream_batch[b->shape - fds::kNumTiles][stages - 1](L, fds::xcd_grid(b->ntiles), 0,
                                                                  b->stream);
@@ -2294,3 +2311,5 @@ extern "C" int fds_output_device_ptr(fds_ctx *c, int comp, double **ptr, int64_t
   *m = (int64_t)c->comp[comp].out_cells.size();
   return 0;
 }
+
+#include "fds_couple.inc"

@@ -1,4 +1,5 @@
-// Correctly rounded float64 power for the ideal-gas equation of state.
+// Correctly rounded float64 power (the ideal-gas equation of state) and exponential (the
+// exponential material coupling).
 //
 // pyfds evaluates p = rho0 c^2/gamma * (((rho0 + rho')/rho0)**gamma - 1) with numpy's power
 // (pyfds/nonlinear_acoustics.py:82-87).  The subtraction turns a one-ulp difference in the
@@ -158,6 +159,12 @@ FDS_CRM_FN double exp_dd(dd z) {
   const int n = (k - j) / kExpTable;
   const dd r = mul(dd{kExpHi[j], kExpLo[j]}, q);  // r.hi = r.hi + r.lo rounded to nearest
   return ldexp(r.hi, n);
+}
+
+// e**z rounded to nearest (the same exp; infinities and NaN follow C99 Annex F)
+FDS_CRM_FN double exp_cr(double z) {
+  if (z != z) return z;
+  return exp_dd(dd{z, 0.0});
 }
 
 // x**y rounded to nearest (see the header comment for the method and its bound)

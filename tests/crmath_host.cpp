@@ -5,3 +5,7 @@
 extern "C" void crm_pow(const double *x, const double *y, double *out, long n) {
   for (long i = 0; i < n; ++i) out[i] = fds::crm::pow_cr(x[i], y[i]);
 }
+
+extern "C" void crm_exp(const double *z, double *out, long n) {
+  for (long i = 0; i < n; ++i) out[i] = fds::crm::exp_cr(z[i]);
+}

@@ -14,6 +14,9 @@ import pyfds_amd as fds
 import coupled_scenes as cs
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+# scenes whose device loop evaluates exp: the device's exp is correctly rounded, numpy's (which
+# wrote the fixture) is within an ulp or two of it, so those scenes hold to 1e-12, not bitwise
+EXP_SCENES = ('mc1d',)
 
 
 def _thermal1d(**kw):
@@ -140,7 +143,23 @@ def test_coupled_runs_match_reference_golden(name):
     keys = [k for k in gold.files if k != 'fingerprint']
     assert sorted(keys) == sorted(res)
     for k in keys:
-        np.testing.assert_array_equal(res[k], gold[k], err_msg='%s %s' % (name, k))
+        if name in EXP_SCENES:
+            np.testing.assert_allclose(res[k], gold[k], rtol=0, atol=1e-12 * np.abs(
+                gold[k]).max(), err_msg='%s %s' % (name, k))
+        else:
+            np.testing.assert_array_equal(res[k], gold[k], err_msg='%s %s' % (name, k))
+
+
+@pytest.mark.gpu
+def test_material_coupling_host_protocol_matches_reference_golden_bitwise(monkeypatch):
+    """The same scene through the reference's step-by-step host protocol (numpy's exp on the
+    host, as the fixture was written) is bit for bit the fixture."""
+    monkeypatch.setattr(fds.coupling, '_device_plan', lambda sync: None)
+    gold = np.load(os.path.join(GOLDEN, 'coupled_mc1d.npz'))
+    fld, plan, comps, params = cs.SCENES['mc1d'](fds)
+    res = cs.collect(cs.run(fld, plan), comps)
+    for k in res:
+        np.testing.assert_array_equal(res[k], gold[k], err_msg=k)
 
 
 @pytest.mark.gpu
@@ -259,3 +278,126 @@ def test_device_coupled_fp32_forms(form):
         err = np.linalg.norm(res[k] - ref) / max(np.linalg.norm(ref), 1e-300)
         print('  %-24s rel L2 %.3e' % (k, err))
         assert err <= 1e-4, (form, k, err)
+
+
+# ------------------------------------------------------------------ device material coupling
+def _mc_scene(ns, n=300, steps=400, threshold=1e-3, power=2.0, boundary=True):
+    """Acoustic1D + Thermal1D: power-law sound velocity (every 4th step), power-law density
+    with a relative-change threshold, and a linear boundary transfer from the acoustic
+    pressure into the temperature, accumulated over 3 steps.  No exp: device and host protocol
+    evaluate the same operations."""
+    dx = 1e-4
+    dt = 0.4 * dx / 1600.0
+    ac = ns.Acoustic1D(x_samples=n, x_delta=dx, t_samples=steps, t_delta=dt,
+                       material=ns.AcousticMaterial(1500.0, 1000.0, absorption_coef=1e-3))
+    th = ns.Thermal1D(x_samples=n, x_delta=dx, t_samples=steps, t_delta=dt,
+                      material=ns.ThermalMaterial(4000.0, 1000.0, 0.6))
+    i = np.arange(n)
+    ac.pressure.values = np.exp(-0.5 * ((i - n / 3) / 4.0) ** 2)
+    th.temperature.values = 20.0 + 50.0 * np.exp(-0.5 * ((i - n / 2) / 20.0) ** 2)
+    t = np.arange(steps) * dt
+    ac.pressure.add_boundary(ns.regions.PointRegion([n // 4], None, 'src'),
+                             value=0.1 * np.sin(2 * np.pi * 1e6 * t), additive=True)
+    # the sound velocity grows by at most 30 % (the step stays stable for any power)
+    inter = [ns.MaterialCouplingPowerLaw(th.temperature, ac, 'sound_velocity', power=power,
+                                         factor=0.3 / 70.0 ** power, stepping=4),
+             ns.MaterialCouplingPowerLaw(th.temperature, th, 'density', power=2, factor=1e-4,
+                                         rel_change_threshold=threshold)]
+    if boundary:
+        inter.append(ns.BoundaryCoupling(ac.pressure, th.temperature,
+                                         ns.LinearTransfer(1e-3), additive=True,
+                                         accumulate=True, stepping=3))
+    ac.velocity.add_output(ns.regions.Region([n // 2, n // 2 + 7], 'v'))
+    th.temperature.add_output(ns.regions.Region([n // 2], 'T'))
+    return ns.SynchronizedFields([ac, th], inter)
+
+
+def test_device_plan_covers_the_shipped_transfer_functions():
+    sync = _mc_scene(fds)
+    plan = fds.coupling._device_plan(sync)
+    assert [d['kind'] for d in plan] == [2, 2, 3]
+    assert [d['target'] for d in plan] == [0, 1, 1]
+    # a user transfer function keeps the host protocol
+    sync = _mc_scene(fds, boundary=False)
+    sync.interactions.append(fds.BoundaryCoupling(sync.fields[0].pressure,
+                                                  sync.fields[1].temperature,
+                                                  lambda v: 2 * v))
+    assert fds.coupling._device_plan(sync) is None
+    # the Kahan form keeps the host protocol
+    sync = _mc_scene(fds)
+    for f in sync.fields:
+        f.form = 'fp32_kahan'
+    assert fds.coupling._device_plan(sync) is None
+    # two material couplings of one target keep the host protocol
+    sync = _mc_scene(fds, boundary=False)
+    sync.interactions.append(fds.MaterialCouplingExponential(
+        sync.fields[1].temperature, sync.fields[0], 'density', a=0.5, b=0.1))
+    assert fds.coupling._device_plan(sync) is None
+
+
+def test_linear_transfer_is_a_scalar_multiple():
+    v = np.array([1.0, -2.0, 3.5])
+    np.testing.assert_array_equal(fds.LinearTransfer(0.25)(v), 0.25 * v)
+
+
+def _run_mc(monkeypatch, host, plan, caplog, **kw):
+    import logging
+    with monkeypatch.context() as m:
+        if host:
+            m.setattr(fds.coupling, '_device_plan', lambda sync: None)
+        sync = _mc_scene(fds, **kw)
+        caplog.clear()
+        with caplog.at_level(logging.INFO, logger='pyfds'):
+            for n in plan:
+                sync.simulate(n)
+        logs = [r.getMessage() for r in caplog.records
+                if r.getMessage().startswith(('Relative', 'Matrices reassembled'))]
+    ac, th = sync.fields
+    res = {'step': np.array(sync.step)}
+    for name, c in [('p', ac.pressure), ('v', ac.velocity), ('T', th.temperature),
+                    ('q', th.heat_flux)]:
+        res[name] = np.array(c.values)
+        for k, o in enumerate(c.outputs):
+            res['%s_out%d' % (name, k)] = np.array(o.signals)
+    for j, it in enumerate(sync.interactions):
+        st = getattr(it, 'last_used_factors', None)
+        if st is None:
+            st = it.accumulated_transfer
+        res['state%d' % j] = np.broadcast_to(np.asarray(st, float), (ac.num_points,)).copy()
+    # the re-assembled coefficients, as the host's matrices see them
+    res['a_p_v'] = ac.a_p_v.toarray()
+    res['a_T_q'] = np.asarray(th._mats['g'], float)
+    return res, logs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('plan', [(400,), (101, 57, 3, 239)])
+def test_device_material_coupling_equals_host_protocol_bitwise(monkeypatch, caplog, plan):
+    """The device loop against the reference's host protocol, over runs that split the
+    coupling periods (101, 57, 3 steps), with the reassembly log lines of the threshold."""
+    dev, dev_log = _run_mc(monkeypatch, False, plan, caplog)
+    host, host_log = _run_mc(monkeypatch, True, plan, caplog)
+    print('reassemblies logged: device %d host %d' % (len(dev_log), len(host_log)))
+    assert sorted(dev) == sorted(host)
+    for k in host:
+        assert np.all(np.isfinite(host[k])), k
+        np.testing.assert_array_equal(dev[k], host[k], err_msg=k)
+    assert dev_log == host_log and len(host_log) > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('power', [0.5, -1.0, 1.0, 3.0])
+def test_device_power_law_paths(monkeypatch, caplog, power):
+    """numpy's ``values ** power`` takes fast paths for 0.5, -1, 1 (and 2, above); other
+    exponents run the correctly rounded power on the device and numpy's on the host (within
+    an ulp), so those hold to 1e-12."""
+    dev, _ = _run_mc(monkeypatch, False, (150,), caplog, power=power, threshold=None)
+    host, _ = _run_mc(monkeypatch, True, (150,), caplog, power=power, threshold=None)
+    for k in host:
+        assert np.all(np.isfinite(host[k])), k
+        if power == 3.0:
+            np.testing.assert_allclose(dev[k], host[k], rtol=0,
+                                       atol=1e-12 * max(np.abs(host[k]).max(), 1e-300),
+                                       err_msg=k)
+        else:
+            np.testing.assert_array_equal(dev[k], host[k], err_msg=k)

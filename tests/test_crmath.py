@@ -82,6 +82,13 @@ def crm(tmp_path_factory):
         so.crm_pow(x.ctypes.data_as(dp), y.ctypes.data_as(dp), out.ctypes.data_as(dp),
                    ctypes.c_long(x.size))
         return out
+
+    def ex(z):
+        z = np.ascontiguousarray(z, np.float64)
+        out = np.empty_like(z)
+        so.crm_exp(z.ctypes.data_as(dp), out.ctypes.data_as(dp), ctypes.c_long(z.size))
+        return out
+    pw.exp = ex
     return pw
 
 
@@ -132,3 +139,19 @@ def test_reference_power_paths_and_cr_oracle():
     spread = max(np.linalg.norm(svml[k] - libm[k]) / np.linalg.norm(libm[k])
                  for k in cr if k != 'step')
     assert 1e-12 < spread < 5e-12, spread
+
+
+def test_product_exp_is_correctly_rounded(crm):
+    # the exponential material coupling's exp (b * q with b ~ -0.01 and q a temperature) and a
+    # wide range; the oracle's pow_cr(e, z) is e**z correctly rounded only up to the rounding
+    # of e itself, so decimal is the reference here
+    rng = np.random.default_rng(15)
+    z = np.concatenate([rng.uniform(-1.0, 1.0, 3000), rng.uniform(-700.0, 700.0, 3000),
+                        -0.01 * rng.uniform(0.0, 60.0, 3000)])
+    with localcontext() as ctx:
+        ctx.prec = 80
+        ref = np.array([float(Decimal(float(v)).exp()) for v in z])
+    np.testing.assert_array_equal(crm.exp(z), ref)
+    special = np.array([0.0, -0.0, float('inf'), float('-inf'), 710.0, -746.0])
+    np.testing.assert_array_equal(crm.exp(special), np.exp(special))
+    assert np.isnan(crm.exp(np.array([float('nan')])))[0]

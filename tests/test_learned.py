@@ -245,11 +245,13 @@ def test_fused_inference_writes_oracle_layer():
     vx0 = (rng.standard_normal(g * g) * 1e-6).astype(np.float32).astype(np.float64)
     vy0 = (rng.standard_normal(g * g) * 1e-6).astype(np.float32).astype(np.float64)
     fld.pressure.values, fld.velocity_x.values, fld.velocity_y.values = p0, vx0, vy0
+    p0_in, vx0_in, vy0_in = p0.copy(), vx0.copy(), vy0.copy()
     lb.attach(fld)
     fld.simulate(1)
     cur = [np.asarray(c.values, np.float64) for c in (fld.pressure, fld.velocity_x,
                                                       fld.velocity_y)]
-    prev = [p0, vx0, vy0]
+    # copies: the run updates the held arrays in place (as the reference's values -= ...)
+    prev = [p0_in, vx0_in, vy0_in]
     for s in range(4):
         x = np.zeros((1, g, CIN))
         for a in range(LAYER, g - LAYER):
