@@ -148,7 +148,8 @@ def tuned(fld, fixed=None):
                 shape='%s %dx%d, %d step(s) per launch%s' % (
                     info['engine'], info['tx'], getattr(fld, 'strip_height', 0) or info['ty'],
                     spl, ', non-temporal' if info['nontemporal'] else '')
-                + (', prefetch %d rows' % info['prefetch'] if info['prefetch'] > 1 else ''),
+                + (', prefetch %d rows' % info['prefetch'] if info['prefetch'] > 1 else '')
+                + (' via LDS rings' if info['lds_ring'] else ''),
                 steps_per_launch=spl, table=table)
 
 

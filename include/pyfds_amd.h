@@ -171,9 +171,11 @@ int fds_set_tile(fds_ctx *ctx, int tile_index);
 int fds_tile_count(void);
 int fds_tile_shape(int tile_index, int *tx, int *ty);
 int fds_tile_engine(int tile_index);
-/* Time steps per launch, cache policy (1: non-temporal state traffic) and prefetch depth (input
- * rows a streaming workgroup loads ahead; 0 for LDS-tile shapes) of a shape. */
-int fds_tile_traits(int tile_index, int *steps, int *nontemporal, int *prefetch);
+/* Time steps per launch, cache policy (1: non-temporal state traffic), prefetch depth (input
+ * rows a streaming workgroup loads ahead; 0 for LDS-tile shapes) and load path (lds_ring 1:
+ * the rows go straight from HBM to LDS rings, no registers in flight; such shapes serve single
+ * lossless fields with nx a multiple of 16 bytes / element size) of a shape. */
+int fds_tile_traits(int tile_index, int *steps, int *nontemporal, int *prefetch, int *lds_ring);
 /* Strip height (rows per workgroup) of a streaming shape; 0 restores the shape's own height.
  * Ignored by LDS-tile shapes.  For the resident 1D engine the value is the number of cells per
  * thread (1, 2, 4 or 8; 0 = chosen from the line length). */

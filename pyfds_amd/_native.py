@@ -132,7 +132,7 @@ def load():
         'fds_tile_count': (ct.c_int, []),
         'fds_tile_engine': (ct.c_int, [ct.c_int]),
         'fds_tile_traits': (ct.c_int, [ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int),
-                                       ct.POINTER(ct.c_int)]),
+                                       ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         'fds_set_strip_height': (ct.c_int, [_p, ct.c_int]),
         'fds_engine': (ct.c_int, [_p]),
         'fds_steps_per_launch': (ct.c_int, [_p]),
@@ -213,14 +213,16 @@ def tile_shapes():
 
 def tile_table():
     """Every shape of the step kernels: dicts with index, engine, tx, ty, steps (per launch),
-    nontemporal and prefetch (rows loaded ahead by a streaming workgroup)."""
+    nontemporal, prefetch (rows loaded ahead by a streaming workgroup) and lds_ring (those
+    rows go straight to LDS); ``single_only``: the shape serves single lossless fields."""
     lib = load()
     out = []
     for i, ((tx, ty), eng) in enumerate(zip(tile_shapes(), tile_engines())):
-        st, nt, pf = ct.c_int(), ct.c_int(), ct.c_int()
-        check(lib.fds_tile_traits(i, ct.byref(st), ct.byref(nt), ct.byref(pf)))
+        st, nt, pf, rg = ct.c_int(), ct.c_int(), ct.c_int(), ct.c_int()
+        check(lib.fds_tile_traits(i, ct.byref(st), ct.byref(nt), ct.byref(pf), ct.byref(rg)))
         out.append(dict(index=i, engine=eng, tx=tx, ty=ty, steps=st.value,
-                        nontemporal=bool(nt.value), prefetch=pf.value))
+                        nontemporal=bool(nt.value), prefetch=pf.value, lds_ring=bool(rg.value),
+                        single_only=pf.value > 1 or bool(rg.value)))
     return out
 
 

@@ -104,7 +104,7 @@ def _tune(fields, steps, chunks, use_cache, batch):
             f.assemble_matrices()
     snap = _snapshot(fields)
     shapes, engines = nat.tile_shapes(), nat.tile_engines()
-    prefetch = [d['prefetch'] for d in nat.tile_table()]
+    single_only = [d['single_only'] for d in nat.tile_table()]
     one_row = fields[0]._device_grid()[1] == 1
     lossy = any(getattr(x, '_coefs', {}).get(4) is not None for x in fields)
     cand = []
@@ -113,9 +113,9 @@ def _tune(fields, steps, chunks, use_cache, batch):
             continue
         if engines[i] == 'stream':
             # 1D models run on the resident engine; batches stream lossless members only
-            # (a lossy field runs any streaming shape one step per launch); the shapes that
-            # only add prefetch depth serve single lossless fields
-            if one_row or (prefetch[i] > 1 and (lossy or batch)) or (lossy and batch):
+            # (a lossy field runs any streaming shape one step per launch); the shapes with
+            # deeper prefetch or the LDS ring serve single lossless fields
+            if one_row or (single_only[i] and (lossy or batch)) or (lossy and batch):
                 continue
             cand.append((i, 0))
             cand += [(i, h) for h in STRIP_HEIGHTS.get(ty, ())]

@@ -450,12 +450,18 @@ int choose_tiles(fds_ctx *c) {
                                                              : fds::kDefaultLossless2D);
     if (t >= fds::kNumTiles && !can_stream(c))
       t = (c->ny == 1) ? fds::kTileRow : fds::kDefaultTile2D;
-    // lossy acoustics streams one step per launch with its own row ring: shapes that only add
-    // prefetch depth have no lossy kernels
+    // lossy acoustics streams one step per launch with its own row ring: shapes with deeper
+    // prefetch or the LDS ring have no lossy kernels
     if (t >= fds::kNumTiles && c->lossy && c->family == kAcoustic &&
-        fds::kStreamShapes[t - fds::kNumTiles][4] > 1)
-      return fail(FDS_E_ARG, "shape %d (prefetch %d rows) serves lossless fields only", t,
-                  fds::kStreamShapes[t - fds::kNumTiles][4]);
+        fds::stream_single_only(t - fds::kNumTiles))
+      return fail(FDS_E_ARG, "shape %d (prefetch %d rows%s) serves lossless fields only", t,
+                  fds::kStreamShapes[t - fds::kNumTiles][4],
+                  fds::kStreamShapes[t - fds::kNumTiles][5] ? ", LDS ring" : "");
+    // the LDS ring loads 16-byte chunks of rows: rows must start on 16-byte boundaries
+    if (t >= fds::kNumTiles && fds::kStreamShapes[t - fds::kNumTiles][5] &&
+        c->nx % (16 / (int)c->elem) != 0)
+      return fail(FDS_E_ARG, "shape %d (LDS ring) needs nx a multiple of %d (nx = %lld)", t,
+                  16 / (int)c->elem, (long long)c->nx);
     c->shape = t;
     c->streaming = t >= fds::kNumTiles;
     shape_of(t, &c->tx, &c->ty);
@@ -780,6 +786,16 @@ int launch_step(fds_ctx *c, int parity, int k, bool store, int stages) {
   fill_args<T>(c, a, parity);
   a.k = k;
   if (store) a.flags |= fds::kFlagStore;
+  if (c->streaming && !c->lossy && stages == c->fuse &&
+      fds::kStreamShapes[c->shape - fds::kNumTiles][5]) {
+    // LDS-ring kernels load 16-byte chunks: every array they read must be 16-byte aligned
+    // (whole device allocations are)
+    bool ok = true;
+    for (int k = 0; k < 3; ++k)
+      ok = ok && ((uintptr_t)a.in[k] & 15) == 0 && ((uintptr_t)a.cin[k] & 15) == 0;
+    for (int q = 0; q < FDS_NUM_COEFF; ++q) ok = ok && ((uintptr_t)a.coef[q] & 15) == 0;
+    if (!ok) return fail(FDS_E_ARG, "shape %d (LDS ring) needs 16-byte aligned arrays", c->shape);
+  }
   const int nb = fds::xcd_grid(c->ntiles);
   pick_launch(c, stages)(&a, nb, lds_bytes(c), c->stream);
   HIP_TRY(hipGetLastError());
@@ -960,14 +976,15 @@ int fds_tile_engine(int i) {
   return i >= fds::kNumTiles ? 1 : 0;
 }
 
-int fds_tile_traits(int i, int *steps, int *nontemporal, int *prefetch) {
+int fds_tile_traits(int i, int *steps, int *nontemporal, int *prefetch, int *lds_ring) {
   if (i < 0 || i >= fds::kNumTiles + fds::kNumStream)
     return fail(FDS_E_ARG, "tile index %d out of range", i);
-  if (!steps || !nontemporal || !prefetch) return fail(FDS_E_ARG, "null argument");
+  if (!steps || !nontemporal || !prefetch || !lds_ring) return fail(FDS_E_ARG, "null argument");
   const bool st = i >= fds::kNumTiles;
   *steps = st ? fds::kStreamShapes[i - fds::kNumTiles][2] : 1;
   *nontemporal = st ? fds::kStreamShapes[i - fds::kNumTiles][3] : 0;
   *prefetch = st ? fds::kStreamShapes[i - fds::kNumTiles][4] : 0;
+  *lds_ring = st ? fds::kStreamShapes[i - fds::kNumTiles][5] : 0;
   return 0;
 }
 
@@ -2006,10 +2023,11 @@ int fds_batch_create(fds_batch **out, fds_ctx *const *ctxs, int n, int tile) {
     return fail(FDS_E_ARG, "tile index %d out of range", t);
   }
   if (t >= fds::kNumTiles && !all_stream) t = fds::kDefaultTile2D;
-  if (t >= fds::kNumTiles && fds::kStreamShapes[t - fds::kNumTiles][4] > 1) {
+  if (t >= fds::kNumTiles && fds::stream_single_only(t - fds::kNumTiles)) {
     fds_batch_destroy(b);
-    return fail(FDS_E_ARG, "shape %d (prefetch %d rows) serves single fields only", t,
-                fds::kStreamShapes[t - fds::kNumTiles][4]);
+    return fail(FDS_E_ARG, "shape %d (prefetch %d rows%s) serves single fields only", t,
+                fds::kStreamShapes[t - fds::kNumTiles][4],
+                fds::kStreamShapes[t - fds::kNumTiles][5] ? ", LDS ring" : "");
   }
   std::vector<int32_t> tinst, base;
   size_t lds = 0;

@@ -17,15 +17,22 @@ constexpr int kTileRow = 5;
 
 // Row-streaming engine (2D models; lossy acoustics one step per launch): TX threads = columns,
 // strips of TY rows, S time steps fused per launch, NT = non-temporal state reads and writes,
-// PF = input rows prefetched ahead (loads in flight per wave); TX = 64 runs one wave per
-// workgroup and exchanges neighbours by lane shuffles.  Shape indices continue the tile table:
-// index kNumTiles + i is kStreamShapes[i] = {TX, TY, S, NT, PF}.
+// PF = input rows loaded ahead (loads in flight per wave), RING = how: 0 into registers, 1 by
+// direct global-to-LDS loads into a ring of PF + 1 rows (no registers held by loads in
+// flight); TX = 64 runs one wave per workgroup and exchanges neighbours by lane shuffles.
+// Shape indices continue the tile table: index kNumTiles + i is
+// kStreamShapes[i] = {TX, TY, S, NT, PF, RING}.
 constexpr int kNumStream = 15;
-constexpr int kStreamShapes[kNumStream][5] = {
-    {256, 16, 1, 0, 1}, {256, 16, 2, 0, 1}, {128, 32, 2, 0, 1}, {256, 32, 3, 0, 1},
-    {256, 16, 1, 1, 1}, {256, 16, 2, 1, 1}, {256, 32, 3, 1, 1}, {64, 16, 1, 0, 1},
-    {64, 16, 2, 0, 1},  {64, 32, 3, 0, 1},  {64, 16, 1, 1, 1},  {64, 16, 2, 1, 1},
-    {64, 32, 3, 0, 2},  {64, 16, 2, 0, 2},  {256, 32, 3, 1, 2}};
+constexpr int kStreamShapes[kNumStream][6] = {
+    {256, 16, 1, 0, 1, 0}, {256, 16, 2, 0, 1, 0}, {128, 32, 2, 0, 1, 0}, {256, 32, 3, 0, 1, 0},
+    {256, 16, 1, 1, 1, 0}, {256, 16, 2, 1, 1, 0}, {256, 32, 3, 1, 1, 0}, {64, 16, 1, 0, 1, 0},
+    {64, 16, 2, 0, 1, 0},  {64, 32, 3, 0, 1, 0},  {64, 16, 1, 1, 1, 0},  {64, 16, 2, 1, 1, 0},
+    {64, 32, 3, 0, 2, 1},  {64, 16, 2, 0, 2, 1},  {64, 32, 3, 0, 1, 1}};
+// Shapes that serve single lossless fields only (no batch or lossy kernels): deeper prefetch
+// or the LDS ring.
+constexpr bool stream_single_only(int i) {
+  return kStreamShapes[i][4] > 1 || kStreamShapes[i][5] != 0;
+}
 constexpr int kMaxStages = 3;   // largest S in kStreamShapes
 // X-macro over the streaming shape indices (each shape is its own translation unit).
 #define FDS_STREAM_SHAPES(X) \

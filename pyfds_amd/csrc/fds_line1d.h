@@ -68,28 +68,6 @@ __host__ __device__ constexpr int line_max_threads(int fam, int i) {
   return fam == kLineNonlinear ? kNlLineMaxThreads[i] : kLineMaxThreads[i];
 }
 
-// One-wave workgroups (64 threads) exchange neighbours by DPP wave shifts instead of LDS, with
-// no barrier at all: lane i receives lane i-1's value (lane 0 gets zero) or lane i+1's value
-// (lane 63 gets zero).  Called with all 64 lanes active.
-__device__ __forceinline__ int dpp_from_left(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);   // wave_shr:1
-}
-__device__ __forceinline__ int dpp_from_right(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);   // wave_shl:1
-}
-__device__ __forceinline__ float wave_from_left(float v) {
-  return __int_as_float(dpp_from_left(__float_as_int(v)));
-}
-__device__ __forceinline__ float wave_from_right(float v) {
-  return __int_as_float(dpp_from_right(__float_as_int(v)));
-}
-__device__ __forceinline__ double wave_from_left(double v) {
-  return __hiloint2double(dpp_from_left(__double2hiint(v)), dpp_from_left(__double2loint(v)));
-}
-__device__ __forceinline__ double wave_from_right(double v) {
-  return __hiloint2double(dpp_from_right(__double2hiint(v)), dpp_from_right(__double2loint(v)));
-}
-
 // Index of this thread in the launch (event lists are CSR over all threads of all segments).
 __device__ __forceinline__ int line_thread() { return blockIdx.x * blockDim.x + threadIdx.x; }
 

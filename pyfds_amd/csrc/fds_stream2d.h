@@ -116,6 +116,79 @@ __device__ __forceinline__ void stn(T *p, int rb, int off, T v) {
   else (p + rb)[off] = v;
 }
 
+// ------------------------------------------------------------------------------------------
+// LDS rings (shapes with RING): every input row of a strip reaches the workgroup by direct
+// global-to-LDS loads (global_load_lds_dwordx4, 16 bytes per lane), PF rows ahead of the row
+// stage 0 consumes, so loads in flight hold no registers and the loop issues no load into a
+// register (whose use the compiler would guard with a wait for all loads in flight).  The
+// state arrays of a row go to slot r % RS of the state ring (RS = PF + 1, read by stage 0);
+// the coefficient arrays to slot r % RC of the coefficient ring (RC = PF + 2S - 1), where the
+// later stages read them 2s iterations after stage 0 (r = row ordinal in the strip).
+//
+// Window: CW elements (CH chunks of E) from the aligned flat index at or below column -1, so
+// every chunk is 16-byte aligned: this needs nx to be a multiple of E and 16-byte-aligned
+// arrays (the host checks both), and then every chunk lies wholly inside or wholly outside
+// the array.  A chunk outside the array loads the array's first chunk instead (the window
+// reaches up to E - 1 cells before column -1 and CW - TX - 2 - lead cells after column TX,
+// beyond what decides whether a tile reads outside the array), and the reads of a tile with
+// out-of-array cells turn those cells into zeros (as ld<true>).
+//
+// Completion: vector-memory loads leave the wave's vmcnt in issue order, so a row's loads are
+// complete once at most NA * (rows issued after it) loads remain outstanding; stores and the
+// event-list loads of tiles with events only make that wait stricter.  Empty asm statements
+// with a memory clobber keep the compiler from moving memory operations across the wait.
+// ------------------------------------------------------------------------------------------
+template <typename T, int TX>
+struct RingGeom {
+  static constexpr int E = 16 / (int)sizeof(T);
+  static constexpr int CW = (TX + 2 + E - 1 + E - 1) / E * E;
+  static constexpr int CH = CW / E;
+  static_assert(CH <= TX, "one ring load per array row needs CH <= TX lanes");
+};
+
+// elements of the two rings of a strip kernel: NS state and NC coefficient arrays
+template <typename T, int TX, int PF, int S, int NS, int NC>
+constexpr int ring_elems() {
+  return ((PF + 1) * NS + (PF + 2 * S - 1) * NC) * RingGeom<T, TX>::CW;
+}
+
+template <int K>
+__device__ __forceinline__ void vm_wait() {
+  static_assert(K >= 0 && K < 64, "vmcnt range");
+  // s_waitcnt fields: vmcnt[3:0] and [15:14]; expcnt[6:4] = 7 and lgkmcnt[11:8] = 15 (no wait)
+  __builtin_amdgcn_s_waitcnt((K & 15) | (7 << 4) | (15 << 8) | ((K >> 4) << 14));
+}
+
+__device__ __forceinline__ void compiler_fence() { __asm__ __volatile__("" ::: "memory"); }
+
+// Wait until at most rows * NA loads are outstanding (rows <= R).
+template <int NA, int R>
+__device__ __forceinline__ void ring_wait(int rows) {
+  if constexpr (R == 0) {
+    vm_wait<0>();
+  } else {
+    if (rows >= R) vm_wait<R * NA>();
+    else ring_wait<NA, R - 1>(rows);
+  }
+}
+
+// Ring loads of one row of NQ arrays (aligned flat start fs) to LDS at `dst` (array q at
+// dst + q * CW); every tile clamps its chunks into the array.
+template <typename T, int TX, int NQ>
+__device__ __forceinline__ void ring_issue(T *dst, const T *const *src, int fs, int n) {
+  using RG = RingGeom<T, TX>;
+  const int t = threadIdx.x;
+  if (t < RG::CH) {
+    int j = fs + t * RG::E;
+    if (j < 0 || j >= n) j = 0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      __builtin_amdgcn_global_load_lds(
+          (const void *)(src[q] + j),
+          (__attribute__((address_space(3))) void *)(dst + q * RG::CW), 16, 0, 0);
+  }
+}
+
 // Whether every read of a streaming tile (rows y0-G-1 .. y0+H+G, columns x0-1 .. x0+TX) falls
 // inside the array, so its loads need no bounds checks.
 __device__ __forceinline__ bool strip_inside(int x0, int y0, int H, int G, int TX, int nx,
@@ -232,12 +305,12 @@ struct AcVel {
 // tile has events; CHK: some reads fall outside the arrays; ISO: the y coefficients are the x
 // ones (isotropic material arrays), so they are neither loaded nor held twice.
 template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool EV, bool CHK, bool ISO,
-          int PF>
+          int PF, bool RING>
 __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
-                               StreamLds<T, TX, S> &L) {
+                               StreamLds<T, TX, S> &L, T *ring) {
   auto &sbp = L.bwd;    // Bx*p of the row, columns -1..TX
   auto &skv = L.fwd;    // forward x products of the row, columns 0..TX
-  constexpr bool WAVE = TX == 64;   // one wave: neighbours exchanged by lane shuffles
+  constexpr bool WAVE = TX == 64;   // one wave: neighbours exchanged by DPP wave shifts
   const int H = a.strip_h, G = a.ghost;
   const int t = threadIdx.x;
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
@@ -274,7 +347,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
     for (int s = 0; s < S; ++s) cur[s] = stage_cursors(a, tile);
   }
   constexpr int RS = PF + 1;   // stage 0 loads its rows PF iterations ahead
-  AcRow<T> nin[RS] = {};       // stage 0 input rows (ring)
+  AcRow<T> nin[RS] = {};      // stage 0 input rows (register ring; RING: unused)
   AcHalo<T> nh[RS] = {};       // stage 0 halo
   T pp[2] = {}, cpp[2] = {};   // stage 0 input p after its events (previous-row p)
   AcCoef<T> cf[S][2] = {};     // stages > 0: coefficients
@@ -317,25 +390,128 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
     d.ky = ISO ? d.kx : ldr<CHK>(Ky, rb, t, n);
   };
 
+  // LDS rings (RING): state p, vx, vy[, cp, cvx, cvy]; coefficients Bx, Kx[, By, Ky]
+  using RG = RingGeom<T, TX>;
+  constexpr int NS = KAHAN ? 6 : 3, NC = ISO ? 2 : 4, NA = NS + NC;
+  constexpr int RC = PF + 2 * S - 1;
+  constexpr int SP = 0, SVX = 1, SVY = 2, SCP = 3, CBX = 0, CKX = 1, CBY = ISO ? 0 : 2,
+                CKY = ISO ? 1 : 3;
+  const int lead = ((x0 - 1) % RG::E + RG::E) % RG::E;
+  T *const cring = ring + RS * NS * RG::CW;
+  const T *ssrc[NS], *csrc[NC];
+  if constexpr (RING) {
+    ssrc[SP] = pin;
+    ssrc[SVX] = vxin;
+    ssrc[SVY] = vyin;
+    if constexpr (KAHAN) {
+      ssrc[SCP] = a.cin[0];
+      ssrc[SCP + 1] = a.cin[1];
+      ssrc[SCP + 2] = a.cin[2];
+    }
+    csrc[CBX] = Bx;
+    csrc[CKX] = Kx;
+    if constexpr (!ISO) {
+      csrc[CBY] = By;
+      csrc[CKY] = Ky;
+    }
+  }
+  // row rho (ordinal rho + G + 1 in the strip) into its slots of both rings
+  auto issue_row = [&](int rho) __attribute__((always_inline)) {
+    const int r = rho + G + 1, fs = (y0 + rho) * nx + x0 - 1 - lead;
+    ring_issue<T, TX, NS>(ring + (r % RS) * NS * RG::CW, ssrc, fs, n);
+    ring_issue<T, TX, NC>(cring + (r % RC) * NC * RG::CW, csrc, fs, n);
+  };
+  // value of ring array q (base: the row's slot) at column c of row rho
+  auto rd = [&](const T *base, int q, int rho, int c) __attribute__((always_inline)) -> T {
+    const T v = base[q * RG::CW + lead + 1 + c];
+    if (!CHK) return v;
+    const int j = (y0 + rho) * nx + x0 + c;
+    return (j >= 0 && j < n) ? v : T(0);
+  };
+  auto sslot = [&](int rho) __attribute__((always_inline)) {
+    return ring + ((rho + G + 1) % RS) * NS * RG::CW;
+  };
+  auto cslot = [&](int rho) __attribute__((always_inline)) {
+    return cring + ((rho + G + 1) % RC) * NC * RG::CW;
+  };
+  auto read_row = [&](int rho, AcRow<T> &d, AcHalo<T> &h) __attribute__((always_inline)) {
+    const T *sb = sslot(rho), *cb = cslot(rho);
+    d.p = rd(sb, SP, rho, t);
+    d.vx = rd(sb, SVX, rho, t);
+    d.vy = rd(sb, SVY, rho, t);
+    d.bx = rd(cb, CBX, rho, t);
+    d.by = ISO ? d.bx : rd(cb, CBY, rho, t);
+    d.kx = rd(cb, CKX, rho, t);
+    d.ky = ISO ? d.kx : rd(cb, CKY, rho, t);
+    if (KAHAN) {
+      d.cp = rd(sb, SCP, rho, t);
+      d.cvx = rd(sb, SCP + 1, rho, t);
+      d.cvy = rd(sb, SCP + 2, rho, t);
+    }
+    if (lh || rh) {
+      const int off = lh ? -1 : TX;
+      h.p = rd(sb, SP, rho, off);
+      h.b = rd(cb, CBX, rho, off);
+      if (KAHAN) h.cp = rd(sb, SCP, rho, off);
+    }
+    if (rh) {
+      h.vx = rd(sb, SVX, rho, TX);
+      h.k = rd(cb, CKX, rho, TX);
+      if (KAHAN) h.cvx = rd(sb, SCP + 1, rho, TX);
+    }
+  };
+
 #pragma unroll
-  for (int r = 0; r < PF; ++r) load_row(-G - 1 + r, nin[r], nh[r]);
+  for (int r = 0; r < PF; ++r) {
+    if constexpr (RING) issue_row(-G - 1 + r);
+    else load_row(-G - 1 + r, nin[r], nh[r]);
+  }
   const int iters = H + 2 * G + S + 1;
-  run_loop<RS>(iters, [&](auto Qc, auto Wc, int i) __attribute__((always_inline)) {
-    constexpr int Q = decltype(Qc)::value, W = decltype(Wc)::value;
+  // RING: slots are computed (LDS addresses), so the loop only unrolls by the parity
+  run_loop<RING ? 1 : RS>(iters, [&](auto Qc, auto Wc, int i) __attribute__((always_inline)) {
+    constexpr int Q = decltype(Qc)::value, W = RING ? 0 : decltype(Wc)::value;
     T bpx[S];
+    AcRow<T> row0 = {};      // RING: stage 0 input row of this iteration, read from the rings
+    AcHalo<T> h0 = {};
+    AcRow<T> &in0 = RING ? row0 : nin[W];
+    AcHalo<T> &hin = RING ? h0 : nh[W];
     // ---- before the barrier
     static_for<S>([&](auto Sc) __attribute__((always_inline)) {
       constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
       const StageRows R(G, H, s);
-      if (s > 0 && rho + 1 >= R.lo - 1 && rho + 1 <= R.hi + 1) load_coef(rho + 1, cf[s][Q ^ 1]);
+      if (!RING && s > 0 && rho + 1 >= R.lo - 1 && rho + 1 <= R.hi + 1)
+        load_coef(rho + 1, cf[s][Q ^ 1]);
       if (rho < R.lo - 1 || rho > R.hi + 1) return;
-      if (s == 0 && rho + PF <= R.hi + 1) load_row(rho + PF, nin[(W + PF) % RS], nh[(W + PF) % RS]);
-      T &p = s == 0 ? nin[W].p : pn[s > 0 ? s - 1 : 0][Q ^ 1];
-      T &cp = s == 0 ? nin[W].cp : cpn[s > 0 ? s - 1 : 0][Q ^ 1];
-      const T cbx = s == 0 ? nin[W].bx : cf[s][Q].bx;
-      const T cby = s == 0 ? nin[W].by : cf[s][Q].by;
-      AcHalo<T> &h = nh[W];
+      if constexpr (RING) {
+        if (s == 0) {
+          compiler_fence();
+          if (rho + PF <= R.hi + 1) issue_row(rho + PF);
+          // loads issued after this row's: the rows rho + 1 .. min(rho + PF, hi + 1)
+          ring_wait<NA, PF>(R.hi + 1 - rho);
+          compiler_fence();
+          read_row(rho, row0, h0);
+        }
+      } else {
+        if (s == 0 && rho + PF <= R.hi + 1)
+          load_row(rho + PF, nin[(W + PF) % RS], nh[(W + PF) % RS]);
+      }
+      T &p = s == 0 ? in0.p : pn[s > 0 ? s - 1 : 0][Q ^ 1];
+      T &cp = s == 0 ? in0.cp : cpn[s > 0 ? s - 1 : 0][Q ^ 1];
+      T cbx, cby;
+      if constexpr (RING) {
+        if (s == 0) {
+          cbx = in0.bx;
+          cby = in0.by;
+        } else {
+          cbx = rd(cslot(rho), CBX, rho, t);
+          cby = ISO ? cbx : rd(cslot(rho), CBY, rho, t);
+        }
+      } else {
+        cbx = s == 0 ? in0.bx : cf[s][Q].bx;
+        cby = s == 0 ? in0.by : cf[s][Q].by;
+      }
+      AcHalo<T> &h = hin;
       if constexpr (EV) {
         stream_events<T, KAHAN>(a.bev[0], a.prog[0], cur[s].b[0], TX + 2, -G - 1, -1, rho, t,
                                 step + s, p, cp, s == 0 && (lh || rh), lh ? -1 : TX, h.p, h.cp);
@@ -361,20 +537,31 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       const StageRows R(G, H, s);
       if (rho < R.lo - 1 || rho > R.hi + 1) return;
       const int sp = s > 0 ? s - 1 : 0;
-      const T ivx = s == 0 ? nin[W].vx : vh[sp][Q].vx;
-      const T icvx = s == 0 ? nin[W].cvx : vh[sp][Q].cvx;
-      const T ivy = s == 0 ? nin[W].vy : vh[sp][Q].vy;
-      const T icvy = s == 0 ? nin[W].cvy : vh[sp][Q].cvy;
-      const T ckx = s == 0 ? nin[W].kx : cf[s][Q].kx;
-      const T cky = s == 0 ? nin[W].ky : cf[s][Q].ky;
+      const T ivx = s == 0 ? in0.vx : vh[sp][Q].vx;
+      const T icvx = s == 0 ? in0.cvx : vh[sp][Q].cvx;
+      const T ivy = s == 0 ? in0.vy : vh[sp][Q].vy;
+      const T icvy = s == 0 ? in0.cvy : vh[sp][Q].cvy;
+      T ckx, cky;
+      if constexpr (RING) {
+        if (s == 0) {
+          ckx = in0.kx;
+          cky = in0.ky;
+        } else {
+          ckx = rd(cslot(rho), CKX, rho, t);
+          cky = ISO ? ckx : rd(cslot(rho), CKY, rho, t);
+        }
+      } else {
+        ckx = s == 0 ? in0.kx : cf[s][Q].kx;
+        cky = s == 0 ? in0.ky : cf[s][Q].ky;
+      }
       const T pprev = s == 0 ? pp[Q ^ 1] : pn[sp][Q];
       const T cpprev = s == 0 ? cpp[Q ^ 1] : cpn[sp][Q];
-      const AcHalo<T> &h = nh[W];
+      const AcHalo<T> &h = hin;
       T vxn = T(0), cvxn = T(0), vyn = T(0), cvyn = T(0);
       if (rho >= R.lo && rho <= R.hi) {
         T left;
         if constexpr (WAVE) {
-          left = __shfl_up(bpx[s], 1);
+          left = wave_from_left(bpx[s]);
           if (s == 0 && lh) left = h.b * h.p;
         } else {
           left = sbp[s][Q][t];
@@ -405,7 +592,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
           else kvh = h.k * hvxn;
         }
         if constexpr (WAVE) {
-          T right = __shfl_down(kvx, 1);
+          T right = wave_from_right(kvx);
           if (s == 0 && rh) right = kvh;
           kvd[s][Q] = (-kvx) + right;   // x difference of row rho, used by next iteration
         } else {
@@ -482,9 +669,9 @@ struct ThHalo {        // left thread: T, Q of column -1; right thread: T, Q, G 
 };
 
 template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool EV, bool CHK, bool ISO,
-          int PF>
+          int PF, bool RING>
 __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
-                              StreamLds<T, TX, S> &L) {
+                              StreamLds<T, TX, S> &L, T *ring) {
   auto &sqt = L.bwd;    // Qx*T of the row
   auto &sgq = L.fwd;    // forward x products of the row
   constexpr bool WAVE = TX == 64;
@@ -557,24 +744,106 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
     d.gy = ISO ? d.gx : ldr<CHK>(Gy, rb, t, n);
   };
 
+  // LDS rings (RING): state T[, cT]; coefficients Qx, Gx[, Qy, Gy]
+  using RG = RingGeom<T, TX>;
+  constexpr int NS = KAHAN ? 2 : 1, NC = ISO ? 2 : 4, NA = NS + NC;
+  constexpr int RC = PF + 2 * S - 1;
+  constexpr int ST = 0, SCT = 1, CQX = 0, CGX = 1, CQY = ISO ? 0 : 2, CGY = ISO ? 1 : 3;
+  const int lead = ((x0 - 1) % RG::E + RG::E) % RG::E;
+  T *const cring = ring + RS * NS * RG::CW;
+  const T *ssrc[NS], *csrc[NC];
+  if constexpr (RING) {
+    ssrc[ST] = tin;
+    if constexpr (KAHAN) ssrc[SCT] = a.cin[0];
+    csrc[CQX] = Qx;
+    csrc[CGX] = Gx;
+    if constexpr (!ISO) {
+      csrc[CQY] = Qy;
+      csrc[CGY] = Gy;
+    }
+  }
+  auto issue_row = [&](int rho) __attribute__((always_inline)) {
+    const int r = rho + G + 1, fs = (y0 + rho) * nx + x0 - 1 - lead;
+    ring_issue<T, TX, NS>(ring + (r % RS) * NS * RG::CW, ssrc, fs, n);
+    ring_issue<T, TX, NC>(cring + (r % RC) * NC * RG::CW, csrc, fs, n);
+  };
+  auto rd = [&](const T *base, int q, int rho, int c) __attribute__((always_inline)) -> T {
+    const T v = base[q * RG::CW + lead + 1 + c];
+    if (!CHK) return v;
+    const int j = (y0 + rho) * nx + x0 + c;
+    return (j >= 0 && j < n) ? v : T(0);
+  };
+  auto sslot = [&](int rho) __attribute__((always_inline)) {
+    return ring + ((rho + G + 1) % RS) * NS * RG::CW;
+  };
+  auto cslot = [&](int rho) __attribute__((always_inline)) {
+    return cring + ((rho + G + 1) % RC) * NC * RG::CW;
+  };
+  auto read_row = [&](int rho, ThRow<T> &d, ThHalo<T> &h) __attribute__((always_inline)) {
+    const T *sb = sslot(rho), *cb = cslot(rho);
+    d.t = rd(sb, ST, rho, t);
+    d.qx = rd(cb, CQX, rho, t);
+    d.qy = ISO ? d.qx : rd(cb, CQY, rho, t);
+    d.gx = rd(cb, CGX, rho, t);
+    d.gy = ISO ? d.gx : rd(cb, CGY, rho, t);
+    if (KAHAN) d.ct = rd(sb, SCT, rho, t);
+    if (lh || rh) {
+      const int off = lh ? -1 : TX;
+      h.t = rd(sb, ST, rho, off);
+      h.q = rd(cb, CQX, rho, off);
+      if (KAHAN) h.ct = rd(sb, SCT, rho, off);
+    }
+    if (rh) h.g = rd(cb, CGX, rho, TX);
+  };
+
 #pragma unroll
-  for (int r = 0; r < PF; ++r) load_row(-G - 1 + r, nin[r], nh[r]);
+  for (int r = 0; r < PF; ++r) {
+    if constexpr (RING) issue_row(-G - 1 + r);
+    else load_row(-G - 1 + r, nin[r], nh[r]);
+  }
   const int iters = H + 2 * G + S + 1;
-  run_loop<RS>(iters, [&](auto Qc, auto Wc, int i) __attribute__((always_inline)) {
-    constexpr int Q = decltype(Qc)::value, W = decltype(Wc)::value;
+  run_loop<RING ? 1 : RS>(iters, [&](auto Qc, auto Wc, int i) __attribute__((always_inline)) {
+    constexpr int Q = decltype(Qc)::value, W = RING ? 0 : decltype(Wc)::value;
     T qtx[S];
+    ThRow<T> row0 = {};      // RING: stage 0 input row of this iteration, read from the rings
+    ThHalo<T> h0 = {};
+    ThRow<T> &in0 = RING ? row0 : nin[W];
+    ThHalo<T> &hin = RING ? h0 : nh[W];
     static_for<S>([&](auto Sc) __attribute__((always_inline)) {
       constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
       const StageRows R(G, H, s);
-      if (s > 0 && rho + 1 >= R.lo - 1 && rho + 1 <= R.hi + 1) load_coef(rho + 1, cf[s][Q ^ 1]);
+      if (!RING && s > 0 && rho + 1 >= R.lo - 1 && rho + 1 <= R.hi + 1)
+        load_coef(rho + 1, cf[s][Q ^ 1]);
       if (rho < R.lo - 1 || rho > R.hi + 1) return;
-      if (s == 0 && rho + PF <= R.hi + 1) load_row(rho + PF, nin[(W + PF) % RS], nh[(W + PF) % RS]);
-      T &tv = s == 0 ? nin[W].t : tn[s > 0 ? s - 1 : 0][Q ^ 1];
-      T &ct = s == 0 ? nin[W].ct : ctn[s > 0 ? s - 1 : 0][Q ^ 1];
-      const T cqx = s == 0 ? nin[W].qx : cf[s][Q].qx;
-      const T cqy = s == 0 ? nin[W].qy : cf[s][Q].qy;
-      ThHalo<T> &h = nh[W];
+      if constexpr (RING) {
+        if (s == 0) {
+          compiler_fence();
+          if (rho + PF <= R.hi + 1) issue_row(rho + PF);
+          ring_wait<NA, PF>(R.hi + 1 - rho);
+          compiler_fence();
+          read_row(rho, row0, h0);
+        }
+      } else {
+        if (s == 0 && rho + PF <= R.hi + 1)
+          load_row(rho + PF, nin[(W + PF) % RS], nh[(W + PF) % RS]);
+      }
+      T &tv = s == 0 ? in0.t : tn[s > 0 ? s - 1 : 0][Q ^ 1];
+      T &ct = s == 0 ? in0.ct : ctn[s > 0 ? s - 1 : 0][Q ^ 1];
+      T cqx, cqy;
+      if constexpr (RING) {
+        if (s == 0) {
+          cqx = in0.qx;
+          cqy = in0.qy;
+        } else {
+          cqx = rd(cslot(rho), CQX, rho, t);
+          cqy = ISO ? cqx : rd(cslot(rho), CQY, rho, t);
+        }
+      } else {
+        cqx = s == 0 ? in0.qx : cf[s][Q].qx;
+        cqy = s == 0 ? in0.qy : cf[s][Q].qy;
+      }
+      ThHalo<T> &h = hin;
       if constexpr (EV) {
         stream_events<T, KAHAN>(a.bev[0], a.prog[0], cur[s].b[0], TX + 2, -G - 1, -1, rho, t,
                                 step + s, tv, ct, s == 0 && (lh || rh), lh ? -1 : TX, h.t, h.ct);
@@ -599,17 +868,28 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
       const StageRows R(G, H, s);
       if (rho < R.lo - 1 || rho > R.hi + 1) return;
       const int sp = s > 0 ? s - 1 : 0;
-      const T cgx = s == 0 ? nin[W].gx : cf[s][Q].gx;
-      const T cgy = s == 0 ? nin[W].gy : cf[s][Q].gy;
+      T cgx, cgy;
+      if constexpr (RING) {
+        if (s == 0) {
+          cgx = in0.gx;
+          cgy = in0.gy;
+        } else {
+          cgx = rd(cslot(rho), CGX, rho, t);
+          cgy = ISO ? cgx : rd(cslot(rho), CGY, rho, t);
+        }
+      } else {
+        cgx = s == 0 ? in0.gx : cf[s][Q].gx;
+        cgy = s == 0 ? in0.gy : cf[s][Q].gy;
+      }
       const T tprev = s == 0 ? tp[Q ^ 1] : tn[sp][Q];
       const T ctprev = s == 0 ? ctp[Q ^ 1] : ctn[sp][Q];
-      const ThHalo<T> &h = nh[W];
+      const ThHalo<T> &h = hin;
       T zero = T(0), zeroc = T(0);
       T fx = T(0), fy = T(0);
       if (rho >= R.lo && rho <= R.hi) {
         T left;
         if constexpr (WAVE) {
-          left = __shfl_up(qtx[s], 1);
+          left = wave_from_left(qtx[s]);
           if (s == 0 && lh) left = h.q * h.t;
         } else {
           left = sqt[s][Q][t];
@@ -632,7 +912,7 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
           else gqh = h.g * hfx;
         }
         if constexpr (WAVE) {
-          T right = __shfl_down(gqx, 1);
+          T right = wave_from_right(gqx);
           if (s == 0 && rh) right = gqh;
           gqd[s][Q] = (-gqx) + right;
         } else {
@@ -692,30 +972,34 @@ __device__ __forceinline__ int strip_path(const StepArgs<T> &a, int tile) {
   return __builtin_amdgcn_readfirstlane((tile_has_events(a, tile) ? 2 : 0) | (inside ? 1 : 0));
 }
 
-#define FDS_STRIP_CASES(FN, AXI, ISO, BASE)                                                  \
-  case BASE + 0: FN<T, KAHAN, TX, AXI, S, NT, false, true, ISO, PF>(a, tile, step, row, L);   \
-    break;                                                                                     \
-  case BASE + 1: FN<T, KAHAN, TX, AXI, S, NT, false, false, ISO, PF>(a, tile, step, row, L);  \
-    break;                                                                                     \
-  case BASE + 2: FN<T, KAHAN, TX, AXI, S, NT, true, true, ISO, PF>(a, tile, step, row, L);    \
-    break;                                                                                     \
-  case BASE + 3: FN<T, KAHAN, TX, AXI, S, NT, true, false, ISO, PF>(a, tile, step, row, L);   \
+#define FDS_STRIP_CASES(FN, AXI, ISO, BASE)                                                   \
+  case BASE + 0:                                                                                \
+    FN<T, KAHAN, TX, AXI, S, NT, false, true, ISO, PF, RING>(a, tile, step, row, L, ring);      \
+    break;                                                                                      \
+  case BASE + 1:                                                                                \
+    FN<T, KAHAN, TX, AXI, S, NT, false, false, ISO, PF, RING>(a, tile, step, row, L, ring);     \
+    break;                                                                                      \
+  case BASE + 2:                                                                                \
+    FN<T, KAHAN, TX, AXI, S, NT, true, true, ISO, PF, RING>(a, tile, step, row, L, ring);       \
+    break;                                                                                      \
+  case BASE + 3:                                                                                \
+    FN<T, KAHAN, TX, AXI, S, NT, true, false, ISO, PF, RING>(a, tile, step, row, L, ring);      \
     break;
 
 // ISO kernels serve fields whose y coefficient arrays are the x ones (the host checks).  PF:
 // rows of stage-0 input prefetched ahead (more loads in flight per wave, more registers).
-template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool ISO, int PF>
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool ISO, int PF, bool RING>
 __device__ __forceinline__ void acoustic_stream(const StepArgs<T> &a, int tile, int64_t step,
-                                                int64_t row, StreamLds<T, TX, S> &L) {
+                                                int64_t row, StreamLds<T, TX, S> &L, T *ring) {
   switch (strip_path<T, TX>(a, tile)) {
     FDS_STRIP_CASES(acoustic_strip, AXI, ISO, 0)
     default: break;
   }
 }
 
-template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool ISO, int PF>
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool ISO, int PF, bool RING>
 __device__ __forceinline__ void thermal_stream(const StepArgs<T> &a, int tile, int64_t step,
-                                               int64_t row, StreamLds<T, TX, S> &L) {
+                                               int64_t row, StreamLds<T, TX, S> &L, T *ring) {
   switch (strip_path<T, TX>(a, tile)) {
     FDS_STRIP_CASES(thermal_strip, AXI, ISO, 0)
     default: break;
@@ -727,6 +1011,8 @@ __device__ __forceinline__ void thermal_stream(const StepArgs<T> &a, int tile, i
 template <typename T, bool KAHAN, int TX, int S, bool NT, int PF>
 __device__ __forceinline__ void any_stream(const StepArgs<T> &a, int tile, int64_t step,
                                            int64_t row, StreamLds<T, TX, S> &L) {
+  constexpr bool RING = false;
+  T *ring = nullptr;
   const int f = a.flags;
   const int model = ((f & kFlagThermal) ? 2 : 0) | ((f & kFlagAxi) ? 1 : 0);
   switch (__builtin_amdgcn_readfirstlane(model * 4 + strip_path<T, TX>(a, tile))) {

@@ -2,7 +2,7 @@
 // one time step per launch.
 //
 // Same strip walk as the lossless kernels (a workgroup of TX threads owns a block of columns and
-// marches down a strip of H rows, one barrier per row, or lane shuffles and no barrier for
+// marches down a strip of H rows, one barrier per row, or DPP wave shifts and no barrier for
 // TX = 64), plus the viscous operator V (pyfds/acoustics.py:104-107, :192-201) on both
 // velocities:
 //   vx <- vx - (((Dbwd(Bx) p) - V vx) [+ (Mt vx) / r^2])      vy <- vy - ((Dbwd(By) p) - V vy)
@@ -34,7 +34,7 @@ struct LsHalo {  // last thread: column TX's old vx and its viscous x factors
 };
 
 template <typename T, int TX, bool AXI>
-struct LossyLds {                     // (one-wave workgroups exchange by lane shuffles)
+struct LossyLds {                     // (one-wave workgroups exchange by DPP wave shifts)
   static constexpr int W = TX == 64 ? 1 : TX + 2;
   T bwd[2][W];                        // Bx*p, columns -1..TX
   T fwd[2][W];                        // forward x products of the new vx, columns 0..TX
@@ -185,10 +185,10 @@ __device__ __forceinline__ void lossy_strip(const StepArgs<T> &a, int tile, int6
     // neighbour products: left's left-factor product, right's right-factor product
     T qlx, qrx, qly, qry;
     if constexpr (WAVE) {
-      qlx = __shfl_up(q[0], 1);
-      qrx = __shfl_down(q[1], 1);
-      qly = __shfl_up(q[2], 1);
-      qry = __shfl_down(q[3], 1);
+      qlx = wave_from_left(q[0]);
+      qrx = wave_from_right(q[1]);
+      qly = wave_from_left(q[2]);
+      qry = wave_from_right(q[3]);
       if (rh) qrx = qh;
     } else if (AXI) {
       qlx = t > 0 ? L.vis[Q][0][t - 1] : T(0);
@@ -203,7 +203,7 @@ __device__ __forceinline__ void lossy_strip(const StepArgs<T> &a, int tile, int6
     }
     if (rho >= 0 && rho <= H - 1) {
       T left;
-      if constexpr (WAVE) left = __shfl_up(bpx, 1);
+      if constexpr (WAVE) left = wave_from_left(bpx);
       else left = L.bwd[Q][t];
       T d = (-left) + bpx;
       T y = T(0);
@@ -230,7 +230,7 @@ __device__ __forceinline__ void lossy_strip(const StepArgs<T> &a, int tile, int6
       if (AXI) kvx = (P.kx / rad) * (vxn * rad);
       else kvx = P.kx * vxn;
       if constexpr (WAVE) {
-        const T right = __shfl_down(kvx, 1);
+        const T right = wave_from_right(kvx);
         kvd[Q] = (-kvx) + right;   // x difference of row rho, used by the next iteration
       } else {
         L.fwd[Q][t] = kvx;

@@ -34,21 +34,28 @@ namespace FDS_FORM_NS {
 using T = FDS_T;
 constexpr bool K = FDS_KAHAN;
 // PF: rows of input prefetched ahead by stage 0 (the shape's own depth for its full stage
-// count; remainder launches of fewer stages use one row)
-template <int TX, bool AXI, int S, bool NT, bool ISO, int PF>
+// count; remainder launches of fewer stages use one row), into registers or (RING, one-wave
+// workgroups) into LDS rings filled by direct global-to-LDS loads.
+template <int TX, bool AXI, int S, bool NT, bool ISO, int PF, bool RING>
 __global__ __launch_bounds__(TX) void k_stream_acoustic(StepArgs<T> a) {
+  static_assert(!RING || TX == 64, "the LDS ring serves one-wave workgroups");
   __shared__ StreamLds<T, TX, S> L;
+  __shared__ T ring[RING ? ring_elems<T, TX, PF, S, K ? 6 : 3, ISO ? 2 : 4>() : 1];
   const int tile = xcd_tile(blockIdx.x, gridDim.x, a.ntiles);
   if (tile < 0) return;
-  acoustic_stream<T, K, TX, AXI, S, NT, ISO, PF>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
+  acoustic_stream<T, K, TX, AXI, S, NT, ISO, PF, RING>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k,
+                                                       L, ring);
 }
 
-template <int TX, bool AXI, int S, bool NT, bool ISO, int PF>
+template <int TX, bool AXI, int S, bool NT, bool ISO, int PF, bool RING>
 __global__ __launch_bounds__(TX) void k_stream_thermal(StepArgs<T> a) {
+  static_assert(!RING || TX == 64, "the LDS ring serves one-wave workgroups");
   __shared__ StreamLds<T, TX, S> L;
+  __shared__ T ring[RING ? ring_elems<T, TX, PF, S, K ? 2 : 1, ISO ? 2 : 4>() : 1];
   const int tile = xcd_tile(blockIdx.x, gridDim.x, a.ntiles);
   if (tile < 0) return;
-  thermal_stream<T, K, TX, AXI, S, NT, ISO, PF>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k, L);
+  thermal_stream<T, K, TX, AXI, S, NT, ISO, PF, RING>(a, tile, a.dyn[0] + a.k, a.dyn[1] + a.k,
+                                                      L, ring);
 }
 
 template <int TX, bool AXI, bool NT, bool ISO>
@@ -103,16 +110,16 @@ __global__ __launch_bounds__(TX) void k_stream_batch_wq(const StepArgs<T> *__res
   }
 }
 
-template <int TX, bool AXI, int S, bool NT, bool ISO, int PF>
+template <int TX, bool AXI, int S, bool NT, bool ISO, int PF, bool RING>
 void launch_stream_acoustic(const void *args, int nblocks, size_t, hipStream_t s) {
-  hipLaunchKernelGGL((k_stream_acoustic<TX, AXI, S, NT, ISO, PF>), dim3(nblocks), dim3(TX), 0, s,
-                     *static_cast<const StepArgs<T> *>(args));
+  hipLaunchKernelGGL((k_stream_acoustic<TX, AXI, S, NT, ISO, PF, RING>), dim3(nblocks), dim3(TX),
+                     0, s, *static_cast<const StepArgs<T> *>(args));
 }
 
-template <int TX, bool AXI, int S, bool NT, bool ISO, int PF>
+template <int TX, bool AXI, int S, bool NT, bool ISO, int PF, bool RING>
 void launch_stream_thermal(const void *args, int nblocks, size_t, hipStream_t s) {
-  hipLaunchKernelGGL((k_stream_thermal<TX, AXI, S, NT, ISO, PF>), dim3(nblocks), dim3(TX), 0, s,
-                     *static_cast<const StepArgs<T> *>(args));
+  hipLaunchKernelGGL((k_stream_thermal<TX, AXI, S, NT, ISO, PF, RING>), dim3(nblocks), dim3(TX),
+                     0, s, *static_cast<const StepArgs<T> *>(args));
 }
 
 template <int TX, bool AXI, bool NT, bool ISO>
@@ -145,18 +152,19 @@ void launch_stream_batch_wq(const BatchLaunch &b, int, size_t, hipStream_t s) {
 // Variants [v]: 0 plain, 1 axial-symmetric, 2 isotropic coefficients (y arrays = x arrays, not
 // axial-symmetric).  [.][f]: f + 1 stages per launch, for f + 1 from 1 (remainder steps) to the
 // shape's own stage count; fewer stages than the tiling's ghost depth allows are exact too.
-// Remainder variants (fewer stages than the shape's) prefetch one row.  Batches (BATCH) run
-// their full launches on the shape's stage count and remainder steps one stage at a time; the
-// work-queue variant (WQ) exists for the full launches (remainder launches of a work-queue
-// batch run on the dispatcher kernel: they only end runs, and a run starts the queue afresh).
-template <int TX, int S, bool NT, int PF, bool BATCH, bool WQ>
+// Remainder variants (fewer stages than the shape's) prefetch one row into registers.  Batches
+// (BATCH) run their full launches on the shape's stage count and remainder steps one stage at
+// a time; the work-queue variant (WQ) exists for the full launches (remainder launches of a
+// work-queue batch run on the dispatcher kernel: they only end runs, and a run starts the
+// queue afresh).
+template <int TX, int S, bool NT, int PF, bool RING, bool BATCH, bool WQ>
 void fill_variants(KernelSet &ks, int I, int f) {
-  ks.stream_acoustic[I][0][f] = launch_stream_acoustic<TX, false, S, NT, false, PF>;
-  ks.stream_acoustic[I][1][f] = launch_stream_acoustic<TX, true, S, NT, false, PF>;
-  ks.stream_acoustic[I][2][f] = launch_stream_acoustic<TX, false, S, NT, true, PF>;
-  ks.stream_thermal[I][0][f] = launch_stream_thermal<TX, false, S, NT, false, PF>;
-  ks.stream_thermal[I][1][f] = launch_stream_thermal<TX, true, S, NT, false, PF>;
-  ks.stream_thermal[I][2][f] = launch_stream_thermal<TX, false, S, NT, true, PF>;
+  ks.stream_acoustic[I][0][f] = launch_stream_acoustic<TX, false, S, NT, false, PF, RING>;
+  ks.stream_acoustic[I][1][f] = launch_stream_acoustic<TX, true, S, NT, false, PF, RING>;
+  ks.stream_acoustic[I][2][f] = launch_stream_acoustic<TX, false, S, NT, true, PF, RING>;
+  ks.stream_thermal[I][0][f] = launch_stream_thermal<TX, false, S, NT, false, PF, RING>;
+  ks.stream_thermal[I][1][f] = launch_stream_thermal<TX, true, S, NT, false, PF, RING>;
+  ks.stream_thermal[I][2][f] = launch_stream_thermal<TX, false, S, NT, true, PF, RING>;
   if constexpr (BATCH) {
     ks.stream_batch[I][f] = launch_stream_batch<TX, S, NT, PF>;
     if constexpr (WQ)
@@ -173,15 +181,18 @@ void FDS_CAT(fill_stream_, FDS_SHAPE)(KernelSet &ks) {
   constexpr int S = kStreamShapes[I][2];
   constexpr bool NT = kStreamShapes[I][3] != 0;
   constexpr int PF = kStreamShapes[I][4];
+  constexpr bool RING = kStreamShapes[I][5] != 0;
   static_assert(S >= 1 && S <= kMaxStages, "stage count out of range");
   static_assert(PF >= 1, "prefetch depth out of range");
-  // shapes that only add prefetch depth serve single fields (no batch, no lossy kernels)
-  constexpr bool B = PF == 1;
-  fill_variants<TX, 1, NT, S == 1 ? PF : 1, B, B && S == 1>(ks, I, 0);
-  if constexpr (S >= 2) fill_variants<TX, 2, NT, S == 2 ? PF : 1, B && S == 2, B>(ks, I, 1);
-  if constexpr (S >= 3) fill_variants<TX, 3, NT, PF, B, B>(ks, I, 2);
+  // shapes with deeper prefetch or the LDS ring serve single fields (no batch, no lossy
+  // kernels); their own depth and ring apply to the full stage count
+  constexpr bool B = !stream_single_only(I);
+  fill_variants<TX, 1, NT, S == 1 ? PF : 1, RING && S == 1, B, B && S == 1>(ks, I, 0);
+  if constexpr (S >= 2)
+    fill_variants<TX, 2, NT, S == 2 ? PF : 1, RING && S == 2, B && S == 2, B>(ks, I, 1);
+  if constexpr (S >= 3) fill_variants<TX, 3, NT, PF, RING, B, B>(ks, I, 2);
   // lossy acoustics: one step per launch on every shape (its own row ring)
-  if constexpr (PF == 1) {
+  if constexpr (B) {
     ks.stream_lossy[I][0] = launch_stream_lossy<TX, false, NT, false>;
     ks.stream_lossy[I][1] = launch_stream_lossy<TX, true, NT, false>;
     ks.stream_lossy[I][2] = launch_stream_lossy<TX, false, NT, true>;

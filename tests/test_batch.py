@@ -231,12 +231,13 @@ def test_queue_schedule_needs_streaming_shape():
         batch.simulate(2)
 
 
-def test_prefetch_shapes_serve_single_fields_only():
+def test_ring_and_prefetch_shapes_serve_single_fields_only():
     members = sweep_fields('fp64')[:2]
-    idx = [d['index'] for d in pyfds_amd._native.tile_table() if d['prefetch'] > 1]
-    assert idx
-    with pytest.raises(ValueError):
-        BatchRunner([f for _, f in members], tile=idx[0]).simulate(2)
+    idx = [d['index'] for d in pyfds_amd._native.tile_table() if d['single_only']]
+    assert len(idx) == 3
+    for i in idx:
+        with pytest.raises(ValueError):
+            BatchRunner([f for _, f in members], tile=i).simulate(2)
 
 
 @pytest.mark.parametrize('form', ['fp32', 'fp32_kahan'])

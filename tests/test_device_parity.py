@@ -87,8 +87,9 @@ def test_fp64_tile_shapes_bit_identical(tile):
 # table: 0 (256 x 16, one step per launch), 1 (256 x 16, two fused steps), 2 (128 x 32, two),
 # 3 (256 x 32, three), 4-6 the same with non-temporal state traffic, 7-9 one-wave 64-wide
 # shapes with one, two and three steps, 10-11 64-wide non-temporal with one and two steps
-# (tests/test_host_api.py pins these traits), 12-14 prefetching two rows ahead: 64 x 32 three
-# steps, 64 x 16 two steps, 256 x 32 three steps non-temporal.
+# (tests/test_host_api.py pins these traits), 12-14 loading their rows straight into LDS rings:
+# 64 x 32 three steps two rows ahead, 64 x 16 two steps two rows ahead, 64 x 32 three steps
+# one row ahead (nx must be a multiple of 2 in FP64, of 4 in FP32).
 NSTREAM = 15
 
 
@@ -109,8 +110,10 @@ def test_fp64_stream_shapes_match_reference_golden(name, slot):
     scene = scenes.SCENES[name]()
     gold = np.load(os.path.join(GOLDEN, name + '.npz'))
     print('scene', name, 'shape', idx[slot], flush=True)
-    if 'lossy' in name and pyfds_amd._native.tile_table()[idx[slot]]['prefetch'] > 1:
-        with pytest.raises(ValueError):           # prefetch shapes serve lossless fields
+    info = pyfds_amd._native.tile_table()[idx[slot]]
+    if ('lossy' in name and info['single_only']) or \
+            (info['lds_ring'] and scene['kwargs']['x_samples'] % 2):
+        with pytest.raises(ValueError):    # lossless fields only; ring rows 16-byte aligned
             run_device(scene, tile=idx[slot])
         return
     res = run_device(scene, tile=idx[slot])
@@ -169,10 +172,10 @@ def test_fp32_stream_engine_equals_tile_engine(name, form):
                                               ('Acoustic2D', 61, 45, 9),
                                               ('Acoustic3DAxi', 140, 50, 10),
                                               ('Acoustic2D', 200, 120, 11),
-                                              ('Acoustic2D', 333, 101, 12),
+                                              ('Acoustic2D', 334, 101, 12),
                                               ('Acoustic3DAxi', 190, 77, 13),
-                                              ('Acoustic2D', 509, 97, 14),
-                                              ('Acoustic3DAxi', 251, 100, 14)])
+                                              ('Acoustic2D', 510, 97, 14),
+                                              ('Acoustic3DAxi', 252, 100, 14)])
 def test_fp64_stream_multi_strip_grids_match_oracle_bitwise(model, nx, ny, slot):
     """Partial column blocks at the right edge and partial strips at the bottom."""
     idx = _stream_indices()[slot]
@@ -289,7 +292,7 @@ def _big_acoustic(nx, ny, steps, form='fp64', seed=1, model='Acoustic2D', mu=0.0
                                                     ('Acoustic2D', 150, 71, 7, 20),
                                                     ('Acoustic3DAxi', 180, 95, 1, 6),
                                                     ('Acoustic2D', 210, 85, 11, 12),
-                                                    ('Acoustic2D', 333, 101, 12, 24),
+                                                    ('Acoustic2D', 334, 101, 12, 24),
                                                     ('Acoustic3DAxi', 200, 97, 12, 20),
                                                     ('Acoustic2D', 150, 71, 13, 12),
                                                     ('Acoustic2D', 270, 83, 14, 24)])
@@ -318,7 +321,8 @@ def test_fp64_multi_tile_grids_match_oracle_bitwise(model, mu, nx, ny):
 
 @pytest.mark.parametrize('tile', [None, 'stream0', 'stream2', 'stream3', 'stream6', 'stream8',
                                   'stream9', 'stream11', 'stream5/12', 'stream9/24',
-                                  'stream3/20', 'stream6/24', 'stream9/20'])
+                                  'stream3/20', 'stream6/24', 'stream9/20', 'stream12',
+                                  'stream13/12', 'stream14/20', 'stream12/6'])
 @pytest.mark.parametrize('model', ['Thermal2D', 'Thermal3DAxi'])
 def test_fp64_thermal_multi_tile_matches_oracle_bitwise(model, tile):
     nx, ny, dx = 190, 150, 1e-3
@@ -373,3 +377,54 @@ def test_fp64_thermal_fused_remainder_launches_match_reference(slot):
     res = run_device(scene, tile=idx, graph_chunk=9)
     for k in ref:
         np.testing.assert_array_equal(res[k], ref[k], err_msg='%s %d' % (k, idx))
+
+
+# LDS-ring shapes (12-14): rows reach LDS by direct 16-byte loads, so rows must start on
+# 16-byte boundaries; FP32 forms (4-byte elements) and tiles reading outside the arrays.
+@pytest.mark.parametrize('form,nx', [('fp64', 201), ('fp32', 202), ('fp32_kahan', 206)])
+def test_ring_shapes_need_aligned_rows(form, nx):
+    scene = _big_acoustic(nx, 40, 12)
+    for slot in (12, 13, 14):
+        with pytest.raises(ValueError):
+            run_device(scene, form=form, tile=_stream_indices()[slot])
+
+
+@pytest.mark.parametrize('form', ['fp32', 'fp32_kahan'])
+@pytest.mark.parametrize('model,nx,ny', [('Acoustic2D', 212, 97), ('Acoustic3DAxi', 132, 61),
+                                         ('Acoustic2D', 64, 48)])
+def test_fp32_ring_shapes_equal_tile_engine(model, nx, ny, form):
+    scene = dict(_big_acoustic(nx, ny, 120, model=model),
+                 plan=[('simulate', 61), ('simulate', 59)])
+    a = run_device(scene, form=form, tile=1)
+    for slot, strip in ((12, 0), (13, 0), (14, 0), (12, 12), (14, 20)):
+        print('ring', model, nx, ny, form, slot, strip, flush=True)
+        b = run_device(scene, form=form, tile=_stream_indices()[slot], strip_height=strip)
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k], err_msg='%s %d %d %s' % (form, slot,
+                                                                                strip, k))
+
+
+@pytest.mark.parametrize('form', ['fp32', 'fp32_kahan'])
+@pytest.mark.parametrize('model', ['Thermal2D', 'Thermal3DAxi'])
+def test_fp32_thermal_ring_shapes_equal_tile_engine(model, form):
+    nx, ny, dx = 196, 150, 1e-3
+    rng = np.random.default_rng(9)
+    scene = dict(
+        name='tring', model=model,
+        kwargs=dict(x_samples=nx, x_delta=dx, y_samples=ny, y_delta=dx, t_samples=150,
+                    t_delta=0.2 * dx ** 2 * 4e6 / 50.0),
+        material=('ThermalMaterial', dict(heat_capacity=4000.0, density=1000.0,
+                                          thermal_conductivity=0.6)),
+        regions=[(scenes._thermal_blocks(nx, ny, 16, 7),
+                  ('ThermalMaterial', dict(heat_capacity=4000.0, density=1000.0,
+                                           thermal_conductivity=(50.0, 30.0))))],
+        init={'temperature': rng.random(nx * ny)},
+        boundaries={'temperature': [dict(indices=list(range(0, nx * ny, nx)), value=0.0,
+                                         additive=False)]},
+        outputs={'temperature': [[int(v) for v in rng.choice(nx * ny, 8, replace=False)]]},
+        plan=[('simulate', 77), ('simulate', 73)])
+    a = run_device(scene, form=form, tile=1)
+    for slot in (12, 13, 14):
+        b = run_device(scene, form=form, tile=_stream_indices()[slot])
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k], err_msg='%s %d %s' % (form, slot, k))

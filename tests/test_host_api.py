@@ -41,16 +41,19 @@ def test_tile_table():
     lib = _native.load()
     assert lib.fds_tile_engine(len(shapes)) < 0 and lib.fds_tile_engine(-1) < 0
     # the streaming table the GPU tests pick slots from: (tx, ty, steps per launch, nt,
-    # prefetch depth)
-    stream = [(d['tx'], d['ty'], d['steps'], d['nontemporal'], d['prefetch'])
+    # prefetch depth, LDS ring)
+    stream = [(d['tx'], d['ty'], d['steps'], d['nontemporal'], d['prefetch'], d['lds_ring'])
               for d in _native.tile_table() if d['engine'] == 'stream']
-    assert stream == [(256, 16, 1, False, 1), (256, 16, 2, False, 1), (128, 32, 2, False, 1),
-                      (256, 32, 3, False, 1), (256, 16, 1, True, 1), (256, 16, 2, True, 1),
-                      (256, 32, 3, True, 1), (64, 16, 1, False, 1), (64, 16, 2, False, 1),
-                      (64, 32, 3, False, 1), (64, 16, 1, True, 1), (64, 16, 2, True, 1),
-                      (64, 32, 3, False, 2), (64, 16, 2, False, 2), (256, 32, 3, True, 2)]
+    F, T = False, True
+    assert stream == [(256, 16, 1, F, 1, F), (256, 16, 2, F, 1, F), (128, 32, 2, F, 1, F),
+                      (256, 32, 3, F, 1, F), (256, 16, 1, T, 1, F), (256, 16, 2, T, 1, F),
+                      (256, 32, 3, T, 1, F), (64, 16, 1, F, 1, F), (64, 16, 2, F, 1, F),
+                      (64, 32, 3, F, 1, F), (64, 16, 1, T, 1, F), (64, 16, 2, T, 1, F),
+                      (64, 32, 3, F, 2, T), (64, 16, 2, F, 2, T), (64, 32, 3, F, 1, T)]
+    assert [d['single_only'] for d in _native.tile_table() if d['engine'] == 'stream'] == \
+        [F] * 12 + [T] * 3
     assert all(d['steps'] == 1 and not d['nontemporal'] and d['prefetch'] == 0
-               for d in _native.tile_table() if d['engine'] == 'tile')
+               and not d['lds_ring'] for d in _native.tile_table() if d['engine'] == 'tile')
 
 
 def test_simulation_without_gpu_fails_loudly():
