@@ -382,10 +382,22 @@ def time_batched(ns, form, steps, warmup, chunk=50, fixed=None):
     cells = batch.cells
     kernel_s = batch.last_run_ms() / 1e3 / steps
     achieved = BYTES_PER_CELL[form] * cells / kernel_s / 1e9
+    batch.close()
+    # A/B of the launch schedules on the same shape (the fields run on; timings only):
+    # one workgroup per tile ('dispatch') against the persistent work queue ('queue')
+    sched = {'dispatch': round(kernel_s * 1e6, 2)}
+    ab_steps = max(3, min(steps, 200))
+    try:
+        qb = BatchRunner(fields, tile=tile, chunk=chunk, schedule='queue')
+        qb.simulate(5)
+        qb.simulate(ab_steps)
+        sched['queue'] = round(qb.last_run_ms() * 1e3 / ab_steps, 2)
+        qb.close()
+    except ValueError as e:          # a shape without work-queue kernels
+        sched['queue'] = 'n/a: %s' % e
     res = dict(gcells=cells * steps / wall / 1e9, ms_per_step=wall / steps * 1e3,
                kernel_us=kernel_s * 1e6, achieved_gbs=achieved, cells=cells, tile=tile,
-               tune_us_per_step=table)
-    batch.close()
+               tune_us_per_step=table, schedule_us_per_step=sched)
     return res, fields
 
 
@@ -737,9 +749,40 @@ def cpu_host():
                 omp_num_threads=os.environ.get('OMP_NUM_THREADS'))
 
 
-def cpu_baseline(wl, steps):
+def _pinned_worker(job):
+    """Pool worker (spawned with OMP_NUM_THREADS=1): pin this process to one CPU, run the
+    headline workload on the oracle; (cell updates, seconds)."""
+    cpu, wl, steps = job
+    if cpu is not None and hasattr(os, 'sched_setaffinity'):
+        os.sched_setaffinity(0, {cpu})
+    return cpu_baseline(wl, steps, rate=False)
+
+
+def cpu_baseline_pinned(wl, steps, cpus):
+    """The oracle on the headline workload in len(cpus) processes at once, each pinned to one
+    CPU of ``cpus`` with OMP_NUM_THREADS (and the BLAS thread counts) at 1, each on its own
+    copy of the workload (the reference has no intra-run parallelism; independent runs are how
+    it uses more cores): total cell updates / the slowest process's time, Gcell-updates/s."""
+    import multiprocessing as mp
+    keys = ('OMP_NUM_THREADS', 'OPENBLAS_NUM_THREADS', 'MKL_NUM_THREADS')
+    old = {k: os.environ.get(k) for k in keys}
+    os.environ.update({k: '1' for k in keys})
+    try:
+        with mp.get_context('spawn').Pool(len(cpus)) as pool:
+            res = pool.map(_pinned_worker, [(c, wl, steps) for c in cpus])
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return sum(c for c, _ in res) / max(t for _, t in res) / 1e9
+
+
+def cpu_baseline(wl, steps, rate=True):
     """The oracle (plain FP64 method, scipy dia_matrix products as the reference uses) on the
-    same workload, single core."""
+    same workload, in this process (one thread: numpy's element-wise operations and scipy's
+    dia_matvec do not spawn threads).  Gcell-updates/s, or (cell updates, seconds)."""
     import oracle
     oracle.Dia.use_scipy = True
     n = wl['nx'] * wl['ny']
@@ -758,7 +801,7 @@ def cpu_baseline(wl, steps):
     t0 = time.perf_counter()
     fld.simulate(steps)
     wall = time.perf_counter() - t0
-    return n * steps / wall / 1e9
+    return n * steps / wall / 1e9 if rate else (n * steps, wall)
 
 
 def main():
@@ -832,13 +875,24 @@ def main():
                       'advances steps_per_launch fused steps in one pass, so the per-step '
                       'algorithmic rate (algorithmic_frac, 64 B per cell update) may pass 1 '
                       'while the physical one cannot')
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu:
-        cpu_val = cpu_baseline(wl, args.cpu_steps)
+        host = cpu_host()
+        cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else [None]
+        cpu_val = cpu_baseline_pinned(wl, args.cpu_steps, cpus[:1])
+        host.update(threads=1, omp_num_threads='1 (set for the baseline processes)',
+                    pinned_to=cpus[:1])
         cpu = dict(value=round(cpu_val, 5), unit='Gcell-updates/s', cores=1, kind='port',
                    sample='oracle (plain FP64 method with scipy dia_matrix products, as the '
                           'reference), Acoustic2D 2048x2048, same scene, %d steps (per-step '
-                          'cost is constant)' % args.cpu_steps, host=cpu_host())
+                          'cost is constant), one process pinned to one CPU, OMP_NUM_THREADS=1'
+                          % args.cpu_steps, host=host)
+        workers = cpus[:16]
+        cpu_all = dict(value=round(cpu_baseline_pinned(wl, args.cpu_steps, workers), 5),
+                       unit='Gcell-updates/s', cores=len(workers), kind='port',
+                       sample='the same oracle run in %d processes at once, one pinned to each '
+                              'CPU, each on its own copy of the workload (%d steps); total cell '
+                              'updates / slowest process' % (len(workers), args.cpu_steps))
     line = {
         'metric': METRIC,
         'value': round(ref['gcells'], 3),
@@ -858,6 +912,7 @@ def main():
         'roofline': roof,
         'roofline_detail': detail,
         'cpu_baseline': cpu,
+        'cpu_baseline_all_cores': cpu_all,
         'forms': out_forms,
     }
     sections = set(args.sections.split(','))
@@ -883,6 +938,7 @@ def main():
                           roofline_frac=round(r['achieved_gbs'] / HBM_PEAK_GBS, 4),
                           bytes_per_cell=BYTES_PER_CELL[f], cells=r['cells'],
                           tile=r['tile'], autotune_us_per_step=r['tune_us_per_step'],
+                          schedule_kernel_us_per_step=r['schedule_us_per_step'],
                           rel_l2_vs_fp64={k: float('%.3g' % v)
                                           for k, v in batched_error(flds, ref).items()})
             for fl in flds:

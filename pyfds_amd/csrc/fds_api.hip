@@ -205,6 +205,7 @@ struct fds_ctx {
   size_t elem = 8;
   // graphs: key (chunk, parity) -> exec
   std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+  int graph_chunk = 0;              // steps of a full graph chunk (other graphs: run tails)
   uint64_t generation = 0, graph_generation = ~0ull;
   std::vector<hipGraph_t> graph_objs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;   // bracket the launches of the last run
@@ -1433,15 +1434,38 @@ int couple_run(fds_ctx *ac, fds_ctx *th, const double *rho, const double *loss, 
 // Capture the chunk graphs a run of nsteps steps replays (at most two: one per buffer parity
 // it starts a chunk on).  Capturing is set-up: it waits for the stream to be idle, then builds
 // the graph on the host.
+// Graph of the whole launches before the last one of a run's tail of `rest` steps (at most one
+// chunk): rest - 1 rounded down to whole launches, when that is at least two launches.  0: the
+// tail runs as plain launches (also once kMaxShortGraphs tail lengths hold graphs).
+constexpr int kMaxShortGraphs = 8;
+
+int short_graph_steps(const fds_ctx *c, int64_t rest, int parity) {
+  const int g = step_group(c);
+  const int64_t len = (rest - 1) / g * g;
+  if (len < 2 * g) return 0;
+  if (c->graphs.count(std::make_pair((int)len, parity))) return (int)len;
+  int shorts = 0;
+  for (const auto &kv : c->graphs) shorts += kv.first.first != c->graph_chunk;
+  return shorts < 2 * kMaxShortGraphs ? (int)len : 0;
+}
+
 int capture_run_graphs(fds_ctx *c, int64_t nsteps, int chunk) {
   if (c->line || chunk <= 0) return 0;
   const int g = step_group(c);
   const int ch = std::max(g, chunk - chunk % g);
+  c->graph_chunk = ch;
   int parity = c->parity;
   for (int64_t left = nsteps, n = 0; left > ch && n < 2; left -= ch, ++n) {
     hipGraphExec_t ex;
     if (int rc = get_graph(c, ch, parity, &ex)) return rc;
     parity ^= ((ch / g) & 1);
+  }
+  // the tail after the full chunks (a run shorter than a chunk is all tail)
+  const int64_t tail = (nsteps - 1) % ch + 1, full = (nsteps - tail) / ch;
+  parity = c->parity ^ (int)((full * (ch / g)) & 1);
+  if (const int len = short_graph_steps(c, tail, parity)) {
+    hipGraphExec_t ex;
+    if (int rc = get_graph(c, len, parity, &ex)) return rc;
   }
   return 0;
 }
@@ -1489,6 +1513,15 @@ int enqueue_steps(fds_ctx *c, int64_t nsteps, int chunk, hipEvent_t e0, hipEvent
       c->parity ^= ((chunk / g) & 1);
       done += chunk;
       graph_done += chunk;
+    }
+    // the tail's whole launches but the last as one graph too (one launch call, not several)
+    if (const int len = short_graph_steps(c, nsteps - done, c->parity)) {
+      hipGraphExec_t ex;
+      if (int rc = get_graph(c, len, c->parity, &ex)) return rc;
+      HIP_TRY(hipGraphLaunch(ex, c->stream));
+      c->parity ^= ((len / g) & 1);
+      done += len;
+      graph_done += len;
     }
   }
   // the rest as plain launches of g stages, then one launch of the remaining stages (k counts

@@ -424,12 +424,7 @@ class DeviceRunner:
                 continue
             if data[k] is not None:
                 for out, s0, cnt in self.out_layout[k]:
-                    cols = data[k][:, s0:s0 + cnt].T
-                    if not out.signals:
-                        out.signals = [list(col) for col in cols]
-                    else:
-                        for sig, col in zip(out.signals, cols):
-                            sig.extend(col)
+                    out._record(data[k][:, s0:s0 + cnt])
             comp._device = (self, k)
             comp._host_dirty = False
 

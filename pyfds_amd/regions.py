@@ -88,11 +88,41 @@ class Boundary:
 
 class Output:
     """Points recorded every step (``pyfds/regions.py:148-165``); ``signals`` holds one list
-    per point."""
+    per point.
+
+    Device runs hand over their samples as one (steps x points) block; the blocks are turned
+    into the per-point lists when ``signals`` is next read (building the lists costs about as
+    much as a short run).  A list taken from ``signals`` before a run therefore does not grow
+    during the run: read ``signals`` again afterwards."""
 
     def __init__(self, region):
         self.region = region
-        self.signals = []
+        self._signals = []
+        self._blocks = []
+
+    @property
+    def signals(self):
+        if self._blocks:
+            self._merge()
+        return self._signals
+
+    @signals.setter
+    def signals(self, value):
+        self._blocks = []
+        self._signals = value
+
+    def _record(self, block):
+        """Queue the samples of a device run (steps x points, float64, owned by this output)."""
+        self._blocks.append(block)
+
+    def _merge(self):
+        blocks, self._blocks = self._blocks, []
+        cols = (blocks[0] if len(blocks) == 1 else np.concatenate(blocks, axis=0)).T
+        if not self._signals:
+            self._signals = [list(col) for col in cols]
+        else:
+            for sig, col in zip(self._signals, cols):
+                sig.extend(col)
 
     @property
     def mean_signal(self):

@@ -365,3 +365,28 @@ def test_pipeline_sizes():
     assert device.pipeline_sizes(2000, 2000, 50, 64, 3) == [288] * 6 + [272]
     assert device.pipeline_sizes(210, 210, 50, 64, 3) == [48] * 4 + [18]
     assert device.pipeline_sizes(150, 150, 0, 4, 2) == [20] * 7 + [10]
+
+
+def test_output_signals_merge_device_blocks_in_order():
+    """Device runs queue (steps x points) blocks; reading ``signals`` turns them into the
+    per-point lists the reference builds step by step (pyfds/fields.py:602-611), in order with
+    host-appended samples, as numpy float64 scalars."""
+    from pyfds_amd import regions as reg
+    out = reg.Output(reg.Region([3, 5], 'o'))
+    a = np.arange(6.0).reshape(3, 2)            # 3 steps, 2 points
+    b = 10 + np.arange(4.0).reshape(2, 2)
+    out._record(a)
+    out._record(b)
+    assert out.signals == [[0.0, 2.0, 4.0, 10.0, 12.0], [1.0, 3.0, 5.0, 11.0, 13.0]]
+    assert all(type(v) is np.float64 for s in out.signals for v in s)
+    out.signals[0].append(99.0)                  # host step, as write_outputs does
+    out.signals[1].append(98.0)
+    out._record(np.array([[7.0, 8.0]]))
+    assert out.signals == [[0.0, 2.0, 4.0, 10.0, 12.0, 99.0, 7.0],
+                           [1.0, 3.0, 5.0, 11.0, 13.0, 98.0, 8.0]]
+    np.testing.assert_array_equal(out.mean_signal, np.mean(np.asarray(out.signals), axis=0))
+    out._record(np.array([[1.0, 1.0]]))
+    out.signals = []                             # assignment drops queued blocks
+    assert out.signals == []
+    out._record(np.array([[2.0, 3.0]]))
+    assert out.signals == [[2.0], [3.0]]
