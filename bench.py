@@ -110,9 +110,9 @@ def time_lossy(ns, wl, form, steps, warmup, fixed=None):
     fld = make_lossy(ns, build_field(ns, wl, form, t_samples=steps + warmup + 50))
     fld.assemble_matrices()
     tune = tuned(fld, fixed)
-    fld.simulate(warmup)
     runner = fld._runner()
     runner.prepare(fld, steps)      # one-time set-up (chunk graphs) outside the timed steps
+    fld.simulate(warmup)            # then the warm-up, right before the timed steps
     t0 = time.perf_counter()
     fld.simulate(steps)
     wall = time.perf_counter() - t0
@@ -292,10 +292,10 @@ def time_form(ns, wl, form, steps, warmup, fixed=None):
     fld = build_field(ns, wl, form, t_samples=steps + warmup + 200)
     fld.assemble_matrices()
     tune = tuned(fld, fixed)
-    if warmup:
-        fld.simulate(warmup)
     runner = fld._runner()
     runner.prepare(fld, steps)      # one-time set-up (chunk graphs) outside the timed steps
+    if warmup:                      # then the warm-up, right before the timed steps
+        fld.simulate(warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     fld.simulate(steps)
@@ -362,7 +362,8 @@ def time_batched(ns, form, steps, warmup, chunk=50, fixed=None):
     import torch
     from pyfds_amd.batch import BatchRunner
     from pyfds_amd import autotune
-    fields = batched_fields(ns, form, steps=steps + warmup + 64)
+    # signals long enough for the timed run and the schedule A/B after it
+    fields = batched_fields(ns, form, steps=steps + warmup + 64 + 210)
     table = None
     if fixed is None:
         tile, chunk, table = autotune.tune_batch(fields, steps=20, chunks=(chunk,),
@@ -450,10 +451,10 @@ def time_thermal(ns, form, steps, warmup, fixed=None):
     fld = thermal_field(ns, form, steps + warmup + 200)
     fld.assemble_matrices()
     tune = tuned(fld, fixed)
-    if warmup:
-        fld.simulate(warmup)
     runner = fld._runner()
     runner.prepare(fld, steps)      # one-time set-up (chunk graphs) outside the timed steps
+    if warmup:                      # then the warm-up, right before the timed steps
+        fld.simulate(warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     fld.simulate(steps)

@@ -1449,8 +1449,16 @@ int short_graph_steps(const fds_ctx *c, int64_t rest, int parity) {
   return shorts < 2 * kMaxShortGraphs ? (int)len : 0;
 }
 
-int capture_run_graphs(fds_ctx *c, int64_t nsteps, int chunk) {
+// both_parities: also the graphs a run starting from the other state buffer replays (runs
+// prepared ahead of other runs of odd launch counts).
+int capture_run_graphs(fds_ctx *c, int64_t nsteps, int chunk, bool both_parities = false) {
   if (c->line || chunk <= 0) return 0;
+  if (both_parities) {
+    c->parity ^= 1;
+    const int rc = capture_run_graphs(c, nsteps, chunk);
+    c->parity ^= 1;
+    if (rc) return rc;
+  }
   const int g = step_group(c);
   const int ch = std::max(g, chunk - chunk % g);
   c->graph_chunk = ch;
@@ -1654,13 +1662,15 @@ int fds_prepare_run(fds_ctx *c, int64_t start, int64_t nsteps, int chunk) {
   if (!c->committed) return fail(FDS_E_STATE, "coefficients not committed");
   if (nsteps < 1 || start < 0) return fail(FDS_E_ARG, "a run needs steps");
   if (int rc = prepare_run(c, start, nsteps)) return rc;
-  if (int rc = capture_run_graphs(c, nsteps, chunk)) return rc;
-  // pinned staging of asynchronous runs of this length (both slots)
-  for (int slot = 0; slot < kMaxFlights; ++slot)
+  if (int rc = capture_run_graphs(c, nsteps, chunk, true)) return rc;
+  // pinned staging and events of asynchronous runs of this length (both slots)
+  for (int slot = 0; slot < kMaxFlights; ++slot) {
+    if (int rc = ensure_events(c->fev[slot])) return rc;
     for (int k = 0; k < 3; ++k) {
       const size_t bytes = c->comp[k].out_cells.size() * (size_t)nsteps * 8;
       if (bytes) HIP_TRY(c->comp[k].stage[slot].ensure(bytes));
     }
+  }
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
 }

@@ -294,10 +294,11 @@ class LearnedBoundary:
         return fld
 
     # ------------------------------------------------------------------ judging
-    def evaluate(self, scene, after=1200, chunk=25, window_margin=0):
+    def evaluate(self, scene, after=1200, chunk=25, window_margin=0, timing=True):
         """Run the enlarged plain grid, the learned grid and the same grid without the net in
         lock step; reflected energy of both small grids over the interior from the wave's
-        arrival at the layer to ``after`` steps later; wall-clock of both methods."""
+        arrival at the layer to ``after`` steps later; wall-clock of both methods (unless
+        ``timing`` is False)."""
         t_hit = self.wave_arrival(scene, window_margin)
         steps = t_hit + after
         big = self.field(scene, self.big, self.off, steps)
@@ -324,7 +325,8 @@ class LearnedBoundary:
         res['finite'] = bool(np.isfinite(np.asarray(lrn.pressure.values)).all())
         for f in (big, lrn, abl):
             f._runner().close()
-        res.update(self.time_methods(scene, steps))
+        if timing:
+            res.update(self.time_methods(scene, steps))
         return res
 
     def time_methods(self, scene, steps):

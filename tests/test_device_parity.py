@@ -428,3 +428,27 @@ def test_fp32_thermal_ring_shapes_equal_tile_engine(model, form):
         b = run_device(scene, form=form, tile=_stream_indices()[slot])
         for k in a:
             np.testing.assert_array_equal(a[k], b[k], err_msg='%s %d %s' % (form, slot, k))
+
+
+def test_values_array_held_across_runs_is_updated_in_place():
+    """The reference updates ``values`` in place (``values -= ...``): an array taken from
+    ``.values`` before a run is the array ``.values`` returns afterwards, holding the new state,
+    and edits made to it in place are honoured by the next run, as an assignment would be."""
+    scene = dict(scenes.SCENES['a2d_main'](), plan=[('simulate', 10)])
+    ref = scenes.run_oracle(scene)
+    f = scenes.build_field(pyfds_amd, pyfds_amd.regions, scene)
+    v = f.pressure.values
+    f.simulate(10)
+    w = f.pressure.values
+    assert w is v
+    np.testing.assert_array_equal(v, ref['values/pressure'])
+    v[7] += 0.25                                   # in-place edit of the held array
+    f.simulate(10)
+    g = scenes.build_field(pyfds_amd, pyfds_amd.regions, scene)
+    g.simulate(10)
+    p = np.array(g.pressure.values)
+    p[7] += 0.25
+    g.pressure.values = p                          # the same edit by assignment
+    g.simulate(10)
+    np.testing.assert_array_equal(f.pressure.values, g.pressure.values)
+    np.testing.assert_array_equal(f.velocity_x.values, g.velocity_x.values)
