@@ -384,6 +384,9 @@ def time_batched(ns, form, steps, warmup, chunk=50, fixed=None):
     kernel_s = batch.last_run_ms() / 1e3 / steps
     achieved = BYTES_PER_CELL[form] * cells / kernel_s / 1e9
     batch.close()
+    # every member's pressure after the warm-up and the timed steps, before the A/B below
+    # advances the same fields further
+    values = [np.array(f.pressure.values) for f in fields]
     # A/B of the launch schedules on the same shape (the fields run on; timings only):
     # one workgroup per tile ('dispatch') against the persistent work queue ('queue')
     sched = {'dispatch': round(kernel_s * 1e6, 2)}
@@ -399,7 +402,7 @@ def time_batched(ns, form, steps, warmup, chunk=50, fixed=None):
     res = dict(gcells=cells * steps / wall / 1e9, ms_per_step=wall / steps * 1e3,
                kernel_us=kernel_s * 1e6, achieved_gbs=achieved, cells=cells, tile=tile,
                tune_us_per_step=table, schedule_us_per_step=sched)
-    return res, fields
+    return res, fields, values
 
 
 def batched_reference(ns, steps):
@@ -416,10 +419,10 @@ def batched_reference(ns, steps):
     return out
 
 
-def batched_error(fields, ref):
-    """Relative L2 error of each member's pressure against the FP64 reference: the largest,
-    the median and the one over all members' cells together."""
-    num = np.array([np.linalg.norm(np.asarray(f.pressure.values) - r) for f, r in zip(fields, ref)])
+def batched_error(values, ref):
+    """Relative L2 error of each member's pressure (``values``, one array per member) against
+    the FP64 reference: the largest, the median and the one over all members' cells together."""
+    num = np.array([np.linalg.norm(v - r) for v, r in zip(values, ref)])
     den = np.array([np.linalg.norm(r) for r in ref])
     per = num / np.where(den > 0, den, 1.0)
     return dict(max=float(per.max()), median=float(np.median(per)),
@@ -930,7 +933,7 @@ def main():
         # every member after the warm-up and the timed steps, FP64 (the oracle's bits)
         ref = batched_reference(ns, args.batch_steps + 10)
         for f in [x for x in args.batch_forms.split(',') if x]:
-            r, flds = time_batched(ns, f, args.batch_steps, 10,
+            r, flds, vals = time_batched(ns, f, args.batch_steps, 10,
                                    fixed=None if args.batch_tile is None
                                    else (args.batch_tile, args.batch_strip))
             bat[f] = dict(value=round(r['gcells'], 3), ms_per_step=round(r['ms_per_step'], 4),
@@ -941,7 +944,7 @@ def main():
                           tile=r['tile'], autotune_us_per_step=r['tune_us_per_step'],
                           schedule_kernel_us_per_step=r['schedule_us_per_step'],
                           rel_l2_vs_fp64={k: float('%.3g' % v)
-                                          for k, v in batched_error(flds, ref).items()})
+                                          for k, v in batched_error(vals, ref).items()})
             for fl in flds:
                 fl._runner().close()
         del ref

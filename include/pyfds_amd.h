@@ -173,8 +173,9 @@ int fds_tile_shape(int tile_index, int *tx, int *ty);
 int fds_tile_engine(int tile_index);
 /* Time steps per launch, cache policy (1: non-temporal state traffic), prefetch depth (input
  * rows a streaming workgroup loads ahead; 0 for LDS-tile shapes) and load path (lds_ring 1:
- * the rows go straight from HBM to LDS rings, no registers in flight; such shapes serve single
- * lossless fields with nx a multiple of 16 bytes / element size) of a shape. */
+ * the rows go straight from HBM to LDS rings, no registers in flight; a field whose rows do not
+ * start on 16-byte boundaries runs the same tiling with register loads) of a shape.  Shapes
+ * with a prefetch depth above 1 serve single lossless fields only. */
 int fds_tile_traits(int tile_index, int *steps, int *nontemporal, int *prefetch, int *lds_ring);
 /* Strip height (rows per workgroup) of a streaming shape; 0 restores the shape's own height.
  * Ignored by LDS-tile shapes.  For the resident 1D engine the value is the number of cells per

@@ -214,7 +214,8 @@ def tile_shapes():
 def tile_table():
     """Every shape of the step kernels: dicts with index, engine, tx, ty, steps (per launch),
     nontemporal, prefetch (rows loaded ahead by a streaming workgroup) and lds_ring (those
-    rows go straight to LDS); ``single_only``: the shape serves single lossless fields."""
+    rows go straight to LDS); ``single_only``: the shape serves single lossless fields (no
+    batches, no lossy fields)."""
     lib = load()
     out = []
     for i, ((tx, ty), eng) in enumerate(zip(tile_shapes(), tile_engines())):
@@ -222,7 +223,7 @@ def tile_table():
         check(lib.fds_tile_traits(i, ct.byref(st), ct.byref(nt), ct.byref(pf), ct.byref(rg)))
         out.append(dict(index=i, engine=eng, tx=tx, ty=ty, steps=st.value,
                         nontemporal=bool(nt.value), prefetch=pf.value, lds_ring=bool(rg.value),
-                        single_only=pf.value > 1 or bool(rg.value)))
+                        single_only=pf.value > 1))
     return out
 
 

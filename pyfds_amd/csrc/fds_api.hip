@@ -452,17 +452,11 @@ int choose_tiles(fds_ctx *c) {
     if (t >= fds::kNumTiles && !can_stream(c))
       t = (c->ny == 1) ? fds::kTileRow : fds::kDefaultTile2D;
     // lossy acoustics streams one step per launch with its own row ring: shapes with deeper
-    // prefetch or the LDS ring have no lossy kernels
+    // prefetch have no lossy kernels
     if (t >= fds::kNumTiles && c->lossy && c->family == kAcoustic &&
         fds::stream_single_only(t - fds::kNumTiles))
-      return fail(FDS_E_ARG, "shape %d (prefetch %d rows%s) serves lossless fields only", t,
-                  fds::kStreamShapes[t - fds::kNumTiles][4],
-                  fds::kStreamShapes[t - fds::kNumTiles][5] ? ", LDS ring" : "");
-    // the LDS ring loads 16-byte chunks of rows: rows must start on 16-byte boundaries
-    if (t >= fds::kNumTiles && fds::kStreamShapes[t - fds::kNumTiles][5] &&
-        c->nx % (16 / (int)c->elem) != 0)
-      return fail(FDS_E_ARG, "shape %d (LDS ring) needs nx a multiple of %d (nx = %lld)", t,
-                  16 / (int)c->elem, (long long)c->nx);
+      return fail(FDS_E_ARG, "shape %d (prefetch %d rows) serves lossless fields only", t,
+                  fds::kStreamShapes[t - fds::kNumTiles][4]);
     c->shape = t;
     c->streaming = t >= fds::kNumTiles;
     shape_of(t, &c->tx, &c->ty);
@@ -655,15 +649,19 @@ void fill_args(const fds_ctx *c, fds::StepArgs<T> &a, int parity) {
   a.ghost = c->ghost;
 }
 
-fds::LaunchFn pick_launch(const fds_ctx *c, int stages) {
+// ring_ok: the arrays suit an LDS-ring shape's 16-byte row loads (else its register-path twin
+// runs, same bits).
+fds::LaunchFn pick_launch(const fds_ctx *c, int stages, bool ring_ok = true) {
   const int t = c->shape;
   if (c->streaming && c->lossy) {
     // variant: axial-symmetric, isotropic (B, K and M y slots alias the x ones), plain
     const int v = c->axi ? 1 : (c->iso_b && c->iso_k && !c->coef_set[FDS_C_MY]) ? 2 : 0;
-    return c->ks->stream_lossy[t - fds::kNumTiles][v];
+    const int s0 = t - fds::kNumTiles;
+    return c->ks->stream_lossy[ring_ok ? s0 : fds::ring_fallback(s0)][v];
   }
   if (c->streaming) {
-    const int s = t - fds::kNumTiles, f = stages - 1;
+    const int s0 = t - fds::kNumTiles, f = stages - 1;
+    const int s = ring_ok ? s0 : fds::ring_fallback(s0);
     // variant: axial-symmetric, isotropic (y coefficient slots alias the x ones), plain
     const int v = c->axi ? 1 : (c->iso_b && c->iso_k) ? 2 : 0;
     return c->family == kThermal ? c->ks->stream_thermal[s][v][f]
@@ -787,18 +785,18 @@ int launch_step(fds_ctx *c, int parity, int k, bool store, int stages) {
   fill_args<T>(c, a, parity);
   a.k = k;
   if (store) a.flags |= fds::kFlagStore;
-  if (c->streaming && !c->lossy && stages == c->fuse &&
-      fds::kStreamShapes[c->shape - fds::kNumTiles][5]) {
-    // LDS-ring kernels load 16-byte chunks: every array they read must be 16-byte aligned
-    // (whole device allocations are)
-    bool ok = true;
+  bool ring_ok = true;
+  if (c->streaming && fds::kStreamShapes[c->shape - fds::kNumTiles][5]) {
+    // LDS-ring kernels load 16-byte chunks of rows: rows must start on 16-byte boundaries and
+    // every array they read must be 16-byte aligned (whole device allocations are); otherwise
+    // the register-path kernel of the same tiling runs
+    ring_ok = c->nx % (16 / (int)c->elem) == 0;
     for (int k = 0; k < 3; ++k)
-      ok = ok && ((uintptr_t)a.in[k] & 15) == 0 && ((uintptr_t)a.cin[k] & 15) == 0;
-    for (int q = 0; q < FDS_NUM_COEFF; ++q) ok = ok && ((uintptr_t)a.coef[q] & 15) == 0;
-    if (!ok) return fail(FDS_E_ARG, "shape %d (LDS ring) needs 16-byte aligned arrays", c->shape);
+      ring_ok = ring_ok && ((uintptr_t)a.in[k] & 15) == 0 && ((uintptr_t)a.cin[k] & 15) == 0;
+    for (int q = 0; q < FDS_NUM_COEFF; ++q) ring_ok = ring_ok && ((uintptr_t)a.coef[q] & 15) == 0;
   }
   const int nb = fds::xcd_grid(c->ntiles);
-  pick_launch(c, stages)(&a, nb, lds_bytes(c), c->stream);
+  pick_launch(c, stages, ring_ok)(&a, nb, lds_bytes(c), c->stream);
   HIP_TRY(hipGetLastError());
   if (c->flow) {
     hipLaunchKernelGGL(k_flow_shift<T>, dim3((unsigned)c->ny), dim3(256), 0, c->stream,
@@ -2068,9 +2066,8 @@ int fds_batch_create(fds_batch **out, fds_ctx *const *ctxs, int n, int tile) {
   if (t >= fds::kNumTiles && !all_stream) t = fds::kDefaultTile2D;
   if (t >= fds::kNumTiles && fds::stream_single_only(t - fds::kNumTiles)) {
     fds_batch_destroy(b);
-    return fail(FDS_E_ARG, "shape %d (prefetch %d rows%s) serves single fields only", t,
-                fds::kStreamShapes[t - fds::kNumTiles][4],
-                fds::kStreamShapes[t - fds::kNumTiles][5] ? ", LDS ring" : "");
+    return fail(FDS_E_ARG, "shape %d (prefetch %d rows) serves single fields only", t,
+                fds::kStreamShapes[t - fds::kNumTiles][4]);
   }
   std::vector<int32_t> tinst, base;
   size_t lds = 0;

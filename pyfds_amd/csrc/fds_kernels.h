@@ -28,11 +28,25 @@ constexpr int kStreamShapes[kNumStream][6] = {
     {256, 16, 1, 1, 1, 0}, {256, 16, 2, 1, 1, 0}, {256, 32, 3, 1, 1, 0}, {64, 16, 1, 0, 1, 0},
     {64, 16, 2, 0, 1, 0},  {64, 32, 3, 0, 1, 0},  {64, 16, 1, 1, 1, 0},  {64, 16, 2, 1, 1, 0},
     {64, 32, 3, 0, 2, 1},  {64, 16, 2, 0, 2, 1},  {64, 32, 3, 0, 1, 1}};
-// Shapes that serve single lossless fields only (no batch or lossy kernels): deeper prefetch
-// or the LDS ring.
-constexpr bool stream_single_only(int i) {
-  return kStreamShapes[i][4] > 1 || kStreamShapes[i][5] != 0;
+// Shapes that serve single lossless fields only (no batch or lossy kernels): deeper prefetch.
+// (Shapes with the LDS ring and one row of prefetch also run batches, streaming their rows in
+// dword mode, and lossy fields, on the lossy kernel.)
+constexpr bool stream_single_only(int i) { return kStreamShapes[i][4] > 1; }
+// The register-path shape with an LDS-ring shape's geometry (width, height, stages, cache
+// policy; prefetch depth aside): it computes the same bits and runs a single field whose rows
+// do not start on 16-byte boundaries.  Shapes without the ring map to themselves.
+constexpr int ring_fallback(int i) {
+  if (!kStreamShapes[i][5]) return i;
+  int best = i;
+  for (int j = 0; j < kNumStream; ++j)
+    if (!kStreamShapes[j][5] && kStreamShapes[j][0] == kStreamShapes[i][0] &&
+        kStreamShapes[j][1] == kStreamShapes[i][1] && kStreamShapes[j][2] == kStreamShapes[i][2] &&
+        kStreamShapes[j][3] == kStreamShapes[i][3] && kStreamShapes[j][4] == 1)
+      best = j;
+  return best;
 }
+static_assert(ring_fallback(12) == 9 && ring_fallback(13) == 8 && ring_fallback(14) == 9,
+              "every LDS-ring shape needs a register-path shape of its geometry");
 constexpr int kMaxStages = 3;   // largest S in kStreamShapes
 // X-macro over the streaming shape indices (each shape is its own translation unit).
 #define FDS_STREAM_SHAPES(X) \

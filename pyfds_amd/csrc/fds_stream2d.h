@@ -138,18 +138,26 @@ __device__ __forceinline__ void stn(T *p, int rb, int off, T v) {
 // event-list loads of tiles with events only make that wait stricter.  Empty asm statements
 // with a memory clobber keep the compiler from moving memory operations across the wait.
 // ------------------------------------------------------------------------------------------
-template <typename T, int TX>
+// Ring load modes: kRingX4 moves 16-byte chunks of an aligned window (one load per array row
+// for 64-wide tiles; rows must start on 16-byte boundaries), kRingDword one dword per lane of
+// the exact window of columns -1 .. TX (any row start: batches of fields of any width).
+constexpr int kRingX4 = 1, kRingDword = 2;
+
+template <typename T, int TX, int MODE>
 struct RingGeom {
-  static constexpr int E = 16 / (int)sizeof(T);
-  static constexpr int CW = (TX + 2 + E - 1 + E - 1) / E * E;
-  static constexpr int CH = CW / E;
-  static_assert(CH <= TX, "one ring load per array row needs CH <= TX lanes");
+  static constexpr int E = MODE == kRingX4 ? 16 / (int)sizeof(T) : 1;   // window alignment
+  static constexpr int CW = MODE == kRingX4 ? (TX + 2 + E - 1 + E - 1) / E * E
+                                            : (TX + 2 + 1) / 2 * 2;   // elements per array row
+  static constexpr int CH = CW / E;                                   // kRingX4: lanes per row
+  static constexpr int DW = (TX + 2) * (int)sizeof(T) / 4;            // kRingDword: dwords
+  static constexpr int IPA = MODE == kRingX4 ? 1 : (DW + 63) / 64;     // loads per array row
+  static_assert(MODE != kRingX4 || CH <= TX, "one ring load per array row needs CH <= TX");
 };
 
 // elements of the two rings of a strip kernel: NS state and NC coefficient arrays
-template <typename T, int TX, int PF, int S, int NS, int NC>
+template <typename T, int TX, int PF, int S, int NS, int NC, int MODE>
 constexpr int ring_elems() {
-  return ((PF + 1) * NS + (PF + 2 * S - 1) * NC) * RingGeom<T, TX>::CW;
+  return ((PF + 1) * NS + (PF + 2 * S - 1) * NC) * RingGeom<T, TX, MODE>::CW;
 }
 
 template <int K>
@@ -172,21 +180,49 @@ __device__ __forceinline__ void ring_wait(int rows) {
   }
 }
 
-// Ring loads of one row of NQ arrays (aligned flat start fs) to LDS at `dst` (array q at
-// dst + q * CW); every tile clamps its chunks into the array.
-template <typename T, int TX, int NQ>
+// Ring loads of one row of NQ arrays (window start fs: aligned in kRingX4 mode, column -1 in
+// kRingDword mode) to LDS at `dst` (array q at dst + q * CW); every tile clamps its chunks (or
+// elements) into the array.
+template <typename T, int TX, int NQ, int MODE>
 __device__ __forceinline__ void ring_issue(T *dst, const T *const *src, int fs, int n) {
-  using RG = RingGeom<T, TX>;
+  using RG = RingGeom<T, TX, MODE>;
   const int t = threadIdx.x;
-  if (t < RG::CH) {
-    int j = fs + t * RG::E;
-    if (j < 0 || j >= n) j = 0;
+  if constexpr (MODE == kRingX4) {
+    if (t < RG::CH) {
+      int j = fs + t * RG::E;
+      if (j < 0 || j >= n) j = 0;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q)
-      __builtin_amdgcn_global_load_lds(
-          (const void *)(src[q] + j),
-          (__attribute__((address_space(3))) void *)(dst + q * RG::CW), 16, 0, 0);
+      for (int q = 0; q < NQ; ++q)
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(src[q] + j),
+            (__attribute__((address_space(3))) void *)(dst + q * RG::CW), 16, 0, 0);
+    }
+  } else {
+    constexpr int DPE = (int)sizeof(T) / 4;   // dwords per element
+#pragma unroll
+    for (int k = 0; k < RG::IPA; ++k) {
+      const int d = k * 64 + t;               // dword of the window
+      if (d < RG::DW) {
+        int j = fs + d / DPE;
+        if (j < 0 || j >= n) j = 0;
+        const int half = d % DPE;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+          __builtin_amdgcn_global_load_lds(
+              (const void *)(reinterpret_cast<const uint32_t *>(src[q] + j) + half),
+              (__attribute__((address_space(3))) void *)(reinterpret_cast<uint32_t *>(
+                  dst + q * RG::CW) + k * 64), 4, 0, 0);
+      }
+    }
   }
+}
+
+// Slot of row ordinal i + add in a ring of m slots (uniform integer arithmetic; unrolling the
+// row loop far enough to make slots compile-time constants measured no faster: the larger
+// loop body costs what the address arithmetic saves).
+__device__ __forceinline__ int ring_slot(int i, int add, int m) {
+  const int v = i + add;
+  return (v % m + m) % m;
 }
 
 // Whether every read of a streaming tile (rows y0-G-1 .. y0+H+G, columns x0-1 .. x0+TX) falls
@@ -305,7 +341,7 @@ struct AcVel {
 // tile has events; CHK: some reads fall outside the arrays; ISO: the y coefficients are the x
 // ones (isotropic material arrays), so they are neither loaded nor held twice.
 template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool EV, bool CHK, bool ISO,
-          int PF, bool RING>
+          int PF, int RING>
 __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
                                StreamLds<T, TX, S> &L, T *ring) {
   auto &sbp = L.bwd;    // Bx*p of the row, columns -1..TX
@@ -390,13 +426,15 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
     d.ky = ISO ? d.kx : ldr<CHK>(Ky, rb, t, n);
   };
 
-  // LDS rings (RING): state p, vx, vy[, cp, cvx, cvy]; coefficients Bx, Kx[, By, Ky]
-  using RG = RingGeom<T, TX>;
-  constexpr int NS = KAHAN ? 6 : 3, NC = ISO ? 2 : 4, NA = NS + NC;
+  // LDS rings (RING: kRingX4 or kRingDword): state p, vx, vy[, cp, cvx, cvy]; coefficients
+  // Bx, Kx[, By, Ky]
+  constexpr int RMODE = RING ? RING : kRingX4;
+  using RG = RingGeom<T, TX, RMODE>;
+  constexpr int NS = KAHAN ? 6 : 3, NC = ISO ? 2 : 4, NA = (NS + NC) * RG::IPA;
   constexpr int RC = PF + 2 * S - 1;
   constexpr int SP = 0, SVX = 1, SVY = 2, SCP = 3, CBX = 0, CKX = 1, CBY = ISO ? 0 : 2,
                 CKY = ISO ? 1 : 3;
-  const int lead = ((x0 - 1) % RG::E + RG::E) % RG::E;
+  const int lead = RMODE == kRingX4 ? ((x0 - 1) % RG::E + RG::E) % RG::E : 0;
   T *const cring = ring + RS * NS * RG::CW;
   const T *ssrc[NS], *csrc[NC];
   if constexpr (RING) {
@@ -415,11 +453,15 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       csrc[CKY] = Ky;
     }
   }
-  // row rho (ordinal rho + G + 1 in the strip) into its slots of both rings
-  auto issue_row = [&](int rho) __attribute__((always_inline)) {
-    const int r = rho + G + 1, fs = (y0 + rho) * nx + x0 - 1 - lead;
-    ring_issue<T, TX, NS>(ring + (r % RS) * NS * RG::CW, ssrc, fs, n);
-    ring_issue<T, TX, NC>(cring + (r % RC) * NC * RG::CW, csrc, fs, n);
+  // row ordinal r = rho + G + 1 of the strip lives in state slot r % RS and coefficient slot
+  // r % RC (at iteration i, stage s holds row ordinal i - 2s)
+  auto sbase = [&](int ss) __attribute__((always_inline)) { return ring + ss * NS * RG::CW; };
+  auto cbase = [&](int cs) __attribute__((always_inline)) { return cring + cs * NC * RG::CW; };
+  // row rho into state slot ss and coefficient slot cs
+  auto issue_row = [&](int rho, int ss, int cs) __attribute__((always_inline)) {
+    const int fs = (y0 + rho) * nx + x0 - 1 - lead;
+    ring_issue<T, TX, NS, RMODE>(sbase(ss), ssrc, fs, n);
+    ring_issue<T, TX, NC, RMODE>(cbase(cs), csrc, fs, n);
   };
   // value of ring array q (base: the row's slot) at column c of row rho
   auto rd = [&](const T *base, int q, int rho, int c) __attribute__((always_inline)) -> T {
@@ -428,14 +470,9 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
     const int j = (y0 + rho) * nx + x0 + c;
     return (j >= 0 && j < n) ? v : T(0);
   };
-  auto sslot = [&](int rho) __attribute__((always_inline)) {
-    return ring + ((rho + G + 1) % RS) * NS * RG::CW;
-  };
-  auto cslot = [&](int rho) __attribute__((always_inline)) {
-    return cring + ((rho + G + 1) % RC) * NC * RG::CW;
-  };
-  auto read_row = [&](int rho, AcRow<T> &d, AcHalo<T> &h) __attribute__((always_inline)) {
-    const T *sb = sslot(rho), *cb = cslot(rho);
+  auto read_row = [&](int rho, int ss, int cs, AcRow<T> &d, AcHalo<T> &h)
+                      __attribute__((always_inline)) {
+    const T *sb = sbase(ss), *cb = cbase(cs);
     d.p = rd(sb, SP, rho, t);
     d.vx = rd(sb, SVX, rho, t);
     d.vy = rd(sb, SVY, rho, t);
@@ -463,18 +500,18 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
 
 #pragma unroll
   for (int r = 0; r < PF; ++r) {
-    if constexpr (RING) issue_row(-G - 1 + r);
+    if constexpr (RING) issue_row(-G - 1 + r, r % RS, r % RC);
     else load_row(-G - 1 + r, nin[r], nh[r]);
   }
   const int iters = H + 2 * G + S + 1;
-  // RING: slots are computed (LDS addresses), so the loop only unrolls by the parity
+  // RING: slots are computed, so the loop only unrolls by the parity
   run_loop<RING ? 1 : RS>(iters, [&](auto Qc, auto Wc, int i) __attribute__((always_inline)) {
-    constexpr int Q = decltype(Qc)::value, W = RING ? 0 : decltype(Wc)::value;
+    constexpr int Q = decltype(Qc)::value, W = decltype(Wc)::value;
     T bpx[S];
     AcRow<T> row0 = {};      // RING: stage 0 input row of this iteration, read from the rings
     AcHalo<T> h0 = {};
-    AcRow<T> &in0 = RING ? row0 : nin[W];
-    AcHalo<T> &hin = RING ? h0 : nh[W];
+    AcRow<T> &in0 = RING ? row0 : nin[RING ? 0 : W];
+    AcHalo<T> &hin = RING ? h0 : nh[RING ? 0 : W];
     // ---- before the barrier
     static_for<S>([&](auto Sc) __attribute__((always_inline)) {
       constexpr int s = decltype(Sc)::value;
@@ -486,11 +523,12 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       if constexpr (RING) {
         if (s == 0) {
           compiler_fence();
-          if (rho + PF <= R.hi + 1) issue_row(rho + PF);
+          if (rho + PF <= R.hi + 1)
+            issue_row(rho + PF, ring_slot(i, PF, RS), ring_slot(i, PF, RC));
           // loads issued after this row's: the rows rho + 1 .. min(rho + PF, hi + 1)
           ring_wait<NA, PF>(R.hi + 1 - rho);
           compiler_fence();
-          read_row(rho, row0, h0);
+          read_row(rho, ring_slot(i, 0, RS), ring_slot(i, 0, RC), row0, h0);
         }
       } else {
         if (s == 0 && rho + PF <= R.hi + 1)
@@ -504,8 +542,9 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
           cbx = in0.bx;
           cby = in0.by;
         } else {
-          cbx = rd(cslot(rho), CBX, rho, t);
-          cby = ISO ? cbx : rd(cslot(rho), CBY, rho, t);
+          const T *cb = cbase(ring_slot(i, -2 * s, RC));
+          cbx = rd(cb, CBX, rho, t);
+          cby = ISO ? cbx : rd(cb, CBY, rho, t);
         }
       } else {
         cbx = s == 0 ? in0.bx : cf[s][Q].bx;
@@ -547,8 +586,9 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
           ckx = in0.kx;
           cky = in0.ky;
         } else {
-          ckx = rd(cslot(rho), CKX, rho, t);
-          cky = ISO ? ckx : rd(cslot(rho), CKY, rho, t);
+          const T *cb = cbase(ring_slot(i, -2 * s, RC));
+          ckx = rd(cb, CKX, rho, t);
+          cky = ISO ? ckx : rd(cb, CKY, rho, t);
         }
       } else {
         ckx = s == 0 ? in0.kx : cf[s][Q].kx;
@@ -669,7 +709,7 @@ struct ThHalo {        // left thread: T, Q of column -1; right thread: T, Q, G 
 };
 
 template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool EV, bool CHK, bool ISO,
-          int PF, bool RING>
+          int PF, int RING>
 __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, int64_t step, int64_t row,
                               StreamLds<T, TX, S> &L, T *ring) {
   auto &sqt = L.bwd;    // Qx*T of the row
@@ -744,12 +784,13 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
     d.gy = ISO ? d.gx : ldr<CHK>(Gy, rb, t, n);
   };
 
-  // LDS rings (RING): state T[, cT]; coefficients Qx, Gx[, Qy, Gy]
-  using RG = RingGeom<T, TX>;
-  constexpr int NS = KAHAN ? 2 : 1, NC = ISO ? 2 : 4, NA = NS + NC;
+  // LDS rings (RING: kRingX4 or kRingDword): state T[, cT]; coefficients Qx, Gx[, Qy, Gy]
+  constexpr int RMODE = RING ? RING : kRingX4;
+  using RG = RingGeom<T, TX, RMODE>;
+  constexpr int NS = KAHAN ? 2 : 1, NC = ISO ? 2 : 4, NA = (NS + NC) * RG::IPA;
   constexpr int RC = PF + 2 * S - 1;
   constexpr int ST = 0, SCT = 1, CQX = 0, CGX = 1, CQY = ISO ? 0 : 2, CGY = ISO ? 1 : 3;
-  const int lead = ((x0 - 1) % RG::E + RG::E) % RG::E;
+  const int lead = RMODE == kRingX4 ? ((x0 - 1) % RG::E + RG::E) % RG::E : 0;
   T *const cring = ring + RS * NS * RG::CW;
   const T *ssrc[NS], *csrc[NC];
   if constexpr (RING) {
@@ -762,10 +803,12 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
       csrc[CGY] = Gy;
     }
   }
-  auto issue_row = [&](int rho) __attribute__((always_inline)) {
-    const int r = rho + G + 1, fs = (y0 + rho) * nx + x0 - 1 - lead;
-    ring_issue<T, TX, NS>(ring + (r % RS) * NS * RG::CW, ssrc, fs, n);
-    ring_issue<T, TX, NC>(cring + (r % RC) * NC * RG::CW, csrc, fs, n);
+  auto sbase = [&](int ss) __attribute__((always_inline)) { return ring + ss * NS * RG::CW; };
+  auto cbase = [&](int cs) __attribute__((always_inline)) { return cring + cs * NC * RG::CW; };
+  auto issue_row = [&](int rho, int ss, int cs) __attribute__((always_inline)) {
+    const int fs = (y0 + rho) * nx + x0 - 1 - lead;
+    ring_issue<T, TX, NS, RMODE>(sbase(ss), ssrc, fs, n);
+    ring_issue<T, TX, NC, RMODE>(cbase(cs), csrc, fs, n);
   };
   auto rd = [&](const T *base, int q, int rho, int c) __attribute__((always_inline)) -> T {
     const T v = base[q * RG::CW + lead + 1 + c];
@@ -773,14 +816,9 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
     const int j = (y0 + rho) * nx + x0 + c;
     return (j >= 0 && j < n) ? v : T(0);
   };
-  auto sslot = [&](int rho) __attribute__((always_inline)) {
-    return ring + ((rho + G + 1) % RS) * NS * RG::CW;
-  };
-  auto cslot = [&](int rho) __attribute__((always_inline)) {
-    return cring + ((rho + G + 1) % RC) * NC * RG::CW;
-  };
-  auto read_row = [&](int rho, ThRow<T> &d, ThHalo<T> &h) __attribute__((always_inline)) {
-    const T *sb = sslot(rho), *cb = cslot(rho);
+  auto read_row = [&](int rho, int ss, int cs, ThRow<T> &d, ThHalo<T> &h)
+                      __attribute__((always_inline)) {
+    const T *sb = sbase(ss), *cb = cbase(cs);
     d.t = rd(sb, ST, rho, t);
     d.qx = rd(cb, CQX, rho, t);
     d.qy = ISO ? d.qx : rd(cb, CQY, rho, t);
@@ -798,17 +836,18 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
 
 #pragma unroll
   for (int r = 0; r < PF; ++r) {
-    if constexpr (RING) issue_row(-G - 1 + r);
+    if constexpr (RING) issue_row(-G - 1 + r, r % RS, r % RC);
     else load_row(-G - 1 + r, nin[r], nh[r]);
   }
   const int iters = H + 2 * G + S + 1;
+  // RING: slots are computed, so the loop only unrolls by the parity
   run_loop<RING ? 1 : RS>(iters, [&](auto Qc, auto Wc, int i) __attribute__((always_inline)) {
-    constexpr int Q = decltype(Qc)::value, W = RING ? 0 : decltype(Wc)::value;
+    constexpr int Q = decltype(Qc)::value, W = decltype(Wc)::value;
     T qtx[S];
     ThRow<T> row0 = {};      // RING: stage 0 input row of this iteration, read from the rings
     ThHalo<T> h0 = {};
-    ThRow<T> &in0 = RING ? row0 : nin[W];
-    ThHalo<T> &hin = RING ? h0 : nh[W];
+    ThRow<T> &in0 = RING ? row0 : nin[RING ? 0 : W];
+    ThHalo<T> &hin = RING ? h0 : nh[RING ? 0 : W];
     static_for<S>([&](auto Sc) __attribute__((always_inline)) {
       constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
@@ -819,10 +858,11 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
       if constexpr (RING) {
         if (s == 0) {
           compiler_fence();
-          if (rho + PF <= R.hi + 1) issue_row(rho + PF);
+          if (rho + PF <= R.hi + 1)
+            issue_row(rho + PF, ring_slot(i, PF, RS), ring_slot(i, PF, RC));
           ring_wait<NA, PF>(R.hi + 1 - rho);
           compiler_fence();
-          read_row(rho, row0, h0);
+          read_row(rho, ring_slot(i, 0, RS), ring_slot(i, 0, RC), row0, h0);
         }
       } else {
         if (s == 0 && rho + PF <= R.hi + 1)
@@ -836,8 +876,9 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
           cqx = in0.qx;
           cqy = in0.qy;
         } else {
-          cqx = rd(cslot(rho), CQX, rho, t);
-          cqy = ISO ? cqx : rd(cslot(rho), CQY, rho, t);
+          const T *cb = cbase(ring_slot(i, -2 * s, RC));
+          cqx = rd(cb, CQX, rho, t);
+          cqy = ISO ? cqx : rd(cb, CQY, rho, t);
         }
       } else {
         cqx = s == 0 ? in0.qx : cf[s][Q].qx;
@@ -874,8 +915,9 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
           cgx = in0.gx;
           cgy = in0.gy;
         } else {
-          cgx = rd(cslot(rho), CGX, rho, t);
-          cgy = ISO ? cgx : rd(cslot(rho), CGY, rho, t);
+          const T *cb = cbase(ring_slot(i, -2 * s, RC));
+          cgx = rd(cb, CGX, rho, t);
+          cgy = ISO ? cgx : rd(cb, CGY, rho, t);
         }
       } else {
         cgx = s == 0 ? in0.gx : cf[s][Q].gx;
@@ -988,7 +1030,7 @@ __device__ __forceinline__ int strip_path(const StepArgs<T> &a, int tile) {
 
 // ISO kernels serve fields whose y coefficient arrays are the x ones (the host checks).  PF:
 // rows of stage-0 input prefetched ahead (more loads in flight per wave, more registers).
-template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool ISO, int PF, bool RING>
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool ISO, int PF, int RING>
 __device__ __forceinline__ void acoustic_stream(const StepArgs<T> &a, int tile, int64_t step,
                                                 int64_t row, StreamLds<T, TX, S> &L, T *ring) {
   switch (strip_path<T, TX>(a, tile)) {
@@ -997,7 +1039,7 @@ __device__ __forceinline__ void acoustic_stream(const StepArgs<T> &a, int tile, 
   }
 }
 
-template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool ISO, int PF, bool RING>
+template <typename T, bool KAHAN, int TX, bool AXI, int S, bool NT, bool ISO, int PF, int RING>
 __device__ __forceinline__ void thermal_stream(const StepArgs<T> &a, int tile, int64_t step,
                                                int64_t row, StreamLds<T, TX, S> &L, T *ring) {
   switch (strip_path<T, TX>(a, tile)) {
@@ -1008,11 +1050,9 @@ __device__ __forceinline__ void thermal_stream(const StepArgs<T> &a, int tile, i
 
 // One tile of instance `a` in a batched streaming launch (lossless models only): one switch
 // over model variant and path.
-template <typename T, bool KAHAN, int TX, int S, bool NT, int PF>
+template <typename T, bool KAHAN, int TX, int S, bool NT, int PF, int RING>
 __device__ __forceinline__ void any_stream(const StepArgs<T> &a, int tile, int64_t step,
-                                           int64_t row, StreamLds<T, TX, S> &L) {
-  constexpr bool RING = false;
-  T *ring = nullptr;
+                                           int64_t row, StreamLds<T, TX, S> &L, T *ring) {
   const int f = a.flags;
   const int model = ((f & kFlagThermal) ? 2 : 0) | ((f & kFlagAxi) ? 1 : 0);
   switch (__builtin_amdgcn_readfirstlane(model * 4 + strip_path<T, TX>(a, tile))) {

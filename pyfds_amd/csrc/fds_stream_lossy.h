@@ -50,9 +50,29 @@ __device__ __forceinline__ bool lossy_strip_inside(int x0, int y0, int H, int TX
   return lo >= 0 && hi < n;
 }
 
-template <typename T, bool KAHAN, int TX, bool AXI, bool NT, bool EV, bool CHK, bool ISO>
+// LDS rings of the lossy kernel (RING, one-wave workgroups): row ordinal r = rho + 1.  The
+// neighbour ring holds vx, vy and My of rows r-1 .. r+1+PF (PF + 3 slots: the viscous term reads
+// the old velocities of the rows above and below); the row ring holds p, Bx, Kx[, Mx, By, Ky]
+// [, Mc, Mt][, cp, cvx, cvy] of rows r .. r+1+PF (PF + 2 slots).  Every input row arrives by
+// direct global-to-LDS loads PF rows ahead of the row below the one being updated.
+constexpr int kLossyRingPF = 1;
+
+template <bool ISO, bool AXI, bool KAHAN>
+constexpr int lossy_row_arrays() {
+  return 3 + (ISO ? 0 : 3) + (AXI ? 2 : 0) + (KAHAN ? 3 : 0);
+}
+
+template <typename T, int TX, bool ISO, bool AXI, bool KAHAN>
+constexpr int lossy_ring_elems() {
+  return ((kLossyRingPF + 3) * 3 + (kLossyRingPF + 2) * lossy_row_arrays<ISO, AXI, KAHAN>()) *
+         RingGeom<T, TX, kRingX4>::CW;
+}
+
+template <typename T, bool KAHAN, int TX, bool AXI, bool NT, bool EV, bool CHK, bool ISO,
+          int RING = 0>
 __device__ __forceinline__ void lossy_strip(const StepArgs<T> &a, int tile, int64_t step,
-                                            int64_t row, LossyLds<T, TX, AXI> &L) {
+                                            int64_t row, LossyLds<T, TX, AXI> &L,
+                                            T *ring = nullptr) {
   constexpr bool WAVE = TX == 64;
   constexpr int NV = AXI ? 4 : 2;     // viscous products per row: (left, right) x (vx, vy)
   const int H = a.strip_h;
@@ -90,7 +110,7 @@ __device__ __forceinline__ void lossy_strip(const StepArgs<T> &a, int tile, int6
   if constexpr (EV) cur = stage_cursors(a, tile);
 
   constexpr int RS = 3;       // rows rho-1, rho, rho+1 in registers; rho+2 in flight
-  LsRow<T> ring[RS] = {};
+  LsRow<T> ring_regs[RS] = {};
   LsHalo<T> halo[RS] = {};
   T bpy[2] = {}, kvy[2] = {}, kvd[2] = {};
   T pp[2] = {}, cpp[2] = {};  // p of the previous row after its events
@@ -122,19 +142,133 @@ __device__ __forceinline__ void lossy_strip(const StepArgs<T> &a, int tile, int6
     }
   };
 
-  load_row(-1, ring[0], halo[0]);
-  load_row(0, ring[1], halo[1]);
+  // ---- LDS rings (RING)
+  constexpr int PF = kLossyRingPF, RN = PF + 3, RR = PF + 2;
+  using RG = RingGeom<T, TX, kRingX4>;
+  constexpr int NRW = lossy_row_arrays<ISO, AXI, KAHAN>();
+  constexpr int NA = 3 + NRW;                 // ring loads per row
+  constexpr int QP = 0, QBX = 1, QKX = 2, QMX = 3, QBY = 4, QKY = 5;
+  constexpr int QMC = ISO ? 3 : 6, QCP = QMC + (AXI ? 2 : 0);
+  constexpr int NVX = 0, NVY = 1, NMY = 2;    // neighbour ring arrays
+  const int lead = ((x0 - 1) % RG::E + RG::E) % RG::E;
+  T *const rring = ring + RN * 3 * RG::CW;
+  const T *nsrc[3], *rsrc[NRW];
+  if constexpr (RING != 0) {
+    nsrc[NVX] = vxin;
+    nsrc[NVY] = vyin;
+    nsrc[NMY] = ISO ? Mx : My;
+    rsrc[QP] = pin;
+    rsrc[QBX] = Bx;
+    rsrc[QKX] = Kx;
+    if constexpr (!ISO) {
+      rsrc[QMX] = Mx;
+      rsrc[QBY] = By;
+      rsrc[QKY] = Ky;
+    }
+    if constexpr (AXI) {
+      rsrc[QMC] = Mc;
+      rsrc[QMC + 1] = Mt;
+    }
+    if constexpr (KAHAN) {
+      rsrc[QCP] = a.cin[0];
+      rsrc[QCP + 1] = a.cin[1];
+      rsrc[QCP + 2] = a.cin[2];
+    }
+  }
+  auto nbase = [&](int r) __attribute__((always_inline)) {
+    return ring + ring_slot(r, 0, RN) * 3 * RG::CW;
+  };
+  auto rbase = [&](int r) __attribute__((always_inline)) {
+    return rring + ring_slot(r, 0, RR) * NRW * RG::CW;
+  };
+  auto issue = [&](int r) __attribute__((always_inline)) {   // row rho = r - 1
+    const int fs = (y0 + r - 1) * nx + x0 - 1 - lead;
+    ring_issue<T, TX, 3, kRingX4>(nbase(r), nsrc, fs, n);
+    ring_issue<T, TX, NRW, kRingX4>(rbase(r), rsrc, fs, n);
+  };
+  // value of ring array q (base: the row's slot) at column c of row ordinal r
+  auto rd = [&](const T *base, int q, int r, int c) __attribute__((always_inline)) -> T {
+    const T v = base[q * RG::CW + lead + 1 + c];
+    if (!CHK) return v;
+    const int j = (y0 + r - 1) * nx + x0 + c;
+    return (j >= 0 && j < n) ? v : T(0);
+  };
+  // the row of ordinal r into (d, h) as load_row would (the halo: column TX)
+  auto read_row = [&](int r, LsRow<T> &d, LsHalo<T> &h) __attribute__((always_inline)) {
+    const T *nb = nbase(r), *rb = rbase(r);
+    d.p = rd(rb, QP, r, t);
+    d.vx = rd(nb, NVX, r, t);
+    d.vy = rd(nb, NVY, r, t);
+    d.bx = rd(rb, QBX, r, t);
+    d.kx = rd(rb, QKX, r, t);
+    d.my = rd(nb, NMY, r, t);
+    if constexpr (ISO) {
+      d.by = d.bx;
+      d.ky = d.kx;
+      d.mx = d.my;
+    } else {
+      d.mx = rd(rb, QMX, r, t);
+      d.by = rd(rb, QBY, r, t);
+      d.ky = rd(rb, QKY, r, t);
+    }
+    if constexpr (AXI) {
+      d.mc = rd(rb, QMC, r, t);
+      d.mt = rd(rb, QMC + 1, r, t);
+    }
+    if constexpr (KAHAN) {
+      d.cp = rd(rb, QCP, r, t);
+      d.cvx = rd(rb, QCP + 1, r, t);
+      d.cvy = rd(rb, QCP + 2, r, t);
+    }
+    if (rh) {
+      h.vx = rd(nb, NVX, r, TX);
+      h.mx = ISO ? rd(nb, NMY, r, TX) : rd(rb, QMX, r, TX);
+      if (AXI) h.mc = rd(rb, QMC, r, TX);
+    }
+  };
+
+  if constexpr (RING != 0) {
+#pragma unroll
+    for (int r = 0; r <= PF; ++r) issue(r);   // rows -1 .. PF-1
+  } else {
+    load_row(-1, ring_regs[0], halo[0]);
+    load_row(0, ring_regs[1], halo[1]);
+  }
   const int iters = H + 2;    // rows -1 .. H
-  run_loop<RS>(iters, [&](auto Qc, auto Wc, int i) __attribute__((always_inline)) {
+  run_loop<RING ? 1 : RS>(iters, [&](auto Qc, auto Wc, int i) __attribute__((always_inline)) {
     constexpr int Q = decltype(Qc)::value, W = decltype(Wc)::value;
     constexpr int WU = (W + 2) % RS, WD = (W + 1) % RS;   // rows rho-1 and rho+1
     const int rho = i - 1;
-    // row rho-1's old velocities and y factor, read before its slot takes row rho+2
-    const T vxu = ring[WU].vx, vyu = ring[WU].vy, myu = ring[WU].my;
-    if (rho + 2 <= H + 1) load_row(rho + 2, ring[WU], halo[WU]);
-    LsRow<T> &P = ring[W];
-    const LsRow<T> &D = ring[WD];
-    const LsHalo<T> &h = halo[W];
+    LsRow<T> rowP = {}, rowD = {};
+    LsHalo<T> hP = {}, hD = {};
+    T vxu, vyu, myu;
+    if constexpr (RING != 0) {
+      // row ordinal i is rho; issue row i + 1 + PF, then wait for row i + 1 (the rows after
+      // it that are in flight: min(PF, H + 2 - (i + 1)))
+      compiler_fence();
+      if (i + 1 + PF <= H + 2) issue(i + 1 + PF);
+      ring_wait<NA, PF>(H + 1 - i);
+      compiler_fence();
+      read_row(i, rowP, hP);
+      const T *nd = nbase(i + 1), *nu = nbase(i - 1);
+      rowD.vx = rd(nd, NVX, i + 1, t);
+      rowD.vy = rd(nd, NVY, i + 1, t);
+      rowD.my = rd(nd, NMY, i + 1, t);
+      // row rho-1 (ordinal i-1): read only where the updates use it (i >= 1)
+      vxu = rd(nu, NVX, i - 1, t);
+      vyu = rd(nu, NVY, i - 1, t);
+      myu = rd(nu, NMY, i - 1, t);
+    } else {
+      // row rho-1's old velocities and y factor, read before its slot takes row rho+2
+      vxu = ring_regs[WU].vx;
+      vyu = ring_regs[WU].vy;
+      myu = ring_regs[WU].my;
+      if (rho + 2 <= H + 1) load_row(rho + 2, ring_regs[WU], halo[WU]);
+    }
+    LsRow<T> &P = RING ? rowP : ring_regs[RING ? 0 : W];
+    const LsRow<T> &D = RING ? rowD : ring_regs[RING ? 0 : WD];
+    const LsHalo<T> &h = RING ? hP : halo[RING ? 0 : W];
+    (void)hD;
 
     // ---- before the barrier: p boundaries and record, row products
     T pcur = P.p, cpcur = P.cp;
@@ -296,14 +430,23 @@ __device__ __forceinline__ int lossy_path(const StepArgs<T> &a, int tile) {
 }
 
 // ISO kernels serve fields whose y coefficient arrays (B, K, M) are the x ones.
-template <typename T, bool KAHAN, int TX, bool AXI, bool NT, bool ISO>
+template <typename T, bool KAHAN, int TX, bool AXI, bool NT, bool ISO, int RING = 0>
 __device__ __forceinline__ void lossy_stream(const StepArgs<T> &a, int tile, int64_t step,
-                                             int64_t row, LossyLds<T, TX, AXI> &L) {
+                                             int64_t row, LossyLds<T, TX, AXI> &L,
+                                             T *ring = nullptr) {
   switch (lossy_path<T, TX>(a, tile)) {
-    case 0: lossy_strip<T, KAHAN, TX, AXI, NT, false, true, ISO>(a, tile, step, row, L); break;
-    case 1: lossy_strip<T, KAHAN, TX, AXI, NT, false, false, ISO>(a, tile, step, row, L); break;
-    case 2: lossy_strip<T, KAHAN, TX, AXI, NT, true, true, ISO>(a, tile, step, row, L); break;
-    case 3: lossy_strip<T, KAHAN, TX, AXI, NT, true, false, ISO>(a, tile, step, row, L); break;
+    case 0:
+      lossy_strip<T, KAHAN, TX, AXI, NT, false, true, ISO, RING>(a, tile, step, row, L, ring);
+      break;
+    case 1:
+      lossy_strip<T, KAHAN, TX, AXI, NT, false, false, ISO, RING>(a, tile, step, row, L, ring);
+      break;
+    case 2:
+      lossy_strip<T, KAHAN, TX, AXI, NT, true, true, ISO, RING>(a, tile, step, row, L, ring);
+      break;
+    case 3:
+      lossy_strip<T, KAHAN, TX, AXI, NT, true, false, ISO, RING>(a, tile, step, row, L, ring);
+      break;
     default: break;
   }
 }

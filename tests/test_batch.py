@@ -70,9 +70,10 @@ def _stream_index(slot):
     return [i for i, e in enumerate(pyfds_amd._native.tile_engines()) if e == 'stream'][slot]
 
 
-@pytest.mark.parametrize('slot', [0, 2, 3, 6, 8, 9, 11])
+@pytest.mark.parametrize('slot', [0, 2, 3, 6, 8, 9, 11, 14])
 def test_fp64_stream_batch_matches_oracle(slot):
-    """A batch of lossless members on the row-streaming engine."""
+    """A batch of lossless members on the row-streaming engine (slot 14: rows streamed into
+    LDS rings one dword per lane, members of any width, 7 columns included)."""
     members = []
     for name in LOSSLESS:
         scene = dict(scenes.SCENES[name](), plan=PLAN)
@@ -88,7 +89,7 @@ def test_fp64_stream_batch_matches_oracle(slot):
             np.testing.assert_array_equal(got[k], ref[k], err_msg='%s %s' % (scene['name'], k))
 
 
-@pytest.mark.parametrize('slot,strip', [(9, 24), (8, 20), (9, 12), (5, 6)])
+@pytest.mark.parametrize('slot,strip', [(9, 24), (8, 20), (9, 12), (5, 6), (14, 12), (14, 20)])
 def test_fp64_stream_batch_strip_heights_match_oracle(slot, strip):
     """Batches on strip heights other than the shape's own (the autotuner's candidates)."""
     members = []
@@ -194,11 +195,15 @@ def sweep_reference():
     return [scenes.run_oracle(scene) for scene in sweep_scenes()]
 
 
-def _run_sweep(form, schedule='dispatch', plan=(SWEEP_STEPS,)):
+def _run_sweep(form, schedule='dispatch', plan=(SWEEP_STEPS,), tile=None, strip=0):
     from pyfds_amd import autotune
     members = sweep_fields(form)
     fields = [f for _, f in members]
-    tile, _, _ = autotune.tune_batch(fields, steps=10, use_cache=False)
+    if tile is None:
+        tile, _, _ = autotune.tune_batch(fields, steps=10, use_cache=False)
+    else:
+        for f in fields:
+            f.strip_height = strip
     print('sweep', form, schedule, 'tile', tile, 'strip', fields[0].strip_height, flush=True)
     batch = BatchRunner(fields, tile=tile, chunk=50, schedule=schedule)
     for n in plan:
@@ -216,6 +221,27 @@ def test_fp64_sweep_batch_matches_oracle(sweep_reference, schedule):
             np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
 
 
+@pytest.mark.parametrize('schedule', ['dispatch', 'queue'])
+def test_fp64_sweep_batch_on_ring_shape_matches_oracle(sweep_reference, schedule):
+    """The uneven sweep on the LDS-ring shape (dword-mode rows, widths 128-640), both launch
+    schedules, in segments of uneven length."""
+    plan = (SWEEP_STEPS,) if schedule == 'dispatch' else (7, 101, SWEEP_STEPS - 108)
+    got_all = _run_sweep('fp64', schedule, plan, tile=_stream_index(14), strip=12)
+    for got, ref in zip(got_all, sweep_reference):
+        for k in ref:
+            np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+
+
+@pytest.mark.parametrize('form', ['fp32', 'fp32_kahan'])
+def test_fp32_ring_batch_equals_register_batch(form):
+    """FP32 forms: the ring shape's batch kernels give the register shape's bits."""
+    a = _run_sweep(form, 'dispatch', (60,), tile=_stream_index(9), strip=12)
+    b = _run_sweep(form, 'dispatch', (60,), tile=_stream_index(14), strip=12)
+    for x, y in zip(a, b):
+        for k in x:
+            np.testing.assert_array_equal(x[k], y[k], err_msg=k)
+
+
 def test_kahan_queue_schedule_equals_dispatch():
     a = _run_sweep('fp32_kahan', 'dispatch')
     b = _run_sweep('fp32_kahan', 'queue', (50, 150))
@@ -231,10 +257,10 @@ def test_queue_schedule_needs_streaming_shape():
         batch.simulate(2)
 
 
-def test_ring_and_prefetch_shapes_serve_single_fields_only():
+def test_prefetch_shapes_serve_single_fields_only():
     members = sweep_fields('fp64')[:2]
     idx = [d['index'] for d in pyfds_amd._native.tile_table() if d['single_only']]
-    assert len(idx) == 3
+    assert len(idx) == 2
     for i in idx:
         with pytest.raises(ValueError):
             BatchRunner([f for _, f in members], tile=i).simulate(2)

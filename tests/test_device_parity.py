@@ -88,8 +88,10 @@ def test_fp64_tile_shapes_bit_identical(tile):
 # 3 (256 x 32, three), 4-6 the same with non-temporal state traffic, 7-9 one-wave 64-wide
 # shapes with one, two and three steps, 10-11 64-wide non-temporal with one and two steps
 # (tests/test_host_api.py pins these traits), 12-14 loading their rows straight into LDS rings:
-# 64 x 32 three steps two rows ahead, 64 x 16 two steps two rows ahead, 64 x 32 three steps
-# one row ahead (nx must be a multiple of 2 in FP64, of 4 in FP32).
+# 64 x 32 three steps two rows ahead, 64 x 16 two steps two rows ahead (lossless single fields
+# only), 64 x 32 three steps one row ahead (lossy fields run the lossy kernel on it); a single
+# field whose rows do not start on 16-byte boundaries (nx odd in FP64, not a multiple of 4 in
+# FP32) runs the register-path kernel of the same tiling.
 NSTREAM = 15
 
 
@@ -111,9 +113,8 @@ def test_fp64_stream_shapes_match_reference_golden(name, slot):
     gold = np.load(os.path.join(GOLDEN, name + '.npz'))
     print('scene', name, 'shape', idx[slot], flush=True)
     info = pyfds_amd._native.tile_table()[idx[slot]]
-    if ('lossy' in name and info['single_only']) or \
-            (info['lds_ring'] and scene['kwargs']['x_samples'] % 2):
-        with pytest.raises(ValueError):    # lossless fields only; ring rows 16-byte aligned
+    if 'lossy' in name and info['single_only']:
+        with pytest.raises(ValueError):    # prefetch shapes serve lossless fields only
             run_device(scene, tile=idx[slot])
         return
     res = run_device(scene, tile=idx[slot])
@@ -200,7 +201,10 @@ def test_fp64_stream_multi_strip_grids_match_oracle_bitwise(model, nx, ny, slot)
                                               ('Acoustic2D', 190, 77, 8),
                                               ('Acoustic3DAxi', 333, 101, 9),
                                               ('Acoustic2D', 61, 45, 10),
-                                              ('Acoustic3DAxi', 200, 120, 11)])
+                                              ('Acoustic3DAxi', 200, 120, 11),
+                                              ('Acoustic2D', 256, 200, 14),
+                                              ('Acoustic3DAxi', 260, 133, 14),
+                                              ('Acoustic2D', 509, 97, 14)])
 def test_fp64_lossy_stream_grids_match_oracle_bitwise(model, nx, ny, slot):
     idx = _stream_indices()[slot]
     scene = _big_acoustic(nx, ny, 120, model=model, mu=3e-3)
@@ -219,7 +223,9 @@ def test_fp64_lossy_stream_grids_match_oracle_bitwise(model, nx, ny, slot):
                                                     ('Acoustic3DAxi', 200, 97, 9, 20),
                                                     ('Acoustic2D', 150, 71, 8, 6),
                                                     ('Acoustic3DAxi', 170, 60, 1, 24),
-                                                    ('Acoustic2D', 260, 90, 3, 12)])
+                                                    ('Acoustic2D', 260, 90, 3, 12),
+                                                    ('Acoustic2D', 190, 77, 14, 12),
+                                                    ('Acoustic3DAxi', 200, 97, 14, 6)])
 def test_fp64_lossy_stream_strip_heights_match_oracle_bitwise(model, nx, ny, slot, strip):
     idx = _stream_indices()[slot]
     scene = _big_acoustic(nx, ny, 120, model=model, mu=3e-3)
@@ -236,7 +242,7 @@ def test_fp32_lossy_stream_equals_tile_engine(name, form):
     scene = scenes.SCENES[name]()
     print('scene', name, form, flush=True)
     a = run_device(scene, form=form, tile=1)
-    for idx in [_stream_indices()[i] for i in (0, 2, 7, 10)]:
+    for idx in [_stream_indices()[i] for i in (0, 2, 7, 10, 14)]:
         b = run_device(scene, form=form, tile=idx)
         for k in a:
             np.testing.assert_array_equal(a[k], b[k], err_msg='%s %s %d' % (name, k, idx))
@@ -382,11 +388,19 @@ def test_fp64_thermal_fused_remainder_launches_match_reference(slot):
 # LDS-ring shapes (12-14): rows reach LDS by direct 16-byte loads, so rows must start on
 # 16-byte boundaries; FP32 forms (4-byte elements) and tiles reading outside the arrays.
 @pytest.mark.parametrize('form,nx', [('fp64', 201), ('fp32', 202), ('fp32_kahan', 206)])
-def test_ring_shapes_need_aligned_rows(form, nx):
-    scene = _big_acoustic(nx, 40, 12)
-    for slot in (12, 13, 14):
-        with pytest.raises(ValueError):
-            run_device(scene, form=form, tile=_stream_indices()[slot])
+def test_ring_shapes_with_unaligned_rows_run_their_register_twin(form, nx):
+    """Rows that do not start on 16-byte boundaries: the ring shape runs the register-path
+    kernel of its tiling (slot 9 for 12 and 14, slot 8 for 13), bit for bit."""
+    scene = dict(_big_acoustic(nx, 40, 12), plan=[('simulate', 7), ('simulate', 5)])
+    for slot, twin in ((12, 9), (13, 8), (14, 9)):
+        a = run_device(scene, form=form, tile=_stream_indices()[slot])
+        b = run_device(scene, form=form, tile=_stream_indices()[twin])
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k], err_msg='%s %d %s' % (form, slot, k))
+    if form == 'fp64':
+        ref = scenes.run_oracle(scene)
+        for k in ref:
+            np.testing.assert_array_equal(a[k], ref[k], err_msg=k)
 
 
 @pytest.mark.parametrize('form', ['fp32', 'fp32_kahan'])

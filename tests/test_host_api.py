@@ -51,7 +51,7 @@ def test_tile_table():
                       (64, 32, 3, F, 1, F), (64, 16, 1, T, 1, F), (64, 16, 2, T, 1, F),
                       (64, 32, 3, F, 2, T), (64, 16, 2, F, 2, T), (64, 32, 3, F, 1, T)]
     assert [d['single_only'] for d in _native.tile_table() if d['engine'] == 'stream'] == \
-        [F] * 12 + [T] * 3
+        [F] * 12 + [T, T, F]
     assert all(d['steps'] == 1 and not d['nontemporal'] and d['prefetch'] == 0
                and not d['lds_ring'] for d in _native.tile_table() if d['engine'] == 'tile')
 
