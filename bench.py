@@ -125,8 +125,7 @@ def time_lossy(ns, wl, form, steps, warmup, fixed=None):
     return dict(value=round(cells * steps / wall / 1e9, 3), ms_per_step=round(wall / steps * 1e3, 4),
                 kernel_us=round(kernel_s * 1e6, 2), achieved_gbs=round(achieved, 1),
                 roofline_frac=round(achieved / HBM_PEAK_GBS, 4),
-                bytes_per_cell=LOSSY_BYTES_PER_CELL[form],
-                autotune={k: v for k, v in tune.items() if k != 'table'})
+                bytes_per_cell=LOSSY_BYTES_PER_CELL[form], autotune=tune)
 
 
 def tuned(fld, fixed=None):
@@ -257,6 +256,29 @@ def launch_timing(fld, launches=30):
     ms = ct.c_double()
     runner.lib.fds_last_run_ms(runner.ctx, ct.byref(ms))
     return ms.value * 1e3 / launches, spl
+
+
+def pattern_floor(tune, nx, ny, elem, n_state, launch_us):
+    """For an LDS-ring shape: the time per launch of the memory-only kernel with the step
+    kernel's access pattern (``fds_stream_pattern_bench``: its tiles, tile order, ring loads,
+    LDS footprint and stores, no arithmetic) at the tuned strip height, and the step kernel's
+    launch time over it (1 = the kernel runs at the floor its memory traffic sets)."""
+    import ctypes as ct
+    from pyfds_amd import _native as nat
+    info = nat.tile_table()[tune['tile']]
+    spl = tune['steps_per_launch']
+    if not info['lds_ring'] or info['tx'] != 64 or spl not in (2, 3):
+        return None
+    strip = tune['strip'] or info['ty']
+    lib = nat.load()
+    ms, b = ct.c_double(), ct.c_double()
+    nat.check(lib.fds_stream_pattern_bench(nx, ny, elem, spl, spl - 1, strip, n_state, 20,
+                                           ct.byref(ms), ct.byref(b)))
+    us = ms.value * 1e3
+    return dict(us_per_launch=round(us, 2), requested_gbs=round(b.value / us / 1e3, 1),
+                kernel_over_floor=round(launch_us / us, 3),
+                geometry='64-column tiles, strips of %d, %d ghost rows/columns per side, '
+                         '%d-stage rings' % (strip, spl - 1, spl))
 
 
 def roofline_entry(bytes_per_update, cells, launch_us, spl, traffic):
@@ -875,6 +897,11 @@ def main():
     roof, detail = roofline_entry(BYTES_PER_CELL[main_form], NX * NY, ref['launch_us'],
                                   ref['tune']['steps_per_launch'], traffic)
     detail['kernel'] = ref['tune']['shape']
+    if main_form in ('fp64', 'fp32'):
+        floor = pattern_floor(ref['tune'], NX, NY, 8 if main_form == 'fp64' else 4, 3,
+                              ref['launch_us'])
+        if floor is not None:
+            detail['pattern_floor'] = floor
     detail['note'] = ('frac: measured HBM bytes per launch / launch time / 8 TB/s; a launch '
                       'advances steps_per_launch fused steps in one pass, so the per-step '
                       'algorithmic rate (algorithmic_frac, 64 B per cell update) may pass 1 '
@@ -977,6 +1004,12 @@ def main():
                 traffic = measure_traffic('thermal', f, r['tune']['tile'], r['tune']['strip'])
             roof, detail = roofline_entry(THERMAL_BYTES_PER_CELL[f], r['cells'], r['launch_us'],
                                           r['tune']['steps_per_launch'], traffic)
+            if f in ('fp64', 'fp32'):
+                n = int(round(r['cells'] ** 0.5))
+                floor = pattern_floor(r['tune'], n, n, 8 if f == 'fp64' else 4, 1,
+                                      r['launch_us'])
+                if floor is not None:
+                    detail['pattern_floor'] = floor
             th[f] = dict(value=round(r['gcells'], 3), ms_per_step=round(r['ms_per_step'], 4),
                          kernel_us=round(r['kernel_us'], 2),
                          algorithmic_frac=round(r['achieved_gbs'] / HBM_PEAK_GBS, 4),

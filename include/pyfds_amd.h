@@ -364,6 +364,19 @@ int fds_output_device_ptr(fds_ctx *ctx, int comp, double **ptr, int64_t *rows, i
  * non-temporal. */
 int fds_stream_copy_bench(int64_t bytes, int reps, int variant, double *gbs, double *ms_each);
 
+/* Calibration (no reference counterpart): the memory access pattern of the LDS-ring streaming
+ * kernels without their arithmetic, on an nx x ny grid of `elem`-byte elements (8: float64,
+ * 4: float32).  One-wave tiles of 64 columns (64 - 2*ghost owned) and strip_h rows, in the
+ * step kernels' order, stream rows -ghost-1 .. strip_h+ghost of n_state state arrays and two
+ * coefficient arrays through the rings of a `stages`-stage kernel (same LDS footprint, one
+ * row ahead) and store the state arrays' owned cells.  n_state 3 (Acoustic2D, isotropic
+ * coefficients) or 1 (Thermal2D); stages 2 or 3; nx a multiple of 16 / elem; nx * ny at most
+ * 2^28.  Returns milliseconds per launch (after one untimed launch) and the bytes the launch
+ * requests (loads and stores).  The step kernel's launch time over this one is how far its
+ * arithmetic keeps it from the floor its memory traffic sets. */
+int fds_stream_pattern_bench(int nx, int ny, int elem, int stages, int ghost, int strip_h,
+                             int n_state, int reps, double *ms_each, double *bytes);
+
 /* Last error message of the calling thread ("" if none). */
 const char *fds_last_error(void);
 

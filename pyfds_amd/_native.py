@@ -24,7 +24,7 @@ EXPORTS = ('fds_create', 'fds_destroy', 'fds_set_scalars', 'fds_set_coeff', 'fds
            'fds_run', 'fds_run_async', 'fds_run_take', 'fds_prepare_run', 'fds_read_outputs', 'fds_set_tile', 'fds_tile_count', 'fds_tile_shape',
            'fds_tile_engine', 'fds_tile_traits', 'fds_set_strip_height', 'fds_engine', 'fds_steps_per_launch', 'fds_set_line_shape', 'fds_set_flow',
            'fds_couple_loss_run',
-           'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_batch_create', 'fds_batch_run',
+           'fds_last_run_ms', 'fds_stream_copy_bench', 'fds_stream_pattern_bench', 'fds_batch_create', 'fds_batch_run',
            'fds_batch_run_async', 'fds_batch_take', 'fds_batch_steps_per_launch',
            'fds_batch_last_run_ms', 'fds_batch_destroy', 'fds_batch_set_schedule',
            'fds_set_learned_boundary',
@@ -106,6 +106,7 @@ def load():
         'fds_set_tile': (ct.c_int, [_p, ct.c_int]),
         'fds_last_run_ms': (ct.c_int, [_p, _dp]),
         'fds_stream_copy_bench': (ct.c_int, [_i64, ct.c_int, ct.c_int, _dp, _dp]),
+        'fds_stream_pattern_bench': (ct.c_int, [ct.c_int] * 8 + [_dp, _dp]),
         'fds_batch_create': (ct.c_int, [ct.POINTER(_p), ct.POINTER(_p), ct.c_int, ct.c_int]),
         'fds_batch_run': (ct.c_int, [_p, _i64, _i64, ct.c_int]),
         'fds_batch_run_async': (ct.c_int, [_p, _i64, _i64, ct.c_int]),

@@ -225,6 +225,14 @@ __device__ __forceinline__ int ring_slot(int i, int add, int m) {
   return (v % m + m) % m;
 }
 
+// Slot `add` rows after the one in slot `base` of a ring of m slots (-m < add < m): one
+// wrap instead of a division.
+__device__ __forceinline__ int wrap_slot(int base, int add, int m) {
+  int v = base + add;
+  v = v >= m ? v - m : v;
+  return v < 0 ? v + m : v;
+}
+
 // Whether every read of a streaming tile (rows y0-G-1 .. y0+H+G, columns x0-1 .. x0+TX) falls
 // inside the array, so its loads need no bounds checks.
 __device__ __forceinline__ bool strip_inside(int x0, int y0, int H, int G, int TX, int nx,
@@ -305,6 +313,39 @@ template <int RS, typename F>
 __device__ __forceinline__ void run_loop(int iters, F &&f) {
   constexpr int U = RS % 2 == 0 ? RS : 2 * RS;
   for (int i = 0; i < iters; i += U) run_block<RS>(f, i, iters, std::make_integer_sequence<int, U>{});
+}
+
+// As run_block, with the body's steady flag ST (a compile-time argument after Q and W).
+template <int RS, bool ST, typename F, int... J>
+__device__ __forceinline__ void run_block_st(F &f, int i, int iters,
+                                             std::integer_sequence<int, J...>) {
+  ((i + J < iters
+        ? (f(std::integral_constant<int, J % 2>{}, std::integral_constant<int, J % RS>{},
+             std::integral_constant<bool, ST>{}, i + J),
+           0)
+        : 0),
+   ...);
+}
+
+// As run_loop, for a body with a steady form (SF): the whole blocks of iterations inside
+// [i0, i1] run with ST = true (the body's row-range tests known to hold), the others with
+// ST = false.  One loop, so each form of the body is inlined once; without SF only the
+// general form exists.
+template <int RS, bool SF, typename F>
+__device__ __forceinline__ void run_loop_steady(int iters, int i0, int i1, F &&f) {
+  constexpr int U = RS % 2 == 0 ? RS : 2 * RS;
+  constexpr auto seq = std::make_integer_sequence<int, U>{};
+  if constexpr (!SF) {
+    for (int i = 0; i < iters; i += U) run_block_st<RS, false>(f, i, iters, seq);
+  } else {
+    const int lo = i0 < 0 ? 0 : i0, hi = i1 + 1 < iters ? i1 + 1 : iters;
+    const int s0 = (lo + U - 1) / U * U;                  // first steady block
+    const int s1 = hi / U * U;                            // past the last one
+    for (int i = 0; i < iters; i += U) {
+      if (i >= s0 && i < s1) run_block_st<RS, true>(f, i, iters, seq);
+      else run_block_st<RS, false>(f, i, iters, seq);
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -505,8 +546,17 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
   }
   const int iters = H + 2 * G + S + 1;
   // RING: slots are computed, so the loop only unrolls by the parity
-  run_loop<RING ? 1 : RS>(iters, [&](auto Qc, auto Wc, int i) __attribute__((always_inline)) {
+  // RING without events: iterations 3S-1 .. H+2G+1-PF run the steady form of the body (ST):
+  // every stage's row is inside its range and stage 0 fetches the row PF ahead, so the range
+  // tests are compiled out
+  run_loop_steady<RING ? 1 : RS, RING && !EV>(iters, 3 * S - 1, H + 2 * G + 1 - PF,
+                                              [&](auto Qc, auto Wc, auto STc, int i)
+                                                  __attribute__((always_inline)) {
     constexpr int Q = decltype(Qc)::value, W = decltype(Wc)::value;
+    constexpr bool ST = decltype(STc)::value;
+    // RING: slots of this iteration's row ordinal i in the state and coefficient rings (the
+    // other rows' slots follow by one wrap)
+    const int is0 = RING ? ring_slot(i, 0, RS) : 0, ic0 = RING ? ring_slot(i, 0, RC) : 0;
     T bpx[S];
     AcRow<T> row0 = {};      // RING: stage 0 input row of this iteration, read from the rings
     AcHalo<T> h0 = {};
@@ -517,21 +567,21 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
       const StageRows R(G, H, s);
-      if (!RING && s > 0 && rho + 1 >= R.lo - 1 && rho + 1 <= R.hi + 1)
+      if (!RING && s > 0 && (ST || (rho + 1 >= R.lo - 1 && rho + 1 <= R.hi + 1)))
         load_coef(rho + 1, cf[s][Q ^ 1]);
-      if (rho < R.lo - 1 || rho > R.hi + 1) return;
+      if (!ST && (rho < R.lo - 1 || rho > R.hi + 1)) return;
       if constexpr (RING) {
         if (s == 0) {
           compiler_fence();
-          if (rho + PF <= R.hi + 1)
-            issue_row(rho + PF, ring_slot(i, PF, RS), ring_slot(i, PF, RC));
+          if (ST || rho + PF <= R.hi + 1)
+            issue_row(rho + PF, wrap_slot(is0, PF, RS), wrap_slot(ic0, PF, RC));
           // loads issued after this row's: the rows rho + 1 .. min(rho + PF, hi + 1)
-          ring_wait<NA, PF>(R.hi + 1 - rho);
+          if constexpr (ST) vm_wait<PF * NA>(); else ring_wait<NA, PF>(R.hi + 1 - rho);
           compiler_fence();
-          read_row(rho, ring_slot(i, 0, RS), ring_slot(i, 0, RC), row0, h0);
+          read_row(rho, is0, ic0, row0, h0);
         }
       } else {
-        if (s == 0 && rho + PF <= R.hi + 1)
+        if (s == 0 && (ST || rho + PF <= R.hi + 1))
           load_row(rho + PF, nin[(W + PF) % RS], nh[(W + PF) % RS]);
       }
       T &p = s == 0 ? in0.p : pn[s > 0 ? s - 1 : 0][Q ^ 1];
@@ -542,7 +592,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
           cbx = in0.bx;
           cby = in0.by;
         } else {
-          const T *cb = cbase(ring_slot(i, -2 * s, RC));
+          const T *cb = cbase(wrap_slot(ic0, -2 * s, RC));
           cbx = rd(cb, CBX, rho, t);
           cby = ISO ? cbx : rd(cb, CBY, rho, t);
         }
@@ -559,7 +609,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       }
       bpx[s] = cbx * p;
       bpy[s][Q] = cby * p;
-      if (!WAVE && rho <= R.hi) {
+      if (!WAVE && (ST || rho <= R.hi)) {
         sbp[s][Q][t + 1] = bpx[s];
         if (s == 0 && lh) sbp[s][Q][0] = h.b * h.p;
       }
@@ -574,7 +624,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
       const StageRows R(G, H, s);
-      if (rho < R.lo - 1 || rho > R.hi + 1) return;
+      if (!ST && (rho < R.lo - 1 || rho > R.hi + 1)) return;
       const int sp = s > 0 ? s - 1 : 0;
       const T ivx = s == 0 ? in0.vx : vh[sp][Q].vx;
       const T icvx = s == 0 ? in0.cvx : vh[sp][Q].cvx;
@@ -586,7 +636,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
           ckx = in0.kx;
           cky = in0.ky;
         } else {
-          const T *cb = cbase(ring_slot(i, -2 * s, RC));
+          const T *cb = cbase(wrap_slot(ic0, -2 * s, RC));
           ckx = rd(cb, CKX, rho, t);
           cky = ISO ? ckx : rd(cb, CKY, rho, t);
         }
@@ -598,7 +648,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       const T cpprev = s == 0 ? cpp[Q ^ 1] : cpn[sp][Q];
       const AcHalo<T> &h = hin;
       T vxn = T(0), cvxn = T(0), vyn = T(0), cvyn = T(0);
-      if (rho >= R.lo && rho <= R.hi) {
+      if (ST || (rho >= R.lo && rho <= R.hi)) {
         T left;
         if constexpr (WAVE) {
           left = wave_from_left(bpx[s]);
@@ -641,7 +691,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
         }
       }
       kvy[s][Q] = T(0);
-      if (rho >= R.lo) {
+      if (ST || rho >= R.lo) {
         const T d = (-bpy[s][Q ^ 1]) + bpy[s][Q];
         vyn = ivy;
         cvyn = icvy;
@@ -656,7 +706,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
         kvy[s][Q] = cky * vyn;
       }
       T pnew = T(0), cpnew = T(0);
-      if (rho > R.lo) {    // p of row rho-1 (its x products were published an iteration ago)
+      if (ST || rho > R.lo) {    // p of row rho-1 (its x products were published an iteration ago)
         const T dxt = WAVE ? kvd[s][Q ^ 1] : (-skv[s][Q ^ 1][t]) + skv[s][Q ^ 1][t + 1];
         const T dyt = (-kvy[s][Q ^ 1]) + kvy[s][Q];
         const T d = dxt + dyt;
@@ -841,8 +891,17 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
   }
   const int iters = H + 2 * G + S + 1;
   // RING: slots are computed, so the loop only unrolls by the parity
-  run_loop<RING ? 1 : RS>(iters, [&](auto Qc, auto Wc, int i) __attribute__((always_inline)) {
+  // RING without events: iterations 3S-1 .. H+2G+1-PF run the steady form of the body (ST):
+  // every stage's row is inside its range and stage 0 fetches the row PF ahead, so the range
+  // tests are compiled out
+  run_loop_steady<RING ? 1 : RS, RING && !EV>(iters, 3 * S - 1, H + 2 * G + 1 - PF,
+                                              [&](auto Qc, auto Wc, auto STc, int i)
+                                                  __attribute__((always_inline)) {
     constexpr int Q = decltype(Qc)::value, W = decltype(Wc)::value;
+    constexpr bool ST = decltype(STc)::value;
+    // RING: slots of this iteration's row ordinal i in the state and coefficient rings (the
+    // other rows' slots follow by one wrap)
+    const int is0 = RING ? ring_slot(i, 0, RS) : 0, ic0 = RING ? ring_slot(i, 0, RC) : 0;
     T qtx[S];
     ThRow<T> row0 = {};      // RING: stage 0 input row of this iteration, read from the rings
     ThHalo<T> h0 = {};
@@ -852,20 +911,20 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
       constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
       const StageRows R(G, H, s);
-      if (!RING && s > 0 && rho + 1 >= R.lo - 1 && rho + 1 <= R.hi + 1)
+      if (!RING && s > 0 && (ST || (rho + 1 >= R.lo - 1 && rho + 1 <= R.hi + 1)))
         load_coef(rho + 1, cf[s][Q ^ 1]);
-      if (rho < R.lo - 1 || rho > R.hi + 1) return;
+      if (!ST && (rho < R.lo - 1 || rho > R.hi + 1)) return;
       if constexpr (RING) {
         if (s == 0) {
           compiler_fence();
-          if (rho + PF <= R.hi + 1)
-            issue_row(rho + PF, ring_slot(i, PF, RS), ring_slot(i, PF, RC));
-          ring_wait<NA, PF>(R.hi + 1 - rho);
+          if (ST || rho + PF <= R.hi + 1)
+            issue_row(rho + PF, wrap_slot(is0, PF, RS), wrap_slot(ic0, PF, RC));
+          if constexpr (ST) vm_wait<PF * NA>(); else ring_wait<NA, PF>(R.hi + 1 - rho);
           compiler_fence();
-          read_row(rho, ring_slot(i, 0, RS), ring_slot(i, 0, RC), row0, h0);
+          read_row(rho, is0, ic0, row0, h0);
         }
       } else {
-        if (s == 0 && rho + PF <= R.hi + 1)
+        if (s == 0 && (ST || rho + PF <= R.hi + 1))
           load_row(rho + PF, nin[(W + PF) % RS], nh[(W + PF) % RS]);
       }
       T &tv = s == 0 ? in0.t : tn[s > 0 ? s - 1 : 0][Q ^ 1];
@@ -876,7 +935,7 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
           cqx = in0.qx;
           cqy = in0.qy;
         } else {
-          const T *cb = cbase(ring_slot(i, -2 * s, RC));
+          const T *cb = cbase(wrap_slot(ic0, -2 * s, RC));
           cqx = rd(cb, CQX, rho, t);
           cqy = ISO ? cqx : rd(cb, CQY, rho, t);
         }
@@ -893,7 +952,7 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
       }
       qtx[s] = cqx * tv;
       qty[s][Q] = cqy * tv;
-      if (!WAVE && rho <= R.hi) {
+      if (!WAVE && (ST || rho <= R.hi)) {
         sqt[s][Q][t + 1] = qtx[s];
         if (s == 0 && lh) sqt[s][Q][0] = h.q * h.t;
       }
@@ -907,7 +966,7 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
       constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
       const StageRows R(G, H, s);
-      if (rho < R.lo - 1 || rho > R.hi + 1) return;
+      if (!ST && (rho < R.lo - 1 || rho > R.hi + 1)) return;
       const int sp = s > 0 ? s - 1 : 0;
       T cgx, cgy;
       if constexpr (RING) {
@@ -915,7 +974,7 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
           cgx = in0.gx;
           cgy = in0.gy;
         } else {
-          const T *cb = cbase(ring_slot(i, -2 * s, RC));
+          const T *cb = cbase(wrap_slot(ic0, -2 * s, RC));
           cgx = rd(cb, CGX, rho, t);
           cgy = ISO ? cgx : rd(cb, CGY, rho, t);
         }
@@ -928,7 +987,7 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
       const ThHalo<T> &h = hin;
       T zero = T(0), zeroc = T(0);
       T fx = T(0), fy = T(0);
-      if (rho >= R.lo && rho <= R.hi) {
+      if (ST || (rho >= R.lo && rho <= R.hi)) {
         T left;
         if constexpr (WAVE) {
           left = wave_from_left(qtx[s]);
@@ -963,7 +1022,7 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
         }
       }
       gqy[s][Q] = T(0);
-      if (rho >= R.lo) {
+      if (ST || rho >= R.lo) {
         fy = -((-qty[s][Q ^ 1]) + qty[s][Q]);
         if constexpr (EV) {
           T dummy = T(0), dummyc = T(0);
@@ -975,7 +1034,7 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
         gqy[s][Q] = cgy * fy;
       }
       T tnew = T(0), ctnew = T(0);
-      if (rho > R.lo) {
+      if (ST || rho > R.lo) {
         const T dxt = WAVE ? gqd[s][Q ^ 1] : (-sgq[s][Q ^ 1][t]) + sgq[s][Q ^ 1][t + 1];
         const T dyt = (-gqy[s][Q ^ 1]) + gqy[s][Q];
         const T d = dxt + dyt;

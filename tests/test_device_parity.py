@@ -466,3 +466,22 @@ def test_values_array_held_across_runs_is_updated_in_place():
     g.simulate(10)
     np.testing.assert_array_equal(f.pressure.values, g.pressure.values)
     np.testing.assert_array_equal(f.velocity_x.values, g.velocity_x.values)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('elem,ns', [(8, 3), (4, 1)])
+def test_pattern_bench_requests_the_step_kernels_bytes(elem, ns):
+    """The memory-only pattern kernel runs and requests what a three-stage ring kernel of the
+    same tiling moves: every tile loads H + 2G + 2 rows of ns + 2 windows of CW elements (CW =
+    68 in float64, 72 in float32) and every cell is stored once per state array."""
+    import ctypes as ct
+    from pyfds_amd import _native
+    lib = _native.load()
+    ms, b = ct.c_double(), ct.c_double()
+    nx, ny, G, H = 512, 256, 2, 12
+    rc = lib.fds_stream_pattern_bench(nx, ny, elem, 3, G, H, ns, 2, ct.byref(ms), ct.byref(b))
+    assert rc == 0, rc
+    ntiles = -(-nx // (64 - 2 * G)) * -(-ny // H)
+    cw = 68 if elem == 8 else 72
+    assert b.value == (ntiles * (H + 2 * G + 2) * (ns + 2) * cw + nx * ny * ns) * elem
+    assert 0 < ms.value < 100

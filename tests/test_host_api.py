@@ -30,6 +30,21 @@ def test_library_exports_every_declared_symbol():
     assert lib.fds_version().startswith(b'pyfds_amd')
 
 
+def test_pattern_bench_rejects_bad_arguments():
+    """The calibration entry point checks its arguments before touching the GPU."""
+    import ctypes as ct
+    lib = _native.load()
+    ms, b = ct.c_double(), ct.c_double()
+    good = dict(nx=512, ny=256, elem=8, stages=3, ghost=2, strip=12, ns=3, reps=1)
+    for bad in (dict(elem=6), dict(stages=1), dict(ns=2), dict(nx=511), dict(strip=0),
+                dict(ghost=-1), dict(reps=0), dict(nx=1 << 15, ny=1 << 14), dict(elem=4, nx=514)):
+        a = dict(good, **bad)
+        rc = lib.fds_stream_pattern_bench(a['nx'], a['ny'], a['elem'], a['stages'], a['ghost'],
+                                          a['strip'], a['ns'], a['reps'], ct.byref(ms),
+                                          ct.byref(b))
+        assert rc == -1, bad
+
+
 def test_tile_table():
     shapes, engines = _native.tile_shapes(), _native.tile_engines()
     assert (64, 16) in shapes and (256, 1) in shapes
