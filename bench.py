@@ -208,19 +208,30 @@ def measure_traffic(kind, form, tile, strip, launches=20):
         scale = COPY_CAL_BYTES / (float(np.median(copy)) * 1024.0)
         out[counter] = dict(kib=float(np.median(vals)), scale=round(scale, 4), kernel=name,
                             dispatches=len(vals))
+        pat = [v for k, v in rows if 'k_stream_pattern<' in k]
+        if pat:
+            out[counter]['pattern_kib'] = float(np.median(pat))
     fetch, write = out['FETCH_SIZE'], out['WRITE_SIZE']
     read_b = fetch['kib'] * 1024 * fetch['scale']
     write_b = write['kib'] * 1024 * write['scale']
-    return dict(bytes=int(round(read_b + write_b)), read_bytes=int(round(read_b)),
-                write_bytes=int(round(write_b)), fetch_kib=round(fetch['kib'], 1),
-                write_kib=round(write['kib'], 1), fetch_scale=fetch['scale'],
-                write_scale=write['scale'], kernel=fetch['kernel'],
-                dispatches=fetch['dispatches'])
+    res = dict(bytes=int(round(read_b + write_b)), read_bytes=int(round(read_b)),
+               write_bytes=int(round(write_b)), fetch_kib=round(fetch['kib'], 1),
+               write_kib=round(write['kib'], 1), fetch_scale=fetch['scale'],
+               write_scale=write['scale'], kernel=fetch['kernel'],
+               dispatches=fetch['dispatches'])
+    if 'pattern_kib' in fetch and 'pattern_kib' in write:
+        # the same counters for the memory-only kernel of the same access pattern
+        pr = fetch['pattern_kib'] * 1024 * fetch['scale']
+        pw = write['pattern_kib'] * 1024 * write['scale']
+        res['pattern'] = dict(bytes=int(round(pr + pw)), read_bytes=int(round(pr)),
+                              write_bytes=int(round(pw)))
+    return res
 
 
 def traffic_probe(spec):
     """Child of measure_traffic (runs under rocprofv3): a 1 GiB calibration copy, then
-    ``launches`` full launches of the step kernel at the given shape."""
+    ``launches`` full launches of the step kernel at the given shape and, for an LDS-ring
+    shape, as many launches of the memory-only kernel of its access pattern."""
     import ctypes as ct
     import pyfds_amd as ns
     from pyfds_amd import _native as nat
@@ -240,6 +251,14 @@ def traffic_probe(spec):
     runner.sync(fld)
     spl = nat.check(lib.fds_steps_per_launch(runner.ctx))
     fld.simulate(spl * launches)
+    info = nat.tile_table()[tile]
+    if info['lds_ring'] and info['tx'] == 64 and spl in (2, 3) and form in ('fp64', 'fp32'):
+        # the memory-only kernel of the same access pattern, for its own byte counters
+        ms, b = ct.c_double(), ct.c_double()
+        nat.check(lib.fds_stream_pattern_bench(
+            fld.x.samples, fld.y.samples, 8 if form == 'fp64' else 4, spl, spl - 1,
+            strip or info['ty'], 3 if kind == 'acoustic2d' else 1, launches, ct.byref(ms),
+            ct.byref(b)))
     print('traffic-probe done', flush=True)
 
 
@@ -901,6 +920,10 @@ def main():
         floor = pattern_floor(ref['tune'], NX, NY, 8 if main_form == 'fp64' else 4, 3,
                               ref['launch_us'])
         if floor is not None:
+            if traffic is not None and 'pattern' in traffic:
+                floor['hbm_bytes'] = traffic['pattern']['bytes']
+                floor['physical_gbs'] = round(traffic['pattern']['bytes'] /
+                                              floor['us_per_launch'] / 1e3, 1)
             detail['pattern_floor'] = floor
     detail['note'] = ('frac: measured HBM bytes per launch / launch time / 8 TB/s; a launch '
                       'advances steps_per_launch fused steps in one pass, so the per-step '
@@ -1009,6 +1032,10 @@ def main():
                 floor = pattern_floor(r['tune'], n, n, 8 if f == 'fp64' else 4, 1,
                                       r['launch_us'])
                 if floor is not None:
+                    if traffic is not None and 'pattern' in traffic:
+                        floor['hbm_bytes'] = traffic['pattern']['bytes']
+                        floor['physical_gbs'] = round(traffic['pattern']['bytes'] /
+                                                      floor['us_per_launch'] / 1e3, 1)
                     detail['pattern_floor'] = floor
             th[f] = dict(value=round(r['gcells'], 3), ms_per_step=round(r['ms_per_step'], 4),
                          kernel_us=round(r['kernel_us'], 2),
