@@ -187,6 +187,7 @@ template <typename T, int TX, int NQ, int MODE>
 __device__ __forceinline__ void ring_issue(T *dst, const T *const *src, int fs, int n) {
   using RG = RingGeom<T, TX, MODE>;
   const int t = threadIdx.x;
+  __builtin_assume(dst != nullptr);   // an LDS ring: its local address needs no null check
   if constexpr (MODE == kRingX4) {
     if (t < RG::CH) {
       int j = fs + t * RG::E;
@@ -528,17 +529,15 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       d.cvx = rd(sb, SCP + 1, rho, t);
       d.cvy = rd(sb, SCP + 2, rho, t);
     }
-    if (lh || rh) {
-      const int off = lh ? -1 : TX;
-      h.p = rd(sb, SP, rho, off);
-      h.b = rd(cb, CBX, rho, off);
-      if (KAHAN) h.cp = rd(sb, SCP, rho, off);
-    }
-    if (rh) {
-      h.vx = rd(sb, SVX, rho, TX);
-      h.k = rd(cb, CKX, rho, TX);
-      if (KAHAN) h.cvx = rd(sb, SCP + 1, rho, TX);
-    }
+    // ghost columns: every lane reads them (lane 0 column -1, the others column TX; LDS
+    // broadcasts), so the row needs no branch; only the edge lanes use the values
+    const int off = lh ? -1 : TX;
+    h.p = rd(sb, SP, rho, off);
+    h.b = rd(cb, CBX, rho, off);
+    if (KAHAN) h.cp = rd(sb, SCP, rho, off);
+    h.vx = rd(sb, SVX, rho, TX);
+    h.k = rd(cb, CKX, rho, TX);
+    if (KAHAN) h.cvx = rd(sb, SCP + 1, rho, TX);
   };
 
 #pragma unroll
@@ -559,13 +558,21 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
     // RING: slots of this iteration's row ordinal i in the state and coefficient rings (the
     // other rows' slots follow by one wrap)
     const int is0 = RING ? ring_slot(i, 0, RS) : 0, ic0 = RING ? ring_slot(i, 0, RC) : 0;
+    if constexpr (RING) {
+      // stage 0's row PF ahead, first thing in the iteration
+      const StageRows R0(G, H, 0);
+      if (ST || i - G - 1 + PF <= R0.hi + 1) {
+        compiler_fence();
+        issue_row(i - G - 1 + PF, wrap_slot(is0, PF, RS), wrap_slot(ic0, PF, RC));
+      }
+    }
     T bpx[S];
     AcRow<T> row0 = {};      // RING: stage 0 input row of this iteration, read from the rings
     AcHalo<T> h0 = {};
     AcRow<T> &in0 = RING ? row0 : nin[RING ? 0 : W];
     AcHalo<T> &hin = RING ? h0 : nh[RING ? 0 : W];
     // ---- before the barrier
-    static_for<S>([&](auto Sc) __attribute__((always_inline)) {
+    auto pre = [&](auto Sc) __attribute__((always_inline)) {
       constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
       const StageRows R(G, H, s);
@@ -575,8 +582,6 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       if constexpr (RING) {
         if (s == 0) {
           compiler_fence();
-          if (ST || rho + PF <= R.hi + 1)
-            issue_row(rho + PF, wrap_slot(is0, PF, RS), wrap_slot(ic0, PF, RC));
           // loads issued after this row's: the rows rho + 1 .. min(rho + PF, hi + 1)
           if constexpr (ST) vm_wait<PF * NA>(); else ring_wait<NA, PF>(R.hi + 1 - rho);
           compiler_fence();
@@ -619,10 +624,9 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
         pp[Q] = p;
         cpp[Q] = cp;
       }
-    });
-    if (!WAVE) __syncthreads();
-    // ---- after the barrier, consumers first
-    static_for<S, true>([&](auto Sc) __attribute__((always_inline)) {
+    };
+    // ---- after the barrier (one wave: no barrier), consumers first
+    auto post = [&](auto Sc) __attribute__((always_inline)) {
       constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
       const StageRows R(G, H, s);
@@ -737,7 +741,22 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
         cpn[s][Q] = cpnew;
         vh[s][Q] = AcVel<T>{vxn, cvxn, vyn, cvyn};
       }
-    });
+    };
+    if constexpr (WAVE && RING && !EV) {
+      // one wave and no barrier: stages S-1 .. 1 (both parts, consumers still first) run
+      // while stage 0's row is in flight, and stage 0 waits for it last
+      static_for<S - 1, true>([&](auto Sc) __attribute__((always_inline)) {
+        constexpr int s = decltype(Sc)::value + 1;
+        pre(std::integral_constant<int, s>{});
+        post(std::integral_constant<int, s>{});
+      });
+      pre(std::integral_constant<int, 0>{});
+      post(std::integral_constant<int, 0>{});
+    } else {
+      static_for<S>(pre);
+      if (!WAVE) __syncthreads();
+      static_for<S, true>(post);
+    }
   });
 }
 
@@ -877,13 +896,13 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
     d.gx = rd(cb, CGX, rho, t);
     d.gy = ISO ? d.gx : rd(cb, CGY, rho, t);
     if (KAHAN) d.ct = rd(sb, SCT, rho, t);
-    if (lh || rh) {
-      const int off = lh ? -1 : TX;
-      h.t = rd(sb, ST, rho, off);
-      h.q = rd(cb, CQX, rho, off);
-      if (KAHAN) h.ct = rd(sb, SCT, rho, off);
-    }
-    if (rh) h.g = rd(cb, CGX, rho, TX);
+    // ghost columns: every lane reads them (lane 0 column -1, the others column TX; LDS
+    // broadcasts), so the row needs no branch; only the edge lanes use the values
+    const int off = lh ? -1 : TX;
+    h.t = rd(sb, ST, rho, off);
+    h.q = rd(cb, CQX, rho, off);
+    if (KAHAN) h.ct = rd(sb, SCT, rho, off);
+    h.g = rd(cb, CGX, rho, TX);
   };
 
 #pragma unroll
@@ -904,12 +923,20 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
     // RING: slots of this iteration's row ordinal i in the state and coefficient rings (the
     // other rows' slots follow by one wrap)
     const int is0 = RING ? ring_slot(i, 0, RS) : 0, ic0 = RING ? ring_slot(i, 0, RC) : 0;
+    if constexpr (RING) {
+      // stage 0's row PF ahead, first thing in the iteration
+      const StageRows R0(G, H, 0);
+      if (ST || i - G - 1 + PF <= R0.hi + 1) {
+        compiler_fence();
+        issue_row(i - G - 1 + PF, wrap_slot(is0, PF, RS), wrap_slot(ic0, PF, RC));
+      }
+    }
     T qtx[S];
     ThRow<T> row0 = {};      // RING: stage 0 input row of this iteration, read from the rings
     ThHalo<T> h0 = {};
     ThRow<T> &in0 = RING ? row0 : nin[RING ? 0 : W];
     ThHalo<T> &hin = RING ? h0 : nh[RING ? 0 : W];
-    static_for<S>([&](auto Sc) __attribute__((always_inline)) {
+    auto pre = [&](auto Sc) __attribute__((always_inline)) {
       constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
       const StageRows R(G, H, s);
@@ -919,8 +946,6 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
       if constexpr (RING) {
         if (s == 0) {
           compiler_fence();
-          if (ST || rho + PF <= R.hi + 1)
-            issue_row(rho + PF, wrap_slot(is0, PF, RS), wrap_slot(ic0, PF, RC));
           if constexpr (ST) vm_wait<PF * NA>(); else ring_wait<NA, PF>(R.hi + 1 - rho);
           compiler_fence();
           read_row(rho, is0, ic0, row0, h0);
@@ -962,9 +987,8 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
         tp[Q] = tv;
         ctp[Q] = ct;
       }
-    });
-    if (!WAVE) __syncthreads();
-    static_for<S, true>([&](auto Sc) __attribute__((always_inline)) {
+    };
+    auto post = [&](auto Sc) __attribute__((always_inline)) {
       constexpr int s = decltype(Sc)::value;
       const int rho = i - G - 1 - 2 * s;
       const StageRows R(G, H, s);
@@ -1059,7 +1083,22 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
         tn[s][Q] = tnew;
         ctn[s][Q] = ctnew;
       }
-    });
+    };
+    if constexpr (WAVE && RING && !EV) {
+      // one wave and no barrier: stages S-1 .. 1 (both parts, consumers still first) run
+      // while stage 0's row is in flight, and stage 0 waits for it last
+      static_for<S - 1, true>([&](auto Sc) __attribute__((always_inline)) {
+        constexpr int s = decltype(Sc)::value + 1;
+        pre(std::integral_constant<int, s>{});
+        post(std::integral_constant<int, s>{});
+      });
+      pre(std::integral_constant<int, 0>{});
+      post(std::integral_constant<int, 0>{});
+    } else {
+      static_for<S>(pre);
+      if (!WAVE) __syncthreads();
+      static_for<S, true>(post);
+    }
   });
 }
 
