@@ -450,17 +450,15 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       d.cvx = ldn<NT, CHK>(a.cin[1], rb, t, n);
       d.cvy = ldn<NT, CHK>(a.cin[2], rb, t, n);
     }
-    if (lh || rh) {   // one instruction for both sides when they share a wave
-      const int off = lh ? t - 1 : t + 1;
-      h.p = ldr<CHK>(pin, rb, off, n);
-      h.b = ldr<CHK>(Bx, rb, off, n);
-      if (KAHAN) h.cp = ldr<CHK>(a.cin[0], rb, off, n);
-    }
-    if (rh) {
-      h.vx = ldr<CHK>(vxin, rb, t + 1, n);
-      h.k = ldr<CHK>(Kx, rb, t + 1, n);
-      if (KAHAN) h.cvx = ldr<CHK>(a.cin[1], rb, t + 1, n);
-    }
+    // ghost columns: every lane loads them (thread 0 column -1, the others column TX, one
+    // cache line), so the row needs no branch; only the edge threads use the values
+    const int off = lh ? -1 : TX;
+    h.p = ldr<CHK>(pin, rb, off, n);
+    h.b = ldr<CHK>(Bx, rb, off, n);
+    if (KAHAN) h.cp = ldr<CHK>(a.cin[0], rb, off, n);
+    h.vx = ldr<CHK>(vxin, rb, TX, n);
+    h.k = ldr<CHK>(Kx, rb, TX, n);
+    if (KAHAN) h.cvx = ldr<CHK>(a.cin[1], rb, TX, n);
   };
   auto load_coef = [&](int rho, AcCoef<T> &d) __attribute__((always_inline)) {
     const int rb = (y0 + rho) * nx + x0;
@@ -839,13 +837,13 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
     d.gx = ldr<CHK>(Gx, rb, t, n);
     d.gy = ISO ? d.gx : ldr<CHK>(Gy, rb, t, n);
     if (KAHAN) d.ct = ldn<NT, CHK>(a.cin[0], rb, t, n);
-    if (lh || rh) {   // one instruction for both sides when they share a wave
-      const int off = lh ? t - 1 : t + 1;
-      h.t = ldr<CHK>(tin, rb, off, n);
-      h.q = ldr<CHK>(Qx, rb, off, n);
-      if (KAHAN) h.ct = ldr<CHK>(a.cin[0], rb, off, n);
-    }
-    if (rh) h.g = ldr<CHK>(Gx, rb, t + 1, n);
+    // ghost columns: every lane loads them (thread 0 column -1, the others column TX, one
+    // cache line), so the row needs no branch; only the edge threads use the values
+    const int off = lh ? -1 : TX;
+    h.t = ldr<CHK>(tin, rb, off, n);
+    h.q = ldr<CHK>(Qx, rb, off, n);
+    if (KAHAN) h.ct = ldr<CHK>(a.cin[0], rb, off, n);
+    h.g = ldr<CHK>(Gx, rb, TX, n);
   };
   auto load_coef = [&](int rho, ThCoef<T> &d) __attribute__((always_inline)) {
     const int rb = (y0 + rho) * nx + x0;

@@ -135,11 +135,11 @@ __device__ __forceinline__ void lossy_strip(const StepArgs<T> &a, int tile, int6
       d.cvx = ldn<NT, CHK>(a.cin[1], rb, t, n);
       d.cvy = ldn<NT, CHK>(a.cin[2], rb, t, n);
     }
-    if (rh) {
-      h.vx = ldr<CHK>(vxin, rb, t + 1, n);
-      h.mx = ldr<CHK>(Mx, rb, t + 1, n);
-      if (AXI) h.mc = ldr<CHK>(Mc, rb, t + 1, n);
-    }
+    // column TX: every lane loads it (one cache line), so the row needs no branch; only the
+    // last thread uses the values
+    h.vx = ldr<CHK>(vxin, rb, TX, n);
+    h.mx = ldr<CHK>(Mx, rb, TX, n);
+    if (AXI) h.mc = ldr<CHK>(Mc, rb, TX, n);
   };
 
   // ---- LDS rings (RING)
@@ -220,11 +220,10 @@ __device__ __forceinline__ void lossy_strip(const StepArgs<T> &a, int tile, int6
       d.cvx = rd(rb, QCP + 1, r, t);
       d.cvy = rd(rb, QCP + 2, r, t);
     }
-    if (rh) {
-      h.vx = rd(nb, NVX, r, TX);
-      h.mx = ISO ? rd(nb, NMY, r, TX) : rd(rb, QMX, r, TX);
-      if (AXI) h.mc = rd(rb, QMC, r, TX);
-    }
+    // column TX: every lane reads it (an LDS broadcast); only the last thread uses it
+    h.vx = rd(nb, NVX, r, TX);
+    h.mx = ISO ? rd(nb, NMY, r, TX) : rd(rb, QMX, r, TX);
+    if (AXI) h.mc = rd(rb, QMC, r, TX);
   };
 
   if constexpr (RING != 0) {
