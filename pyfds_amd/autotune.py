@@ -74,9 +74,10 @@ def _restore(fields, snap):
 
 
 # strip heights also tried for the 16-row and the 32-row streaming shapes (the others: their own
-# height); measured on 2048^2 acoustics, the time per step depends on the height irregularly
-# (power-of-two heights did worst), so the tuner times each
-STRIP_HEIGHTS = {16: (6, 12, 20, 24), 32: (12, 20, 24)}
+# height); the time per step depends on the height irregularly (on 2048^2 acoustics the
+# register-path shapes did worst at powers of two, the LDS-ring shapes best at 8 or 12 rows;
+# on 4096^2 thermal the ring shapes at 16), so the tuner times each
+STRIP_HEIGHTS = {16: (6, 12, 20, 24), 32: (8, 12, 16, 20, 24)}
 
 
 def tune_field(fld, steps=40, chunks=(50,), use_cache=True):

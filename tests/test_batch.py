@@ -89,7 +89,8 @@ def test_fp64_stream_batch_matches_oracle(slot):
             np.testing.assert_array_equal(got[k], ref[k], err_msg='%s %s' % (scene['name'], k))
 
 
-@pytest.mark.parametrize('slot,strip', [(9, 24), (8, 20), (9, 12), (5, 6), (14, 12), (14, 20)])
+@pytest.mark.parametrize('slot,strip', [(9, 24), (8, 20), (9, 12), (5, 6), (14, 12), (14, 20),
+                                        (14, 8), (14, 16)])
 def test_fp64_stream_batch_strip_heights_match_oracle(slot, strip):
     """Batches on strip heights other than the shape's own (the autotuner's candidates)."""
     members = []

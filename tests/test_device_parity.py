@@ -136,7 +136,8 @@ def test_fp32_stream_strip_heights_equal_tile_engine(form, slot, strip):
 
 
 @pytest.mark.parametrize('name,slot,strip', [('t2d_blocks', 9, 20), ('t2d_aniso', 6, 24),
-                                             ('taxi', 9, 12)])
+                                             ('taxi', 9, 12), ('t2d_blocks', 14, 16),
+                                             ('taxi', 12, 8)])
 def test_kahan_thermal_stream_strip_heights_equal_tile_engine(name, slot, strip):
     scene = scenes.SCENES[name]()
     a = run_device(scene, form='fp32_kahan', tile=1)
@@ -301,7 +302,11 @@ def _big_acoustic(nx, ny, steps, form='fp64', seed=1, model='Acoustic2D', mu=0.0
                                                     ('Acoustic2D', 334, 101, 12, 24),
                                                     ('Acoustic3DAxi', 200, 97, 12, 20),
                                                     ('Acoustic2D', 150, 71, 13, 12),
-                                                    ('Acoustic2D', 270, 83, 14, 24)])
+                                                    ('Acoustic2D', 270, 83, 14, 24),
+                                                    ('Acoustic2D', 262, 93, 14, 8),
+                                                    ('Acoustic3DAxi', 200, 97, 14, 16),
+                                                    ('Acoustic2D', 190, 77, 12, 8),
+                                                    ('Acoustic3DAxi', 150, 71, 12, 16)])
 def test_fp64_stream_strip_heights_match_oracle_bitwise(model, nx, ny, slot, strip):
     idx = _stream_indices()[slot]
     scene = _big_acoustic(nx, ny, 120, model=model)
@@ -328,7 +333,8 @@ def test_fp64_multi_tile_grids_match_oracle_bitwise(model, mu, nx, ny):
 @pytest.mark.parametrize('tile', [None, 'stream0', 'stream2', 'stream3', 'stream6', 'stream8',
                                   'stream9', 'stream11', 'stream5/12', 'stream9/24',
                                   'stream3/20', 'stream6/24', 'stream9/20', 'stream12',
-                                  'stream13/12', 'stream14/20', 'stream12/6'])
+                                  'stream13/12', 'stream14/20', 'stream12/6', 'stream14/8',
+                                  'stream12/16', 'stream14/16'])
 @pytest.mark.parametrize('model', ['Thermal2D', 'Thermal3DAxi'])
 def test_fp64_thermal_multi_tile_matches_oracle_bitwise(model, tile):
     nx, ny, dx = 190, 150, 1e-3
@@ -410,7 +416,7 @@ def test_fp32_ring_shapes_equal_tile_engine(model, nx, ny, form):
     scene = dict(_big_acoustic(nx, ny, 120, model=model),
                  plan=[('simulate', 61), ('simulate', 59)])
     a = run_device(scene, form=form, tile=1)
-    for slot, strip in ((12, 0), (13, 0), (14, 0), (12, 12), (14, 20)):
+    for slot, strip in ((12, 0), (13, 0), (14, 0), (12, 12), (14, 20), (14, 8), (12, 16)):
         print('ring', model, nx, ny, form, slot, strip, flush=True)
         b = run_device(scene, form=form, tile=_stream_indices()[slot], strip_height=strip)
         for k in a:
