@@ -54,18 +54,55 @@ def _run(scene, form='fp64', tile=None, strip=0):
     return res
 
 
+def lossy_scene(steps, seed=4):
+    """The benchmark's lossy configuration (every material viscous, as bench.make_lossy sets
+    it) on the headline grid, from random fields so that every cell's viscous terms count."""
+    scene = headline_scene(steps)
+    visc = dict(shear_viscosity=1e-3, bulk_viscosity=2.4e-3)
+    scene['name'] = 'headline_lossy'
+    scene['material'] = ('AcousticMaterial', dict(WATER, **visc))
+    scene['regions'] = [(idx, ('AcousticMaterial', dict(STEEL, **visc)))
+                        for idx, _ in scene['regions']]
+    rng = np.random.default_rng(seed)
+    n = 2048 * 2048
+    scene['init'] = {'pressure': rng.standard_normal(n),
+                     'velocity_x': 1e-3 * rng.standard_normal(n),
+                     'velocity_y': 1e-3 * rng.standard_normal(n)}
+    return scene
+
+
+@pytest.fixture(scope='module')
+def lossy_oracle():
+    scene = dict(lossy_scene(9), plan=[('simulate', 5), ('simulate', 4)])
+    return scene, scenes.run_oracle(scene)
+
+
+@pytest.mark.parametrize('engine', ['stream64x12', 'ring64x12', 'tile'])
+def test_acoustic2d_2048_lossy_matches_oracle_bitwise(lossy_oracle, engine):
+    """Lossy Acoustic2D at the benchmark's size: the register and LDS-ring streaming shapes
+    the autotuner picks from, and the LDS-tile engine, against the oracle bit for bit."""
+    scene, ref = lossy_oracle
+    tile, strip = {'stream64x12': (_stream(7), 12), 'ring64x12': (_stream(14), 12),
+                   'tile': (1, 0)}[engine]
+    print('lossy 2048^2', engine, flush=True)
+    got = _run(scene, tile=tile, strip=strip)
+    for k in ref:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg='%s %s' % (engine, k))
+
+
 @pytest.fixture(scope='module')
 def headline_oracle():
     scene = dict(headline_scene(24), plan=[('simulate', 13), ('simulate', 11)])
     return scene, scenes.run_oracle(scene)
 
 
-@pytest.mark.parametrize('engine', ['ring64x12', 'stream64x24', 'tile64x8'])
+@pytest.mark.parametrize('engine', ['ring64x12', 'ring64x8', 'stream64x24', 'tile64x8'])
 def test_acoustic2d_2048_matches_oracle_bitwise(headline_oracle, engine):
     """24 steps in two runs of 13 and 11 (fused launches and their remainders) on the
     LDS-ring shape, the register-path three-step shape and the LDS-tile engine."""
     scene, ref = headline_oracle
-    tile, strip = {'ring64x12': (_stream(14), 12), 'stream64x24': (_stream(9), 24),
+    tile, strip = {'ring64x12': (_stream(14), 12), 'ring64x8': (_stream(14), 8),
+                   'stream64x24': (_stream(9), 24),
                    'tile64x8': (1, 0)}[engine]
     print('headline 2048^2', engine, flush=True)
     got = _run(scene, tile=tile, strip=strip)
@@ -96,11 +133,12 @@ def thermal_scene(steps, n=4096, seed=3):
         plan=[('simulate', 5), ('simulate', 4)])
 
 
-@pytest.mark.parametrize('engine', ['ring64x20', 'stream256x24nt'])
+@pytest.mark.parametrize('engine', ['ring64x20', 'ring64x16pf2', 'stream256x24nt'])
 def test_thermal2d_4096_matches_oracle_bitwise(engine):
     scene = thermal_scene(9)
     ref = scenes.run_oracle(scene)
-    tile, strip = {'ring64x20': (_stream(14), 20), 'stream256x24nt': (_stream(6), 24)}[engine]
+    tile, strip = {'ring64x20': (_stream(14), 20), 'ring64x16pf2': (_stream(12), 16),
+                   'stream256x24nt': (_stream(6), 24)}[engine]
     print('thermal 4096^2', engine, flush=True)
     got = _run(scene, tile=tile, strip=strip)
     for k in ref:
