@@ -37,6 +37,7 @@ __global__ __launch_bounds__(64) void k_stream_pattern(PatternArgs<T, NS, NC> a)
   const int lead = ((x0 - 1) % RG::E + RG::E) % RG::E;
   const bool owned = t >= G && t < 64 - G && x0 + t < nx;
   T *const cring = ring + RS * NS * RG::CW;
+  const lds_ptr<T> lring = ring_lds(&ring[0]), lcring = lring + RS * NS * RG::CW;
   const T *ssrc[NS], *csrc[NC];
 #pragma unroll
   for (int q = 0; q < NS; ++q) ssrc[q] = a.src[q];
@@ -45,8 +46,8 @@ __global__ __launch_bounds__(64) void k_stream_pattern(PatternArgs<T, NS, NC> a)
   // row rho (ordinal r = rho + G + 1) into state slot r % RS and coefficient slot r % RC
   auto issue = [&](int rho, int r) __attribute__((always_inline)) {
     const int fs = (y0 + rho) * nx + x0 - 1 - lead;
-    ring_issue<T, 64, NS, kRingX4>(ring + (r % RS) * NS * RG::CW, ssrc, fs, n);
-    ring_issue<T, 64, NC, kRingX4>(cring + (r % RC) * NC * RG::CW, csrc, fs, n);
+    ring_issue<T, 64, NS, kRingX4>(lring + (r % RS) * NS * RG::CW, ssrc, fs, n);
+    ring_issue<T, 64, NC, kRingX4>(lcring + (r % RC) * NC * RG::CW, csrc, fs, n);
   };
   const int rows = H + 2 * G + 2;
   issue(-G - 1, 0);

@@ -183,20 +183,28 @@ __device__ __forceinline__ void ring_wait(int rows) {
 // Ring loads of one row of NQ arrays (window start fs: aligned in kRingX4 mode, column -1 in
 // kRingDword mode) to LDS at `dst` (array q at dst + q * CW); every tile clamps its chunks (or
 // elements) into the array.
+template <typename T>
+using lds_ptr = __attribute__((address_space(3))) T *;
+
+// The LDS address of a ring (converted once per strip: a generic-to-LDS conversion tests for
+// null, so each conversion costs scalar instructions).
+template <typename T>
+__device__ __forceinline__ lds_ptr<T> ring_lds(T *ring) {
+  return (lds_ptr<T>)ring;
+}
+
 template <typename T, int TX, int NQ, int MODE>
-__device__ __forceinline__ void ring_issue(T *dst, const T *const *src, int fs, int n) {
+__device__ __forceinline__ void ring_issue(lds_ptr<T> dst, const T *const *src, int fs, int n) {
   using RG = RingGeom<T, TX, MODE>;
   const int t = threadIdx.x;
-  __builtin_assume(dst != nullptr);   // an LDS ring: its local address needs no null check
   if constexpr (MODE == kRingX4) {
     if (t < RG::CH) {
       int j = fs + t * RG::E;
       if (j < 0 || j >= n) j = 0;
 #pragma unroll
       for (int q = 0; q < NQ; ++q)
-        __builtin_amdgcn_global_load_lds(
-            (const void *)(src[q] + j),
-            (__attribute__((address_space(3))) void *)(dst + q * RG::CW), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void *)(src[q] + j),
+                                         (lds_ptr<void>)(dst + q * RG::CW), 16, 0, 0);
     }
   } else {
     constexpr int DPE = (int)sizeof(T) / 4;   // dwords per element
@@ -211,8 +219,7 @@ __device__ __forceinline__ void ring_issue(T *dst, const T *const *src, int fs, 
         for (int q = 0; q < NQ; ++q)
           __builtin_amdgcn_global_load_lds(
               (const void *)(reinterpret_cast<const uint32_t *>(src[q] + j) + half),
-              (__attribute__((address_space(3))) void *)(reinterpret_cast<uint32_t *>(
-                  dst + q * RG::CW) + k * 64), 4, 0, 0);
+              (lds_ptr<void>)((lds_ptr<uint32_t>)(dst + q * RG::CW) + k * 64), 4, 0, 0);
       }
     }
   }
@@ -478,6 +485,7 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
                 CKY = ISO ? 1 : 3;
   const int lead = RMODE == kRingX4 ? ((x0 - 1) % RG::E + RG::E) % RG::E : 0;
   T *const cring = ring + RS * NS * RG::CW;
+  const lds_ptr<T> lring = ring_lds(ring), lcring = lring + RS * NS * RG::CW;
   const T *ssrc[NS], *csrc[NC];
   if constexpr (RING) {
     ssrc[SP] = pin;
@@ -502,8 +510,8 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
   // row rho into state slot ss and coefficient slot cs
   auto issue_row = [&](int rho, int ss, int cs) __attribute__((always_inline)) {
     const int fs = (y0 + rho) * nx + x0 - 1 - lead;
-    ring_issue<T, TX, NS, RMODE>(sbase(ss), ssrc, fs, n);
-    ring_issue<T, TX, NC, RMODE>(cbase(cs), csrc, fs, n);
+    ring_issue<T, TX, NS, RMODE>(lring + ss * NS * RG::CW, ssrc, fs, n);
+    ring_issue<T, TX, NC, RMODE>(lcring + cs * NC * RG::CW, csrc, fs, n);
   };
   // value of ring array q (base: the row's slot) at column c of row rho
   auto rd = [&](const T *base, int q, int rho, int c) __attribute__((always_inline)) -> T {
@@ -545,10 +553,11 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
   }
   const int iters = H + 2 * G + S + 1;
   // RING: slots are computed, so the loop only unrolls by the parity
-  // RING without events: iterations 3S-1 .. H+2G+1-PF run the steady form of the body (ST):
+  // RING without events, G = S-1: iterations 3S-1 .. H+2G+1-PF run the steady form of the body (ST):
   // every stage's row is inside its range and stage 0 fetches the row PF ahead, so the range
   // tests are compiled out
-  run_loop_steady<RING ? 1 : RS, RING && !EV>(iters, 3 * S - 1, H + 2 * G + 1 - PF,
+  run_loop_steady<RING ? 1 : RS, RING && !EV>(iters, G == S - 1 ? 3 * S - 1 : iters,
+                                              H + 2 * G + 1 - PF,
                                               [&](auto Qc, auto Wc, auto STc, int i)
                                                   __attribute__((always_inline)) {
     constexpr int Q = decltype(Qc)::value, W = decltype(Wc)::value;
@@ -720,12 +729,13 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       }
       if (s == S - 1) {
         // last stage: write the owned cells
-        if (owned && rho >= 1 && rho - 1 < H && y0 + rho - 1 < a.ny) {
+        // (steady: the last stage's row is 1 .. H-S, so only the grid's edge is tested)
+        if (owned && (ST || (rho >= 1 && rho - 1 < H)) && y0 + rho - 1 < a.ny) {
           const int rb = (y0 + rho - 1) * nx + x0;
           stn<NT>(a.out[0], rb, t, pnew);
           if (KAHAN) stn<NT>(a.cout[0], rb, t, cpnew);
         }
-        if (owned && rho >= 0 && rho < H && y0 + rho < a.ny) {
+        if (owned && (ST || (rho >= 0 && rho < H)) && y0 + rho < a.ny) {
           const int rb = (y0 + rho) * nx + x0;
           stn<NT>(a.out[1], rb, t, vxn);
           stn<NT>(a.out[2], rb, t, vyn);
@@ -861,6 +871,7 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
   constexpr int ST = 0, SCT = 1, CQX = 0, CGX = 1, CQY = ISO ? 0 : 2, CGY = ISO ? 1 : 3;
   const int lead = RMODE == kRingX4 ? ((x0 - 1) % RG::E + RG::E) % RG::E : 0;
   T *const cring = ring + RS * NS * RG::CW;
+  const lds_ptr<T> lring = ring_lds(ring), lcring = lring + RS * NS * RG::CW;
   const T *ssrc[NS], *csrc[NC];
   if constexpr (RING) {
     ssrc[ST] = tin;
@@ -876,8 +887,8 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
   auto cbase = [&](int cs) __attribute__((always_inline)) { return cring + cs * NC * RG::CW; };
   auto issue_row = [&](int rho, int ss, int cs) __attribute__((always_inline)) {
     const int fs = (y0 + rho) * nx + x0 - 1 - lead;
-    ring_issue<T, TX, NS, RMODE>(sbase(ss), ssrc, fs, n);
-    ring_issue<T, TX, NC, RMODE>(cbase(cs), csrc, fs, n);
+    ring_issue<T, TX, NS, RMODE>(lring + ss * NS * RG::CW, ssrc, fs, n);
+    ring_issue<T, TX, NC, RMODE>(lcring + cs * NC * RG::CW, csrc, fs, n);
   };
   auto rd = [&](const T *base, int q, int rho, int c) __attribute__((always_inline)) -> T {
     const T v = base[q * RG::CW + lead + 1 + c];
@@ -910,10 +921,11 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
   }
   const int iters = H + 2 * G + S + 1;
   // RING: slots are computed, so the loop only unrolls by the parity
-  // RING without events: iterations 3S-1 .. H+2G+1-PF run the steady form of the body (ST):
+  // RING without events, G = S-1: iterations 3S-1 .. H+2G+1-PF run the steady form of the body (ST):
   // every stage's row is inside its range and stage 0 fetches the row PF ahead, so the range
   // tests are compiled out
-  run_loop_steady<RING ? 1 : RS, RING && !EV>(iters, 3 * S - 1, H + 2 * G + 1 - PF,
+  run_loop_steady<RING ? 1 : RS, RING && !EV>(iters, G == S - 1 ? 3 * S - 1 : iters,
+                                              H + 2 * G + 1 - PF,
                                               [&](auto Qc, auto Wc, auto STc, int i)
                                                   __attribute__((always_inline)) {
     constexpr int Q = decltype(Qc)::value, W = decltype(Wc)::value;
@@ -1067,12 +1079,13 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
         if (KAHAN) kahan_sub(tnew, ctnew, d); else tnew = tnew - d;
       }
       if (s == S - 1) {
-        if (owned && rho >= 1 && rho - 1 < H && y0 + rho - 1 < a.ny) {
+        // (steady: the last stage's row is 1 .. H-S, so only the grid's edge is tested)
+        if (owned && (ST || (rho >= 1 && rho - 1 < H)) && y0 + rho - 1 < a.ny) {
           const int rb = (y0 + rho - 1) * nx + x0;
           stn<NT>(a.out[0], rb, t, tnew);
           if (KAHAN) stn<NT>(a.cout[0], rb, t, ctnew);
         }
-        if (store && owned && rho >= 0 && rho < H && y0 + rho < a.ny) {
+        if (store && owned && (ST || (rho >= 0 && rho < H)) && y0 + rho < a.ny) {
           const int rb = (y0 + rho) * nx + x0;
           stn<NT>(a.out[1], rb, t, fx);
           stn<NT>(a.out[2], rb, t, fy);

@@ -152,6 +152,7 @@ __device__ __forceinline__ void lossy_strip(const StepArgs<T> &a, int tile, int6
   constexpr int NVX = 0, NVY = 1, NMY = 2;    // neighbour ring arrays
   const int lead = ((x0 - 1) % RG::E + RG::E) % RG::E;
   T *const rring = ring + RN * 3 * RG::CW;
+  const lds_ptr<T> lring = ring_lds(ring), lrring = lring + RN * 3 * RG::CW;
   const T *nsrc[3], *rsrc[NRW];
   if constexpr (RING != 0) {
     nsrc[NVX] = vxin;
@@ -183,8 +184,8 @@ __device__ __forceinline__ void lossy_strip(const StepArgs<T> &a, int tile, int6
   };
   auto issue = [&](int r) __attribute__((always_inline)) {   // row rho = r - 1
     const int fs = (y0 + r - 1) * nx + x0 - 1 - lead;
-    ring_issue<T, TX, 3, kRingX4>(nbase(r), nsrc, fs, n);
-    ring_issue<T, TX, NRW, kRingX4>(rbase(r), rsrc, fs, n);
+    ring_issue<T, TX, 3, kRingX4>(lring + ring_slot(r, 0, RN) * 3 * RG::CW, nsrc, fs, n);
+    ring_issue<T, TX, NRW, kRingX4>(lrring + ring_slot(r, 0, RR) * NRW * RG::CW, rsrc, fs, n);
   };
   // value of ring array q (base: the row's slot) at column c of row ordinal r
   auto rd = [&](const T *base, int q, int r, int c) __attribute__((always_inline)) -> T {
