@@ -553,13 +553,13 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
   }
   const int iters = H + 2 * G + S + 1;
   // RING: slots are computed, so the loop only unrolls by the parity
-  // RING without events, G = S-1: iterations 3S-1 .. H+2G+1-PF run the steady form of the body (ST):
-  // every stage's row is inside its range and stage 0 fetches the row PF ahead, so the range
-  // tests are compiled out
-  run_loop_steady<RING ? 1 : RS, RING && !EV>(iters, G == S - 1 ? 3 * S - 1 : iters,
-                                              H + 2 * G + 1 - PF,
-                                              [&](auto Qc, auto Wc, auto STc, int i)
-                                                  __attribute__((always_inline)) {
+  // RING without events, G = S-1, a strip wholly inside the grid: iterations 3S-1 .. H+2G+1-PF
+  // run the steady form of the body (ST): every stage's row is inside its range, stage 0
+  // fetches the row PF ahead and the last stage's rows are 1 .. H-S, so the range tests and
+  // the stores' row tests are compiled out
+  run_loop_steady<RING ? 1 : RS, RING && !EV>(
+      iters, G == S - 1 && y0 + H <= a.ny ? 3 * S - 1 : iters, H + 2 * G + 1 - PF,
+      [&](auto Qc, auto Wc, auto STc, int i) __attribute__((always_inline)) {
     constexpr int Q = decltype(Qc)::value, W = decltype(Wc)::value;
     constexpr bool ST = decltype(STc)::value;
     // RING: slots of this iteration's row ordinal i in the state and coefficient rings (the
@@ -729,13 +729,13 @@ __device__ __forceinline__ void acoustic_strip(const StepArgs<T> &a, int tile, i
       }
       if (s == S - 1) {
         // last stage: write the owned cells
-        // (steady: the last stage's row is 1 .. H-S, so only the grid's edge is tested)
-        if (owned && (ST || (rho >= 1 && rho - 1 < H)) && y0 + rho - 1 < a.ny) {
+        // (steady: the last stage's row is 1 .. H-S of a strip inside the grid)
+        if (owned && (ST || (rho >= 1 && rho - 1 < H && y0 + rho - 1 < a.ny))) {
           const int rb = (y0 + rho - 1) * nx + x0;
           stn<NT>(a.out[0], rb, t, pnew);
           if (KAHAN) stn<NT>(a.cout[0], rb, t, cpnew);
         }
-        if (owned && (ST || (rho >= 0 && rho < H)) && y0 + rho < a.ny) {
+        if (owned && (ST || (rho >= 0 && rho < H && y0 + rho < a.ny))) {
           const int rb = (y0 + rho) * nx + x0;
           stn<NT>(a.out[1], rb, t, vxn);
           stn<NT>(a.out[2], rb, t, vyn);
@@ -921,13 +921,13 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
   }
   const int iters = H + 2 * G + S + 1;
   // RING: slots are computed, so the loop only unrolls by the parity
-  // RING without events, G = S-1: iterations 3S-1 .. H+2G+1-PF run the steady form of the body (ST):
-  // every stage's row is inside its range and stage 0 fetches the row PF ahead, so the range
-  // tests are compiled out
-  run_loop_steady<RING ? 1 : RS, RING && !EV>(iters, G == S - 1 ? 3 * S - 1 : iters,
-                                              H + 2 * G + 1 - PF,
-                                              [&](auto Qc, auto Wc, auto STc, int i)
-                                                  __attribute__((always_inline)) {
+  // RING without events, G = S-1, a strip wholly inside the grid: iterations 3S-1 .. H+2G+1-PF
+  // run the steady form of the body (ST): every stage's row is inside its range, stage 0
+  // fetches the row PF ahead and the last stage's rows are 1 .. H-S, so the range tests and
+  // the stores' row tests are compiled out
+  run_loop_steady<RING ? 1 : RS, RING && !EV>(
+      iters, G == S - 1 && y0 + H <= a.ny ? 3 * S - 1 : iters, H + 2 * G + 1 - PF,
+      [&](auto Qc, auto Wc, auto STc, int i) __attribute__((always_inline)) {
     constexpr int Q = decltype(Qc)::value, W = decltype(Wc)::value;
     constexpr bool ST = decltype(STc)::value;
     // RING: slots of this iteration's row ordinal i in the state and coefficient rings (the
@@ -1079,13 +1079,13 @@ __device__ __forceinline__ void thermal_strip(const StepArgs<T> &a, int tile, in
         if (KAHAN) kahan_sub(tnew, ctnew, d); else tnew = tnew - d;
       }
       if (s == S - 1) {
-        // (steady: the last stage's row is 1 .. H-S, so only the grid's edge is tested)
-        if (owned && (ST || (rho >= 1 && rho - 1 < H)) && y0 + rho - 1 < a.ny) {
+        // (steady: the last stage's row is 1 .. H-S of a strip inside the grid)
+        if (owned && (ST || (rho >= 1 && rho - 1 < H && y0 + rho - 1 < a.ny))) {
           const int rb = (y0 + rho - 1) * nx + x0;
           stn<NT>(a.out[0], rb, t, tnew);
           if (KAHAN) stn<NT>(a.cout[0], rb, t, ctnew);
         }
-        if (store && owned && (ST || (rho >= 0 && rho < H)) && y0 + rho < a.ny) {
+        if (store && owned && (ST || (rho >= 0 && rho < H && y0 + rho < a.ny))) {
           const int rb = (y0 + rho) * nx + x0;
           stn<NT>(a.out[1], rb, t, fx);
           stn<NT>(a.out[2], rb, t, fy);
