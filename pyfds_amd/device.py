@@ -159,20 +159,32 @@ def _same_value(v, snap):
     if kind == 's':
         return np.ndim(v) == 0 and float(v) == data
     if kind == 'a':
+        # by bits: equal bits are equal values; a difference only in bits (-0.0, NaN payloads)
+        # costs a recompile, which the program comparison then finds unchanged
         return isinstance(v, np.ndarray) and v.shape == data.shape and v.dtype == data.dtype \
-            and np.array_equal(v, data)
+            and v.tobytes() == data.tobytes()
     if np.ndim(v) == 0 or isinstance(v, np.ndarray) or len(v) != len(data):
         return False
     return all(np.array_equal(np.asarray(x, dtype=np.float64), d) for x, d in zip(v, data))
 
 
+def _int_list_copy(idx):
+    """A copy of ``idx`` when it is a flat Python list of ints (regions hold their indices so),
+    else None.  Comparing such a list with its copy by list equality gives the answer that
+    comparing the two as arrays gives, without rebuilding an array from the list."""
+    if type(idx) is not list or not all(type(x) is int for x in idx):
+        return None
+    return list(idx)
+
+
 def _snapshot(items, value_of=None):
     """What a boundary or output list holds, by identity and by content: the objects, their
-    regions, copies of the index arrays and (boundaries) of the values."""
+    regions, copies of the index arrays (and of flat int lists as lists) and (boundaries) of
+    the values."""
     snap = []
     for it in items:
         idx = it.region.indices
-        entry = (it, it.region, None, np.array(idx, copy=True))
+        entry = (it, it.region, _int_list_copy(idx), np.array(idx, copy=True))
         if value_of is not None:
             entry += (bool(it.additive), it.value, _copy_value(it.value))
         snap.append(entry)
@@ -187,9 +199,14 @@ def _unchanged(items, snap, boundaries):
     for it, entry in zip(items, snap):
         if it is not entry[0] or it.region is not entry[1]:
             return False
-        arr = np.asarray(it.region.indices)
-        if arr.shape != entry[3].shape or not np.array_equal(arr, entry[3]):
-            return False
+        idx = it.region.indices
+        if entry[2] is not None and type(idx) is list:
+            if idx != entry[2]:
+                return False
+        else:
+            arr = np.asarray(idx)
+            if arr.shape != entry[3].shape or not np.array_equal(arr, entry[3]):
+                return False
         if boundaries:
             if bool(it.additive) != entry[4] or not _same_value(it.value, entry[6]):
                 return False

@@ -328,6 +328,58 @@ def test_boundary_programs_follow_reference_semantics(seed):
         np.testing.assert_array_equal(got, want)
 
 
+def test_upload_change_detection_sees_every_edit():
+    """A run re-uploads a boundary program or output list when anything the reference reads
+    from them at run time changed since the last upload (pyfds/fields.py:591-611 reads the
+    region's indices and the boundary's value and additive flag every step): edits made in
+    place to an index list or a signal array count, values equal to the old ones do not."""
+    idx = [5, 6, 7]
+    sig = np.arange(4.0)
+    b = reg.Boundary(reg.LineRegion(idx, [0, 0.2], 'b'), value=sig)
+    o = reg.Output(reg.LineRegion(list(range(64)), [0, 0.2], 'o'))
+    bs, os_ = device._snapshot([b], True), device._snapshot([o])
+    assert device._unchanged([b], bs, True) and device._unchanged([o], os_, False)
+    # an index list edited in place, grown, or holding the same values as numpy ints
+    idx[1] = 9
+    assert not device._unchanged([b], bs, True)
+    idx[1] = 6
+    assert device._unchanged([b], bs, True)
+    idx[1] = np.int64(6)
+    assert device._unchanged([b], bs, True)
+    idx.append(8)
+    assert not device._unchanged([b], bs, True)
+    idx.pop()
+    o.region.indices[63] = 0
+    assert not device._unchanged([o], os_, False)
+    o.region.indices[63] = 63
+    assert device._unchanged([o], os_, False)
+    # indices held as an array instead of the list
+    o.region.indices = np.arange(64)
+    assert device._unchanged([o], os_, False)
+    o.region.indices[0] = 1
+    assert not device._unchanged([o], os_, False)
+    # a signal edited in place, replaced by an equal copy, by one of another dtype or length
+    sig[2] = 7.0
+    assert not device._unchanged([b], bs, True)
+    sig[2] = 2.0
+    assert device._unchanged([b], bs, True)
+    b.value = sig.copy()
+    assert device._unchanged([b], bs, True)
+    b.value = sig.astype(np.float32)
+    assert not device._unchanged([b], bs, True)
+    b.value = sig[:3].copy()
+    assert not device._unchanged([b], bs, True)
+    b.value = sig
+    b.additive = True
+    assert not device._unchanged([b], bs, True)
+    b.additive = False
+    assert device._unchanged([b], bs, True)
+    # another region object, or another list of boundaries
+    b.region = reg.LineRegion([5, 6, 7], [0, 0.2], 'b')
+    assert not device._unchanged([b], bs, True)
+    assert not device._unchanged([], bs, True)
+
+
 def test_boundary_errors():
     with pytest.raises(IndexError):
         device.compile_boundaries([reg.Boundary(reg.Region([5]), 1.0)], 5)
